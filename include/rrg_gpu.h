@@ -1,0 +1,128 @@
+/*
+ * rrg_gpu.h -- C ABI of the MI355X gzip/DEFLATE decoder (librrg_gpu.so).
+ *
+ * Plain pointers and sizes only.  One context per host thread; a context owns
+ * one HIP stream and its device buffers and is not thread-safe (the same
+ * contract as the reference's per-worker native decoders,
+ * crates/rusty-rapidgzip/src/libdeflate_ffi.rs:28-35).
+ *
+ * Reference interfaces replaced (paths relative to crates/rusty-rapidgzip/src/):
+ *   rrg_gpu_read_gz        read_gz(path, Sender<Arc<Vec<u8>>>, Config)   lib.rs:366-496
+ *   rrg_gpu_decode         read_gz over an in-memory (mmapped) file       lib.rs:435-484
+ *   rrg_cfg                Config{num_threads, chunk_size_bytes, verbose} lib.rs:49-100
+ *   rrg_stats              DecodeStats                                    lib.rs:102-108
+ *   rrg_error              Error / GzipError / DeflateError               lib.rs:110-118, gzip.rs:25-61,
+ *                                                                          deflate/mod.rs:22-28
+ *   rrg_chunk_cb           Sender::send of one in-order Arc<Vec<u8>>      pipeline.rs:921-937
+ *   rrg_gpu_create/destroy the opaque per-worker handle pattern of
+ *                          isal_shim.c:191-261 (rrg_isal_alloc/free)
+ *   rrg_gpu_decode_device  no reference counterpart: the device-resident
+ *                          form (input and output in HBM) used to measure
+ *                          the kernels without PCIe transfers
+ */
+#ifndef RRG_GPU_H
+#define RRG_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RRG_GPU_ABI_VERSION 1
+
+/* Error kinds; values match the variants of the reference error enums. */
+enum rrg_error_kind {
+    RRG_OK = 0,
+    RRG_E_EOF = 1,          /* DeflateError::UnexpectedEof */
+    RRG_E_INVALID = 2,      /* DeflateError::Invalid(message) */
+    RRG_E_BADMAGIC = 3,     /* GzipError::BadMagic */
+    RRG_E_METHOD = 4,       /* GzipError::UnsupportedMethod */
+    RRG_E_RESERVED = 5,     /* GzipError::ReservedFlags */
+    RRG_E_TRUNCATED = 6,    /* GzipError::Truncated */
+    RRG_E_CRC = 7,          /* GzipError::CrcMismatch */
+    RRG_E_ISIZE = 8,        /* GzipError::IsizeMismatch */
+    RRG_E_IO = 9,           /* Error::Io (open/mmap failure, empty file, no final block) */
+    RRG_E_DEVICE = 10,      /* HIP runtime failure */
+    RRG_E_ARG = 11,         /* bad argument / capacity too small */
+};
+
+typedef struct {
+    int kind;               /* rrg_error_kind */
+    uint32_t expected;      /* CrcMismatch.expected / IsizeMismatch.expected */
+    uint32_t got;           /* CrcMismatch.got */
+    uint64_t got64;         /* IsizeMismatch.got / CrcMismatch.uncompressed */
+    uint32_t member;        /* 0-based gzip member index */
+    uint64_t trailer_byte;  /* trailer offset in the deflate body (as the reference reports it) */
+    uint32_t method;        /* UnsupportedMethod(u8) / ReservedFlags(u8) */
+    char message[192];
+} rrg_error;
+
+typedef struct {
+    uint64_t compressed_bytes;
+    uint64_t uncompressed_bytes;
+    uint64_t chunks_decoded;
+    uint64_t speculation_failures;
+} rrg_stats;
+
+typedef struct {
+    size_t chunk_size_bytes;  /* Config.chunk_size_bytes, default 4 MiB */
+    int num_threads;          /* Config.num_threads (host threads for framing; 0 = auto) */
+    int verbose;              /* Config.verbose: 0 = Off, 1 = On (stage timings on stderr) */
+    /* GPU decomposition (no reference counterpart; 0 = automatic) */
+    size_t segment_bytes;     /* compressed bytes per lane-decoded segment */
+    int lut_bits;             /* literal/length LUT bits: 8, 9 or 10 */
+    int lanes_per_block;      /* lanes (segments in flight) per workgroup */
+} rrg_cfg;
+
+/* Return nonzero to close the stream (the reference's dropped Receiver). */
+typedef int (*rrg_chunk_cb)(void* user, const uint8_t* data, size_t len);
+
+/* Per-call stage timings of the last decode (HIP events on the context's stream). */
+typedef struct {
+    double ms_total;      /* host wall clock of the whole call */
+    double ms_find;       /* block finder kernel */
+    double ms_inflate;    /* speculative inflate kernel(s), retries included */
+    double ms_tail;       /* tail-chain kernels */
+    double ms_resolve;    /* resolve + narrow kernel */
+    double ms_crc;        /* CRC kernel */
+    uint64_t n_segments;
+    uint64_t n_markers_tail;
+    uint64_t n_inflate_launches;
+    uint64_t n_tail_rounds;
+    uint64_t compressed_bytes;
+    uint64_t uncompressed_bytes;
+} rrg_profile;
+
+int rrg_gpu_abi_version(void);
+void rrg_cfg_default(rrg_cfg* cfg);
+
+/* Context on HIP device `device`; NULL on failure. */
+void* rrg_gpu_create(int device);
+void rrg_gpu_destroy(void* ctx);
+
+/* read_gz: decode the gzip file at `path`, handing each in-order output chunk
+ * to cb.  Returns RRG_OK or an error kind (details in *err). */
+int rrg_gpu_read_gz(void* ctx, const char* path, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user,
+                    rrg_stats* stats, rrg_error* err);
+
+/* Same over a file already in host memory. */
+int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user,
+                   rrg_stats* stats, rrg_error* err);
+
+/* Device-resident form.  d_file: the gzip file in device memory.  h_file:
+ * optional host copy (used only to walk BGZF member headers; NULL = copy the
+ * headers down on demand).  On return *d_out points at the decompressed bytes
+ * in device memory (owned by ctx, valid until the next call), chunk_ends[i] is
+ * the end offset of output chunk i (the reference's Vec boundaries), *n_chunks
+ * their count (at most chunk_cap are written). */
+int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg,
+                          const uint8_t** d_out, uint64_t* out_len, uint64_t* chunk_ends, size_t chunk_cap,
+                          size_t* n_chunks, rrg_stats* stats, rrg_error* err);
+
+void rrg_gpu_last_profile(void* ctx, rrg_profile* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

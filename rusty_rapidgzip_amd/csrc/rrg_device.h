@@ -1,0 +1,95 @@
+// rrg_device.h -- data layout shared by the HIP kernels and the host driver.
+//
+// Everything here is plain-old-data that lives in HBM.  The host driver fills
+// SegDesc, the kernels fill SegResult / MemberRec / cell buffers.
+#pragma once
+#include <stdint.h>
+
+namespace rrg {
+
+// A "segment" is the unit one lane decodes: from start_bit until the first
+// block boundary at or after hint_bit (or until the final member ends).
+// Reference counterpart: WorkItem{start_bit,end_bit_hint} (pipeline.rs:377-386).
+struct SegDesc {
+    uint64_t start_bit;
+    uint64_t hint_bit;
+    uint64_t cells_base;  // first u16 cell of this segment's output region
+    uint64_t cells_cap;   // capacity in cells
+    uint32_t flags;       // SEG_SPEC | SEG_SINGLE_MEMBER
+    uint32_t mb_base;     // first MemberRec slot
+    uint32_t mb_cap;      // MemberRec slots
+    uint32_t pad;
+};
+
+enum : uint32_t {
+    SEG_SPEC = 1u,           // first member may reference bytes before the segment (markers)
+    SEG_SINGLE_MEMBER = 2u,  // stop after one member's trailer (BGZF members)
+};
+
+struct SegResult {
+    uint64_t n_cells;   // cells produced
+    uint64_t end_bit;   // actual_end_bit (pipeline.rs:1176-1182)
+    uint32_t status;    // ST_*
+    uint32_t detail;    // DET_* for ST_INVALID
+    uint32_t final_block;
+    uint32_t n_mb;      // member boundaries recorded
+};
+
+// Member boundary inside a segment (pipeline.rs:417-427 MemberBoundary).
+struct MemberRec {
+    uint64_t out_off;       // cell offset in the segment where the member ended
+    uint64_t trailer_byte;  // byte offset of the trailer in the body
+    uint32_t crc;
+    uint32_t isize;
+};
+
+enum : uint32_t {
+    ST_OK = 0,
+    ST_EOF = 1,          // DeflateError::UnexpectedEof
+    ST_INVALID = 2,      // DeflateError::Invalid
+    ST_BADMAGIC = 3,     // GzipError::BadMagic
+    ST_METHOD = 4,       // GzipError::UnsupportedMethod
+    ST_RESERVED = 5,     // GzipError::ReservedFlags
+    ST_TRUNCATED = 6,    // GzipError::Truncated
+    ST_OVERFLOW = 11,    // cell capacity exceeded (host retries bigger)
+    ST_MB_OVERFLOW = 12, // member-record capacity exceeded (host retries bigger)
+};
+
+enum : uint32_t {
+    DET_NONE = 0,
+    DET_HLIT_HDIST,
+    DET_CODE16_FIRST,
+    DET_CODE16_OVERRUN,
+    DET_CODE17_OVERRUN,
+    DET_CODE18_OVERRUN,
+    DET_BAD_CL_SYMBOL,
+    DET_EOB_ZERO,
+    DET_OVERSUBSCRIBED,
+    DET_NO_CODE,
+    DET_LITLEN_RANGE,
+    DET_DIST_RANGE,
+    DET_DIST_FAR,        // back-reference before the member start
+    DET_STORED_NLEN,
+    DET_RESERVED_BTYPE,
+    DET_MARKER_RANGE,    // marker references bytes before the start of the output
+    DET_COUNT
+};
+
+// Tile of the resolve kernel: a run of one segment's cells written to out.
+struct ResolveTile {
+    uint64_t cell_addr;  // absolute cell index of the tile's first cell
+    uint64_t out_off;    // global output offset of the tile's first byte
+    uint32_t seg;        // chain-order segment index
+    uint32_t n;          // cells in the tile
+};
+
+// Chain-order segment geometry used by the tail / resolve kernels.
+struct ChainSeg {
+    uint64_t cells_base;  // absolute cell index of the segment's first cell
+    uint64_t len;         // cells
+    uint64_t tz_base;     // index of the segment's tail zone in the dense tail arrays
+    uint32_t spec;        // first member may hold markers
+    uint32_t pad;
+};
+
+}  // namespace rrg

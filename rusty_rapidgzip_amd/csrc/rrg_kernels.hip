@@ -1,0 +1,1044 @@
+// rrg_kernels.hip -- CDNA4 (gfx950) kernels for the gzip/DEFLATE decode path.
+//
+// Stages (reference counterparts, paths relative to crates/rusty-rapidgzip/src/):
+//   k_find_blocks   block finder          deflate/block_finder.rs:57-301
+//   k_inflate       speculative decode    pipeline.rs:1080-1185, deflate/fast_inflate.rs:1794-2234
+//   k_tail_*        tail-window chain     pipeline.rs:742-856, 1043-1078
+//   k_resolve       marker resolution     deflate/speculative.rs:168-213
+//   k_crc_pieces    CRC-32 pieces         pipeline.rs:706-733 (crc32fast update)
+//
+// All of it is integer work: bit extraction, LDS table lookups, byte copies and
+// GF(2) arithmetic.  No MFMA.
+#include <hip/hip_runtime.h>
+
+#include "rrg_device.h"
+#include "rrg_launch.h"
+
+namespace rrg {
+
+// ---------------------------------------------------------------------------
+// Bit access.  The compressed body is stored as little-endian u64 words padded
+// with >= 4 zero words, so a 64-bit window at any position up to nbits+128 is
+// in bounds.  Bits at or beyond nbits read as zero; every consumer checks
+// availability against nbits before acting on what it read.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t bits64_at(const uint64_t* __restrict__ in64, uint64_t pos) {
+    const uint64_t q = pos >> 6;
+    const uint32_t s = (uint32_t)(pos & 63);
+    const uint64_t lo = in64[q];
+    const uint64_t hi = in64[q + 1];
+    return s ? (lo >> s) | (hi << (64 - s)) : lo;
+}
+
+__device__ __forceinline__ uint32_t byte_at(const uint64_t* __restrict__ in64, uint64_t i) {
+    return (uint32_t)((in64[i >> 3] >> ((i & 7) * 8)) & 0xffu);
+}
+
+// RFC 1951 code-length order (deflate/tables.rs:4-6)
+__constant__ uint8_t c_cl_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+// Length / distance bases computed instead of looked up (same values as
+// deflate/tables.rs:10-30).
+__device__ __forceinline__ uint32_t len_extra(uint32_t i) { return (i < 8 || i == 28) ? 0u : (i - 4) >> 2; }
+__device__ __forceinline__ uint32_t len_base(uint32_t i) {
+    return i < 8 ? i + 3 : (i == 28 ? 258u : ((4u + (i & 3)) << len_extra(i)) + 3u);
+}
+__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 4 ? 0u : (d >> 1) - 1; }
+__device__ __forceinline__ uint32_t dist_base(uint32_t d) {
+    return d < 4 ? d + 1 : ((2u + (d & 1)) << dist_extra(d)) + 1u;
+}
+
+// ===========================================================================
+// Block finder (block_finder.rs:109-141).
+//
+// The reference walks a chain of positions: at p it peeks 13 bits w, jumps by
+// NEXT_LUT[w] (the first shift at which the zero-padded window passes the
+// BFINAL=0/BTYPE=2/HLIT<=29/HDIST<=29 predicate, 13 if none), and at a jump of
+// 0 verifies the candidate, moving on by one bit if it fails.  Because padded
+// shifts of 11 and 12 never pass, some offsets are never visited; the answer
+// is "the first verified offset ON that chain".  One wavefront per target
+// reproduces it exactly: the 64 lanes evaluate step and verification for 64
+// consecutive offsets in parallel, then the wave follows the chain through
+// that window with readlane hops.
+// ===========================================================================
+__device__ __forceinline__ bool is_cand13(uint32_t b) {
+    return !(b & 1u) && ((b >> 1) & 3u) == 2u && ((b >> 3) & 31u) <= 29u && ((b >> 8) & 31u) <= 29u;
+}
+
+// Strict verification (block_finder.rs:175-301) of a candidate at `p`.
+// `lens` is this lane's 340-byte LDS scratch, `cnt` 32 u16.
+struct VerifyBits {
+    const uint64_t* in64;
+    uint64_t nbits;
+    uint64_t pos;
+    __device__ __forceinline__ bool read(uint32_t n, uint32_t* v) {
+        if (pos + n > nbits) return false;
+        *v = n ? (uint32_t)(bits64_at(in64, pos) & ((1ull << n) - 1)) : 0u;
+        pos += n;
+        return true;
+    }
+};
+
+// canonical decode (RFC 1951 3.2.2) over lengths lens[0..n): returns symbol or
+// -1.  Follows the reference's lookup rules: 10 bits must be available for the
+// LUT probe, 15 for codes longer than 10.
+__device__ int verify_decode(VerifyBits& b, const uint8_t* lens, int n, const uint16_t* cnt) {
+    if (b.pos + 10 > b.nbits) return -1;
+    const uint64_t w = bits64_at(b.in64, b.pos);
+    int code = 0, first = 0, index = 0;
+    for (int l = 1; l <= 15; ++l) {
+        code |= (int)((w >> (l - 1)) & 1u);
+        const int count = cnt[l];
+        if (code - count < first) {
+            if (l > 10 && b.pos + 15 > b.nbits) return -1;
+            // symbol = the (code-first)-th symbol of length l in symbol order
+            int want = code - first;  // rank within length l
+            for (int s = 0; s < n; ++s) {
+                if (lens[s] == l) {
+                    if (want == 0) {
+                        b.pos += (uint64_t)l;
+                        return s;
+                    }
+                    --want;
+                }
+            }
+            return -1;
+        }
+        index += count;
+        first += count;
+        first <<= 1;
+        code <<= 1;
+    }
+    return -1;
+}
+
+// Kraft check over lens[0..n) at 15 bits; strict: complete when >= 2 symbols.
+__device__ bool verify_kraft(const uint8_t* lens, int n, uint16_t* cnt) {
+    for (int l = 0; l < 16; ++l) cnt[l] = 0;
+    for (int s = 0; s < n; ++s) cnt[lens[s]]++;
+    cnt[0] = 0;
+    uint32_t total = 0, nsym = 0;
+    for (int l = 1; l <= 15; ++l) {
+        total += (uint32_t)cnt[l] << (15 - l);
+        nsym += cnt[l];
+    }
+    if (nsym >= 2 && total != 32768u) return false;
+    return true;
+}
+
+__device__ bool verify_candidate(const uint64_t* in64, uint64_t nbits, uint64_t p, uint8_t* lens, uint16_t* cnt) {
+    VerifyBits b{in64, nbits, p};
+    uint32_t v;
+    if (!b.read(3, &v)) return false;
+    if (!b.read(5, &v)) return false;
+    const int hlit = (int)v + 257;
+    if (!b.read(5, &v)) return false;
+    const int hdist = (int)v + 1;
+    if (!b.read(4, &v)) return false;
+    const int hclen = (int)v + 4;
+    if (hlit > 286 || hdist > 30) return false;
+    // precode lengths live in lens[320..339) while the litlen/dist lengths fill lens[0..316)
+    uint8_t* cl = lens + 320;
+    for (int i = 0; i < 19; ++i) cl[i] = 0;
+    for (int i = 0; i < hclen; ++i) {
+        if (!b.read(3, &v)) return false;
+        cl[c_cl_order[i]] = (uint8_t)v;
+    }
+    if (!verify_kraft(cl, 19, cnt)) return false;
+    const int total = hlit + hdist;
+    int i = 0;
+    while (i < total) {
+        const int sym = verify_decode(b, cl, 19, cnt);
+        if (sym < 0) return false;
+        if (sym <= 15) {
+            lens[i++] = (uint8_t)sym;
+        } else if (sym == 16) {
+            if (i == 0) return false;
+            if (!b.read(2, &v)) return false;
+            const int n = (int)v + 3;
+            if (i + n > total) return false;
+            const uint8_t prev = lens[i - 1];
+            for (int j = 0; j < n; ++j) lens[i + j] = prev;
+            i += n;
+        } else if (sym == 17 || sym == 18) {
+            if (!b.read(sym == 17 ? 3 : 7, &v)) return false;
+            const int n = (int)v + (sym == 17 ? 3 : 11);
+            if (i + n > total) return false;
+            for (int j = 0; j < n; ++j) lens[i + j] = 0;
+            i += n;
+        } else {
+            return false;
+        }
+    }
+    if (lens[256] == 0) return false;
+    // lens[0..hlit) literal/length, lens[hlit..total) distance; both strict.
+    uint16_t* cntL = cnt + 16;
+    if (!verify_kraft(lens, hlit, cntL)) return false;
+    if (!verify_kraft(lens + hlit, hdist, cnt)) return false;
+    // trial decode of 64 symbols (block_finder.rs:268-301)
+    for (int k = 0; k < 64; ++k) {
+        const int sym = verify_decode(b, lens, hlit, cntL);
+        if (sym < 0) return false;
+        if (sym <= 255) continue;
+        if (sym == 256) return true;
+        if (sym > 285) return false;
+        if (!b.read(len_extra((uint32_t)sym - 257), &v)) return false;
+        const int ds = verify_decode(b, lens + hlit, hdist, cnt);
+        if (ds < 0 || ds >= 30) return false;
+        if (!b.read(dist_extra((uint32_t)ds), &v)) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t nbits,
+                                                    const uint64_t* __restrict__ targets,
+                                                    uint64_t* __restrict__ found, uint32_t n_targets) {
+    __shared__ uint8_t s_next[8192];
+    __shared__ uint8_t s_lens[64 * 340];
+    __shared__ uint16_t s_cnt[64 * 32];
+    const int lane = (int)threadIdx.x;
+    for (int i = lane; i < 8192; i += 64) {
+        uint32_t w = (uint32_t)i, step = 0;
+        while (step < 13 && !is_cand13(w)) {
+            w >>= 1;
+            ++step;
+        }
+        s_next[i] = (uint8_t)step;
+    }
+    __syncthreads();
+    const uint32_t t = blockIdx.x;
+    if (t >= n_targets) return;
+    uint64_t cur = targets[t];
+    const uint64_t limit = nbits;
+    uint64_t result = ~0ull;
+    uint8_t* lens = s_lens + lane * 340;
+    uint16_t* cnt = s_cnt + lane * 32;
+    uint64_t base = cur;
+    bool done = (cur + 13 > limit);
+    while (!done) {
+        const uint64_t p = base + (uint64_t)lane;
+        const bool inb = p + 13 <= limit;
+        uint32_t step = 255;
+        bool ver = false;
+        if (inb) {
+            const uint32_t w = (uint32_t)(bits64_at(in64, p) & 0x1fffu);
+            step = s_next[w];
+            if (step == 0) ver = verify_candidate(in64, nbits, p, lens, cnt);
+        }
+        const uint64_t vm = __ballot(ver);
+        const int nx = (step == 0) ? 1 : (int)step;
+        // follow the chain through this 64-offset window (uniform control)
+        while (true) {
+            const uint64_t rel = cur - base;
+            if (rel >= 64) break;
+            if (cur + 13 > limit) {
+                done = true;
+                break;
+            }
+            if ((vm >> rel) & 1ull) {
+                result = cur;
+                done = true;
+                break;
+            }
+            cur += (uint64_t)__builtin_amdgcn_readlane(nx, (int)rel);
+        }
+        base += 64;
+    }
+    if (lane == 0) found[t] = result;
+}
+
+hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t nbits, const uint64_t* d_targets, uint64_t* d_found,
+                              uint32_t n_targets, hipStream_t s) {
+    if (n_targets == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_find_blocks, dim3(n_targets), dim3(64), 0, s, d_in64, nbits, d_targets, d_found,
+                       n_targets);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Speculative inflate: one lane decodes one segment.  Persistent grid; lanes
+// pull segment indices from a device work queue as they finish.
+//
+// Per-lane LDS: literal/length LUT (2^LB u16), distance LUT (128 u16, also the
+// 7-bit precode LUT while a header is read), code lengths (320 B), counts and
+// canonical-code scratch.  LUT entry = (symbol << 4) | code length; a zero
+// entry means a code longer than the LUT (or an unassigned code) and goes to
+// the canonical slow path, which reads the length-sorted symbols that the
+// builder stored in this lane's global scratch.
+//
+// Cells (u16) go straight to HBM.  A back-reference that reaches before the
+// segment's first byte (first member only) emits marker cells
+// 0x8000 | prefix_offset (fast_inflate.rs:2209-2222); later members must stay
+// inside themselves (libdeflate_ffi.rs:64-123).
+// ===========================================================================
+constexpr int DIST_LB = 7;
+
+template <int LB>
+struct LaneTables {
+    static constexpr int kLit = 1 << LB;
+    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) * 2 + 320 + 5 * 32;
+};
+
+int inflate_lds_per_lane(int lut_bits) {
+    switch (lut_bits) {
+        case 8: return LaneTables<8>::kBytes;
+        case 10: return LaneTables<10>::kBytes;
+        default: return LaneTables<9>::kBytes;
+    }
+}
+
+// Canonical table build (huffman.rs:215-325 semantics, incomplete accepted).
+// Returns false when oversubscribed with >= 2 symbols.
+__device__ bool build_table(const uint8_t* __restrict__ lens, int n, uint16_t* __restrict__ lut, int lbits,
+                            uint16_t* __restrict__ cnt, uint16_t* __restrict__ tnext, uint16_t* __restrict__ toffs,
+                            uint16_t* __restrict__ sorted) {
+    for (int l = 0; l < 16; ++l) cnt[l] = 0;
+    for (int s = 0; s < n; ++s) cnt[lens[s]]++;
+    cnt[0] = 0;
+    uint32_t total = 0, nsym = 0;
+    for (int l = 1; l <= 15; ++l) {
+        total += (uint32_t)cnt[l] << (15 - l);
+        nsym += cnt[l];
+    }
+    if (nsym >= 2 && total > 32768u) return false;
+    uint32_t code = 0, off = 0;
+    for (int l = 1; l <= 15; ++l) {
+        code = (code + cnt[l - 1]) << 1;
+        tnext[l] = (uint16_t)code;
+        toffs[l] = (uint16_t)off;
+        off += cnt[l];
+    }
+    uint32_t* lut32 = reinterpret_cast<uint32_t*>(lut);
+    for (int i = 0; i < (1 << lbits) / 2; ++i) lut32[i] = 0;
+    for (int s = 0; s < n; ++s) {
+        const int l = lens[s];
+        if (!l) continue;
+        const uint32_t c = tnext[l];
+        tnext[l] = (uint16_t)(c + 1);
+        if (sorted) sorted[toffs[l]] = (uint16_t)s;
+        toffs[l] = (uint16_t)(toffs[l] + 1);
+        if (l <= lbits) {
+            const uint32_t rev = __brev(c) >> (32 - l);
+            const uint16_t e = (uint16_t)((s << 4) | l);
+            for (uint32_t idx = rev; idx < (1u << lbits); idx += 1u << l) lut[idx] = e;
+        }
+    }
+    return true;
+}
+
+// Slow path for a LUT miss: canonical walk from length 1 (RFC 1951 3.2.2).
+__device__ bool decode_long(uint64_t bits, const uint16_t* __restrict__ cnt, const uint16_t* __restrict__ sorted,
+                            uint32_t* sym, uint32_t* len) {
+    int code = 0, first = 0, index = 0;
+    for (int l = 1; l <= 15; ++l) {
+        code |= (int)((bits >> (l - 1)) & 1u);
+        const int count = cnt[l];
+        if (code - count < first) {
+            *sym = sorted[index + (code - first)];
+            *len = (uint32_t)l;
+            return true;
+        }
+        index += count;
+        first += count;
+        first <<= 1;
+        code <<= 1;
+    }
+    return false;
+}
+
+__device__ int parse_gz_header(const uint64_t* __restrict__ in64, uint64_t len, uint64_t at, uint64_t* hl) {
+    // gzip.rs:252-300
+    const uint64_t n = len - at;
+    if (n < 10) return ST_TRUNCATED;
+    if (byte_at(in64, at) != 0x1f || byte_at(in64, at + 1) != 0x8b) return ST_BADMAGIC;
+    if (byte_at(in64, at + 2) != 8) return ST_METHOD;
+    const uint32_t flg = byte_at(in64, at + 3);
+    if (flg & 0xe0u) return ST_RESERVED;
+    uint64_t p = 10;
+    if (flg & 4u) {
+        if (p + 2 > n) return ST_TRUNCATED;
+        const uint64_t xlen = byte_at(in64, at + p) | (byte_at(in64, at + p + 1) << 8);
+        p += 2 + xlen;
+        if (p > n) return ST_TRUNCATED;
+    }
+    for (uint32_t f = 8; f <= 16; f <<= 1) {
+        if (flg & f) {
+            while (p < n && byte_at(in64, at + p) != 0) ++p;
+            if (p >= n) return ST_TRUNCATED;
+            ++p;
+        }
+    }
+    if (flg & 2u) {
+        if (p + 2 > n) return ST_TRUNCATED;
+        p += 2;
+    }
+    *hl = p;
+    return ST_OK;
+}
+
+enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_STORED, S_BLOCK_END, S_EXIT };
+
+template <int LB>
+__global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in64, uint64_t nbits, uint64_t in_len,
+                                                const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
+                                                MemberRec* __restrict__ mrec, const uint32_t* __restrict__ work,
+                                                uint32_t n_work, uint32_t* __restrict__ counter,
+                                                uint16_t* __restrict__ cells, uint16_t* __restrict__ sorted_scratch) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int LMASK = (1 << LB) - 1;
+    const int tid = (int)threadIdx.x;
+    uint8_t* my = lds + tid * LaneTables<LB>::kBytes;
+    uint16_t* litlut = reinterpret_cast<uint16_t*>(my);
+    uint16_t* distlut = litlut + (1 << LB);
+    uint8_t* lens = reinterpret_cast<uint8_t*>(distlut + (1 << DIST_LB));
+    uint16_t* cntL = reinterpret_cast<uint16_t*>(lens + 320);
+    uint16_t* cntD = cntL + 16;
+    uint16_t* tnext = cntD + 16;
+    uint16_t* toffs = tnext + 16;
+    uint16_t* cltmp = toffs + 16;  // 16 u16 = 32 B: precode lengths (19 B) while a header is read
+    uint16_t* sortedL = sorted_scratch + ((size_t)blockIdx.x * blockDim.x + tid) * 320;
+    uint16_t* sortedD = sortedL + 288;
+
+    int state = S_FETCH;
+    uint32_t seg = 0;
+    uint64_t pos = 0, hint = 0, bitbuf = 0;
+    uint32_t bitcnt = 0;
+    uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
+    uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
+    uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
+
+#define RRG_REFILL()                      \
+    do {                                  \
+        bitbuf = bits64_at(in64, pos);    \
+        bitcnt = 64;                      \
+    } while (0)
+#define RRG_CONSUME(n)           \
+    do {                         \
+        const uint32_t _n = (n); \
+        bitbuf >>= _n;           \
+        bitcnt -= _n;            \
+        pos += _n;               \
+    } while (0)
+#define RRG_FAIL(st, det)   \
+    do {                    \
+        status = (st);      \
+        detail = (det);     \
+        state = S_FETCH;    \
+        goto finish;        \
+    } while (0)
+// read n (<= 32) bits with the reference's availability rule
+#define RRG_GETBITS(dst, n)                                         \
+    do {                                                            \
+        const uint32_t _k = (n);                                    \
+        if (pos + _k > nbits) RRG_FAIL(ST_EOF, DET_NONE);           \
+        if (bitcnt < _k) RRG_REFILL();                              \
+        (dst) = _k ? (uint32_t)(bitbuf & ((1ull << _k) - 1)) : 0u; \
+        RRG_CONSUME(_k);                                            \
+    } while (0)
+
+    while (true) {
+        if (state == S_FETCH) {
+            const uint32_t w = atomicAdd(counter, 1u);
+            if (w >= n_work) {
+                state = S_EXIT;
+            } else {
+                seg = work[w];
+                const SegDesc d = segs[seg];
+                pos = d.start_bit;
+                hint = d.hint_bit;
+                out_base = d.cells_base;
+                out_cap = d.cells_cap;
+                out_n = 0;
+                member_base = 0;
+                spec = d.flags & SEG_SPEC;
+                single = d.flags & SEG_SINGLE_MEMBER;
+                mb_base = d.mb_base;
+                mb_cap = d.mb_cap;
+                n_mb = 0;
+                status = ST_OK;
+                detail = DET_NONE;
+                final_block = 0;
+                bfinal = 0;
+                if (pos > nbits) {
+                    status = ST_EOF;
+                    goto finish;
+                }
+                RRG_REFILL();
+                state = S_BLOCK_START;
+            }
+        }
+        if (__ballot(state != S_EXIT) == 0) break;
+
+        if (state == S_SYM) {
+            // ---- Huffman symbols (fast_inflate.rs:2065-2229) ----
+            for (int it = 0; it < 64; ++it) {
+                if (bitcnt < 15) RRG_REFILL();
+                uint32_t e = litlut[bitbuf & LMASK];
+                uint32_t l = e & 15u, sym = e >> 4;
+                if (l == 0) {
+                    if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                    if (!decode_long(bitbuf, cntL, sortedL, &sym, &l)) RRG_FAIL(ST_INVALID, DET_NO_CODE);
+                }
+                if (pos + l > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                RRG_CONSUME(l);
+                if (sym < 256) {
+                    if (out_n >= out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
+                    cells[out_base + out_n] = (uint16_t)sym;
+                    ++out_n;
+                    continue;
+                }
+                if (sym == 256) {
+                    state = S_BLOCK_END;
+                    break;
+                }
+                if (sym > 285) RRG_FAIL(ST_INVALID, DET_LITLEN_RANGE);
+                if (bitcnt < 48) RRG_REFILL();
+                const uint32_t li = sym - 257;
+                const uint32_t lx = len_extra(li);
+                if (pos + lx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));
+                RRG_CONSUME(lx);
+                uint32_t de = distlut[bitbuf & ((1u << DIST_LB) - 1)];
+                uint32_t dl = de & 15u, dsym = de >> 4;
+                if (dl == 0) {
+                    if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                    if (!decode_long(bitbuf, cntD, sortedD, &dsym, &dl)) RRG_FAIL(ST_INVALID, DET_NO_CODE);
+                }
+                if (pos + dl > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                RRG_CONSUME(dl);
+                if (dsym >= 30) RRG_FAIL(ST_INVALID, DET_DIST_RANGE);
+                const uint32_t dx = dist_extra(dsym);
+                if (pos + dx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));
+                RRG_CONSUME(dx);
+                if (out_n + length > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
+                uint16_t* o = cells + out_base;
+                const uint64_t cur = out_n - member_base;
+                if ((uint64_t)distance > cur) {
+                    if (!spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
+                    const uint32_t gap = (uint32_t)((uint64_t)distance - cur);
+                    const uint32_t pre = gap < length ? gap : length;
+                    for (uint32_t k = 0; k < pre; ++k) o[out_n + k] = (uint16_t)(0x8000u | (gap - k - 1));
+                    out_n += pre;
+                    length -= pre;
+                }
+                if (length) {
+                    const uint64_t src = out_n - distance;
+                    if (distance == 1) {
+                        const uint16_t v = o[src];
+                        for (uint32_t k = 0; k < length; ++k) o[out_n + k] = v;
+                    } else {
+                        // o[dst+k] = o[src + (k mod distance)]: every load reads a
+                        // cell written before this copy started.
+                        uint32_t jm = 0;
+                        for (uint32_t k = 0; k < length; k += 8) {
+                            uint16_t v[8];
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                v[i] = (k + i < length) ? o[src + jm] : (uint16_t)0;
+                                jm = (jm + 1 == distance) ? 0u : jm + 1;
+                            }
+#pragma unroll
+                            for (int i = 0; i < 8; ++i)
+                                if (k + i < length) o[out_n + k + i] = v[i];
+                        }
+                    }
+                    out_n += length;
+                }
+            }
+        } else if (state == S_BLOCK_START) {
+            // ---- block header (fast_inflate.rs:1914-1935, inflate.rs:74-145) ----
+            if (pos >= hint) {
+                state = S_FETCH;
+                goto finish;
+            }
+            uint32_t v, bt;
+            RRG_GETBITS(v, 1);
+            bfinal = v;
+            RRG_GETBITS(bt, 2);
+            if (bt == 0) {
+                pos = (pos + 7) & ~7ull;
+                RRG_REFILL();
+                uint32_t ln, nln;
+                RRG_GETBITS(ln, 16);
+                RRG_GETBITS(nln, 16);
+                if ((ln ^ nln) != 0xffffu) RRG_FAIL(ST_INVALID, DET_STORED_NLEN);
+                if (pos + 8ull * ln > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                if (out_n + ln > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
+                stored_left = ln;
+                state = S_STORED;
+            } else if (bt == 1) {
+                for (int s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
+                build_table(lens, 288, litlut, LB, cntL, tnext, toffs, sortedL);
+                for (int s = 0; s < 30; ++s) lens[s] = 5;
+                build_table(lens, 30, distlut, DIST_LB, cntD, tnext, toffs, sortedD);
+                state = S_SYM;
+            } else if (bt == 2) {
+                uint32_t a, b2, c2;
+                RRG_GETBITS(a, 5);
+                RRG_GETBITS(b2, 5);
+                RRG_GETBITS(c2, 4);
+                const int hlit = (int)a + 257, hdist = (int)b2 + 1, hclen = (int)c2 + 4;
+                if (hlit > 286 || hdist > 30) RRG_FAIL(ST_INVALID, DET_HLIT_HDIST);
+                uint8_t* cl = reinterpret_cast<uint8_t*>(cltmp);
+                for (int i = 0; i < 19; ++i) lens[i] = 0;
+                for (int i = 0; i < hclen; ++i) {
+                    uint32_t x;
+                    RRG_GETBITS(x, 3);
+                    lens[c_cl_order[i]] = (uint8_t)x;
+                }
+                for (int i = 0; i < 19; ++i) cl[i] = lens[i];
+                // precode LUT (7 bits) lives in the distance LUT while lengths are read
+                if (!build_table(cl, 19, distlut, DIST_LB, cntD, tnext, toffs, nullptr))
+                    RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                const int total = hlit + hdist;
+                int i = 0;
+                while (i < total) {
+                    if (pos + 10 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                    if (bitcnt < 32) RRG_REFILL();
+                    const uint32_t e = distlut[bitbuf & 127u];
+                    const uint32_t l = e & 15u;
+                    if (l == 0) {
+                        if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
+                        RRG_FAIL(ST_INVALID, DET_NO_CODE);
+                    }
+                    RRG_CONSUME(l);
+                    const uint32_t sym = e >> 4;
+                    if (sym <= 15) {
+                        lens[i++] = (uint8_t)sym;
+                    } else if (sym == 16) {
+                        if (i == 0) RRG_FAIL(ST_INVALID, DET_CODE16_FIRST);
+                        uint32_t x;
+                        RRG_GETBITS(x, 2);
+                        const int n = (int)x + 3;
+                        if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE16_OVERRUN);
+                        const uint8_t prev = lens[i - 1];
+                        for (int j = 0; j < n; ++j) lens[i + j] = prev;
+                        i += n;
+                    } else if (sym == 17) {
+                        uint32_t x;
+                        RRG_GETBITS(x, 3);
+                        const int n = (int)x + 3;
+                        if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE17_OVERRUN);
+                        for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                        i += n;
+                    } else {
+                        uint32_t x;
+                        RRG_GETBITS(x, 7);
+                        const int n = (int)x + 11;
+                        if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE18_OVERRUN);
+                        for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                        i += n;
+                    }
+                }
+                if (lens[256] == 0) RRG_FAIL(ST_INVALID, DET_EOB_ZERO);
+                if (!build_table(lens, hlit, litlut, LB, cntL, tnext, toffs, sortedL))
+                    RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                if (!build_table(lens + hlit, hdist, distlut, DIST_LB, cntD, tnext, toffs, sortedD))
+                    RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                state = S_SYM;
+            } else {
+                RRG_FAIL(ST_INVALID, DET_RESERVED_BTYPE);
+            }
+        } else if (state == S_STORED) {
+            // ---- stored block, 8 bytes per step (fast_inflate.rs:1937-1963) ----
+            for (int it = 0; it < 32 && stored_left; ++it) {
+                const uint64_t w = bits64_at(in64, pos);
+                const uint32_t n = stored_left < 8 ? stored_left : 8u;
+                uint16_t* o = cells + out_base + out_n;
+                for (uint32_t k = 0; k < n; ++k) o[k] = (uint16_t)((w >> (8 * k)) & 0xffu);
+                out_n += n;
+                pos += 8ull * n;
+                stored_left -= n;
+            }
+            if (!stored_left) {
+                RRG_REFILL();
+                state = S_BLOCK_END;
+            }
+        } else if (state == S_BLOCK_END) {
+            if (!bfinal) {
+                state = S_BLOCK_START;
+            } else {
+                // member end: trailer, then the next member's header (pipeline.rs:1125-1170)
+                pos = (pos + 7) & ~7ull;
+                const uint64_t tb = pos >> 3;
+                if (tb + 8 > in_len) RRG_FAIL(ST_TRUNCATED, DET_NONE);
+                const uint64_t w = bits64_at(in64, pos);
+                if (n_mb >= mb_cap) RRG_FAIL(ST_MB_OVERFLOW, DET_NONE);
+                MemberRec m;
+                m.out_off = out_n;
+                m.trailer_byte = tb;
+                m.crc = (uint32_t)w;
+                m.isize = (uint32_t)(w >> 32);
+                mrec[mb_base + n_mb] = m;
+                ++n_mb;
+                const uint64_t after = tb + 8;
+                pos = after * 8;
+                if (after >= in_len) {
+                    final_block = 1;
+                    state = S_FETCH;
+                    goto finish;
+                }
+                if (single) {
+                    state = S_FETCH;
+                    goto finish;
+                }
+                uint64_t hl = 0;
+                const int hs = parse_gz_header(in64, in_len, after, &hl);
+                if (hs != ST_OK) RRG_FAIL((uint32_t)hs, DET_NONE);
+                pos = (after + hl) * 8;
+                RRG_REFILL();
+                member_base = out_n;
+                spec = 0;
+                state = S_BLOCK_START;
+            }
+        }
+        continue;
+    finish : {
+        SegResult r;
+        r.n_cells = out_n;
+        r.end_bit = pos;
+        r.status = status;
+        r.detail = detail;
+        r.final_block = final_block;
+        r.n_mb = n_mb;
+        res[seg] = r;
+    }
+    }
+#undef RRG_REFILL
+#undef RRG_CONSUME
+#undef RRG_FAIL
+#undef RRG_GETBITS
+}
+
+int inflate_blocks_per_cu(int lut_bits, int lanes) {
+    const size_t shm = (size_t)lanes * inflate_lds_per_lane(lut_bits);
+    int nb = 0;
+    hipError_t e;
+    switch (lut_bits) {
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<8>, lanes, shm); break;
+        case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<10>, lanes, shm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<9>, lanes, shm); break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_len, const SegDesc* d_segs,
+                          SegResult* d_res, MemberRec* d_mrec, const uint32_t* d_work, uint32_t n_work,
+                          uint32_t* d_counter, uint16_t* d_cells, uint16_t* d_sorted, const InflateGeom& g,
+                          hipStream_t s) {
+    if (n_work == 0) return hipSuccess;
+    const size_t shm = (size_t)g.lanes_per_block * inflate_lds_per_lane(g.lut_bits);
+    switch (g.lut_bits) {
+        case 8:
+            hipLaunchKernelGGL(k_inflate<8>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits, in_len,
+                               d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            break;
+        case 10:
+            hipLaunchKernelGGL(k_inflate<10>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits,
+                               in_len, d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            break;
+        default:
+            hipLaunchKernelGGL(k_inflate<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits, in_len,
+                               d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            break;
+    }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Tail chain.  A segment's markers can only reference the 32 Ki cells before
+// it, which always lie in the last min(len, 32 Ki) cells of earlier segments
+// (their "tail zones").  Every marker in a tail zone gets a list entry whose
+// pointer is the tail-zone index it copies; pointer jumping then resolves all
+// of them in O(log #segments) rounds, and the patch kernel writes the bytes
+// back into the cells as literals.  This replaces the reference's serial
+// build_prev_tail_fast sweep (pipeline.rs:1043-1078).
+// ===========================================================================
+__device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, uint32_t c, uint32_t off,
+                                              uint32_t* s_out, uint64_t* local_out) {
+    uint64_t rem = (uint64_t)off + 1;
+    int64_t s = (int64_t)c - 1;
+    while (s >= 0 && rem > cs[s].len) {
+        rem -= cs[s].len;
+        --s;
+    }
+    if (s < 0) return false;
+    *s_out = (uint32_t)s;
+    *local_out = cs[s].len - rem;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_tail_init(const ChainSeg* __restrict__ cs, uint32_t n_seg,
+                                                   const uint16_t* __restrict__ cells, uint32_t* __restrict__ tz_state,
+                                                   uint64_t* __restrict__ lst_self, uint64_t* __restrict__ lst_ptr,
+                                                   uint64_t* __restrict__ lst_cell, uint64_t lst_cap,
+                                                   uint32_t* __restrict__ counters) {
+    const uint32_t c = blockIdx.x;
+    if (c >= n_seg) return;
+    const ChainSeg me = cs[c];
+    const uint64_t tl = me.len < 32768 ? me.len : 32768;
+    const uint64_t first = me.len - tl;
+    for (uint64_t j = threadIdx.x; j < ((tl + 255) & ~255ull); j += 256) {
+        bool is_marker = false;
+        uint64_t k = 0, tgt = 0, cell_addr = 0;
+        if (j < tl) {
+            cell_addr = me.cells_base + first + j;
+            const uint16_t cell = cells[cell_addr];
+            k = me.tz_base + j;
+            if (!(cell & 0x8000u)) {
+                tz_state[k] = 0x80000000u | (cell & 0xffu);
+            } else {
+                uint32_t s;
+                uint64_t local;
+                if (!marker_target(cs, c, cell & 0x7fffu, &s, &local)) {
+                    atomicMin(&counters[1], c);
+                    tz_state[k] = 0x80000000u;
+                } else {
+                    const uint64_t tls = cs[s].len < 32768 ? cs[s].len : 32768;
+                    tgt = cs[s].tz_base + (local - (cs[s].len - tls));
+                    is_marker = true;
+                }
+            }
+        }
+        // wave-aggregated list append
+        const uint64_t m = __ballot(is_marker);
+        if (m) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counters[0], (uint32_t)__popcll(m));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+            if (is_marker) {
+                const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                if (idx >= lst_cap) continue;  // host re-runs with a larger list
+                tz_state[k] = idx;
+                lst_self[idx] = k;
+                lst_ptr[idx] = tgt;
+                lst_cell[idx] = cell_addr;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tail_jump(uint64_t n_list, uint32_t* __restrict__ tz_state,
+                                                   const uint64_t* __restrict__ lst_self, uint64_t* lst_ptr,
+                                                   uint32_t* __restrict__ remaining) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool pending = false;
+    if (i < n_list) {
+        const uint64_t self = lst_self[i];
+        if (!(tz_state[self] & 0x80000000u)) {
+            const uint64_t p = lst_ptr[i];
+            const uint32_t sp = __atomic_load_n(&tz_state[p], __ATOMIC_RELAXED);
+            if (sp & 0x80000000u) {
+                __atomic_store_n(&tz_state[self], sp, __ATOMIC_RELAXED);
+            } else {
+                const uint64_t np = __atomic_load_n(&lst_ptr[sp], __ATOMIC_RELAXED);
+                __atomic_store_n(&lst_ptr[i], np, __ATOMIC_RELAXED);
+                pending = true;
+            }
+        }
+    }
+    const uint64_t m = __ballot(pending);
+    if (m && (threadIdx.x & 63u) == (uint32_t)__ffsll((unsigned long long)m) - 1)
+        atomicAdd(remaining, (uint32_t)__popcll(m));
+}
+
+__global__ __launch_bounds__(256) void k_tail_patch(uint64_t n_list, const uint32_t* __restrict__ tz_state,
+                                                    const uint64_t* __restrict__ lst_self,
+                                                    const uint64_t* __restrict__ lst_cell, uint16_t* __restrict__ cells) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_list) cells[lst_cell[i]] = (uint16_t)(tz_state[lst_self[i]] & 0xffu);
+}
+
+hipError_t launch_tail_init(const ChainSeg* d_cs, uint32_t n_seg, uint64_t lst_cap, const uint16_t* d_cells,
+                            uint32_t* d_tz_state, uint64_t* d_lst_self, uint64_t* d_lst_ptr, uint64_t* d_lst_cell,
+                            uint32_t* d_counters, hipStream_t s) {
+    if (n_seg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_init, dim3(n_seg), dim3(256), 0, s, d_cs, n_seg, d_cells, d_tz_state, d_lst_self,
+                       d_lst_ptr, d_lst_cell, lst_cap, d_counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_tail_jump(uint64_t n_list, uint32_t* d_tz_state, const uint64_t* d_lst_self, uint64_t* d_lst_ptr,
+                            uint32_t* d_remaining, hipStream_t s) {
+    if (n_list == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_jump, dim3((unsigned)((n_list + 255) / 256)), dim3(256), 0, s, n_list, d_tz_state,
+                       d_lst_self, d_lst_ptr, d_remaining);
+    return hipGetLastError();
+}
+
+hipError_t launch_tail_patch(uint64_t n_list, const uint32_t* d_tz_state, const uint64_t* d_lst_self,
+                             const uint64_t* d_lst_cell, uint16_t* d_cells, hipStream_t s) {
+    if (n_list == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_patch, dim3((unsigned)((n_list + 255) / 256)), dim3(256), 0, s, n_list, d_tz_state,
+                       d_lst_self, d_lst_cell, d_cells);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Resolve + narrow.  Each thread owns one aligned 8-byte output word.  The 8
+// cells behind it start at a segment-constant cell phase, fetched with two
+// aligned 16-byte loads; marker cells read the (already patched, literal) cell
+// they copy from the preceding tail zone.  Words that straddle a tile edge are
+// written byte by byte.
+// ===========================================================================
+__device__ __forceinline__ uint64_t funnel64(uint64_t lo, uint64_t hi, uint32_t sh) {
+    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+}
+
+__global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__ tiles,
+                                                 const ChainSeg* __restrict__ cs, const uint16_t* __restrict__ cells,
+                                                 uint8_t* __restrict__ out, uint32_t* __restrict__ err) {
+    const ResolveTile t = tiles[blockIdx.x];
+    const ChainSeg me = cs[t.seg];
+    const uint64_t ob = t.out_off, oe = t.out_off + t.n;
+    // cell index of output byte p is t.cell_addr + (p - ob)
+    const uint64_t w0 = ob >> 3, w1 = (oe + 7) >> 3;
+    const uint64_t* c64 = reinterpret_cast<const uint64_t*>(cells);
+    for (uint64_t w = w0 + threadIdx.x; w < w1; w += 256) {
+        const uint64_t p0 = w << 3;
+        // cell index for byte p0 (may be before the tile; only in-tile bytes are kept)
+        const int64_t cidx = (int64_t)t.cell_addr + ((int64_t)p0 - (int64_t)ob);
+        const int64_t a = cidx & ~3ll;  // 4-cell (8-byte) aligned
+        const uint32_t sh = (uint32_t)(cidx - a) * 16u;
+        const int64_t q = a >> 2;
+        const uint64_t q0 = c64[q < 0 ? 0 : q], q1 = c64[q + 1], q2 = c64[q + 2];
+        const uint64_t lo = funnel64(q0, q1, sh), hi = funnel64(q1, q2, sh);
+        uint64_t word = 0;
+        uint32_t keep = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t p = p0 + (uint64_t)i;
+            if (p < ob || p >= oe) continue;
+            keep |= 1u << i;
+            uint32_t cell = (uint32_t)(((i < 4 ? lo : hi) >> (16 * (i & 3))) & 0xffffu);
+            if (cell & 0x8000u) {
+                uint32_t s;
+                uint64_t local;
+                if (marker_target(cs, t.seg, cell & 0x7fffu, &s, &local)) {
+                    cell = cells[cs[s].cells_base + local];
+                } else {
+                    atomicMin(err, t.seg);
+                    cell = 0;
+                }
+            }
+            word |= (uint64_t)(cell & 0xffu) << (8 * i);
+        }
+        if (keep == 0xffu) {
+            reinterpret_cast<uint64_t*>(out)[w] = word;
+        } else {
+            for (int i = 0; i < 8; ++i)
+                if (keep & (1u << i)) out[p0 + i] = (uint8_t)(word >> (8 * i));
+        }
+    }
+    (void)me;
+}
+
+hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs, const uint16_t* d_cells,
+                          uint8_t* d_out, uint32_t* d_err, hipStream_t s) {
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resolve, dim3(n_tiles), dim3(256), 0, s, d_tiles, d_cs, d_cells, d_out, d_err);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// CRC-32 of pieces (raw: register starts at 0, no final inversion; the host
+// folds pieces into member CRCs with GF(2) shifts).  256 threads per piece;
+// thread t owns the 1 KiB sub-block that ends 1 KiB*(255-t) before the piece
+// end, so every sub-block is "full" (leading bytes before the piece start
+// are skipped, which is the same as zeros for a register that starts at 0).
+// Slice-by-8 tables in LDS; the 256 partial CRCs are folded in a tree with
+// multipliers x^(8*1024*2^k) mod P.
+// ===========================================================================
+__constant__ uint32_t c_crc_shift[8];  // x^(8*1024*2^k) mod P, reflected
+
+__device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
+    // a*b mod P over GF(2), reflected bit order (bit 31 = x^0)
+    uint32_t prod = 0;
+#pragma unroll 4
+    for (int i = 0; i < 32; ++i) {
+        prod ^= (0u - ((a >> (31 - i)) & 1u)) & b;
+        b = (b >> 1) ^ (0xEDB88320u & (0u - (b & 1u)));
+    }
+    return prod;
+}
+
+__global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ out,
+                                                    const uint64_t* __restrict__ piece_off,
+                                                    const uint32_t* __restrict__ piece_len, uint32_t* __restrict__ crc) {
+    __shared__ uint32_t T[8][256];
+    __shared__ uint32_t part[256];
+    const uint32_t tid = threadIdx.x;
+    {
+        uint32_t c = tid;
+        for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
+        T[0][tid] = c;
+    }
+    __syncthreads();
+    for (int s = 1; s < 8; ++s) {
+        const uint32_t prev = T[s - 1][tid];
+        T[s][tid] = (prev >> 8) ^ T[0][prev & 0xffu];
+        __syncthreads();
+    }
+    const uint64_t off = piece_off[blockIdx.x];
+    const uint64_t end = off + piece_len[blockIdx.x];
+    // sub-block [sb, sb+1024): sb = end - 1024*(256 - tid), clipped at off
+    const int64_t sb = (int64_t)end - 1024ll * (int64_t)(256 - tid);
+    uint64_t b = sb < (int64_t)off ? off : (uint64_t)sb;
+    const uint64_t e = (uint64_t)(sb + 1024);
+    uint32_t c = 0;
+    // head bytes up to 8-byte alignment
+    while (b < e && (b & 7)) {
+        c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
+        ++b;
+    }
+    const uint64_t* w64 = reinterpret_cast<const uint64_t*>(out);
+    for (; b + 8 <= e; b += 8) {
+        const uint64_t v = w64[b >> 3];
+        const uint32_t lo = (uint32_t)v ^ c, hi = (uint32_t)(v >> 32);
+        c = T[7][lo & 0xffu] ^ T[6][(lo >> 8) & 0xffu] ^ T[5][(lo >> 16) & 0xffu] ^ T[4][lo >> 24] ^
+            T[3][hi & 0xffu] ^ T[2][(hi >> 8) & 0xffu] ^ T[1][(hi >> 16) & 0xffu] ^ T[0][hi >> 24];
+    }
+    for (; b < e; ++b) c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
+    part[tid] = c;
+    __syncthreads();
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t span = 1u << k;
+        if ((tid & ((span << 1) - 1)) == 0) part[tid] = gf2_mul(c_crc_shift[k], part[tid]) ^ part[tid + span];
+        __syncthreads();
+    }
+    if (tid == 0) crc[blockIdx.x] = part[0];
+}
+
+hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
+                             uint32_t n_pieces, uint32_t* d_crc, hipStream_t s) {
+    if (n_pieces == 0) return hipSuccess;
+    static bool init = false;
+    if (!init) {
+        // x^(8*1024*2^k) mod P: square x^8 ten times to reach x^(8*1024), then keep squaring
+        uint32_t x = 0x40000000u;  // x^1
+        auto mul = [](uint32_t a, uint32_t b) {
+            uint32_t prod = 0;
+            for (int i = 0; i < 32; ++i) {
+                if (a & (0x80000000u >> i)) prod ^= b;
+                b = (b & 1u) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+            }
+            return prod;
+        };
+        for (int i = 0; i < 13; ++i) x = mul(x, x);  // x^(2^13) = x^(8*1024)
+        uint32_t k8[8];
+        for (int k = 0; k < 8; ++k) {
+            k8[k] = x;
+            x = mul(x, x);
+        }
+        hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_shift), k8, sizeof k8);
+        if (e != hipSuccess) return e;
+        init = true;
+    }
+    hipLaunchKernelGGL(k_crc_pieces, dim3(n_pieces), dim3(256), 0, s, d_out, d_piece_off, d_piece_len, d_crc);
+    return hipGetLastError();
+}
+
+}  // namespace rrg

@@ -1,0 +1,46 @@
+// rrg_launch.h -- host-side launchers for the HIP kernels in rrg_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rrg_device.h"
+
+namespace rrg {
+
+// Block finder: found[i] = first verified dynamic-block offset >= targets[i]
+// (UINT64_MAX when none).  One wavefront per target.
+hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t nbits, const uint64_t* d_targets,
+                              uint64_t* d_found, uint32_t n_targets, hipStream_t s);
+
+// Persistent inflate over the segment indices in d_work[0..n_work).
+struct InflateGeom {
+    int lanes_per_block;  // threads per workgroup (one lane decodes one segment)
+    int blocks;           // persistent workgroups
+    int lut_bits;         // literal/length LUT bits (8..10)
+};
+int inflate_lds_per_lane(int lut_bits);
+int inflate_blocks_per_cu(int lut_bits, int lanes);
+hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_len, const SegDesc* d_segs,
+                          SegResult* d_res, MemberRec* d_mrec, const uint32_t* d_work, uint32_t n_work,
+                          uint32_t* d_counter, uint16_t* d_cells, uint16_t* d_sorted, const InflateGeom& g,
+                          hipStream_t s);
+
+// Tail chain by pointer jumping over the markers in every segment's last
+// 32 Ki cells (the only cells later segments can reference).
+hipError_t launch_tail_init(const ChainSeg* d_cs, uint32_t n_seg, uint64_t lst_cap, const uint16_t* d_cells,
+                            uint32_t* d_tz_state, uint64_t* d_lst_self, uint64_t* d_lst_ptr,
+                            uint64_t* d_lst_cell, uint32_t* d_counters, hipStream_t s);
+hipError_t launch_tail_jump(uint64_t n_list, uint32_t* d_tz_state, const uint64_t* d_lst_self,
+                            uint64_t* d_lst_ptr, uint32_t* d_remaining, hipStream_t s);
+hipError_t launch_tail_patch(uint64_t n_list, const uint32_t* d_tz_state, const uint64_t* d_lst_self,
+                             const uint64_t* d_lst_cell, uint16_t* d_cells, hipStream_t s);
+
+// Resolve markers and narrow u16 cells to u8 output.
+hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs,
+                          const uint16_t* d_cells, uint8_t* d_out, uint32_t* d_err, hipStream_t s);
+
+// Raw (init 0, no final xor) CRC-32 of each piece [off, off+len), len <= 256 KiB.
+hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
+                             uint32_t n_pieces, uint32_t* d_crc, hipStream_t s);
+
+}  // namespace rrg

@@ -1,0 +1,223 @@
+"""GPU parity: the HIP decoder through the C ABI against the oracle.
+
+Bit-exact bytes, identical chunk boundaries (the reference's Vec stream),
+identical DecodeStats, and errors (never faults) on corrupt input.
+"""
+import gzip as pygzip
+import hashlib
+import json
+import os
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def lcg_payload(n, seed):
+    rng = np.random.default_rng(seed & 0xFFFFFFFF)
+    return (rng.integers(0, 95, n, dtype=np.uint8) + 32).tobytes()
+
+
+def gz(data, level=6):
+    return pygzip.compress(data, compresslevel=level, mtime=0)
+
+
+def gpu_decode(decoder, data, chunk, seg=0, lut_bits=9):
+    import rusty_rapidgzip_amd as rrg
+
+    out = []
+    st = decoder.decode_bytes(data, out, rrg.Config(chunk_size_bytes=chunk, segment_bytes=seg, lut_bits=lut_bits))
+    return out, st
+
+
+def check_same(decoder, data, chunk, seg=0, lut_bits=9, expect=None):
+    rc, ochunks, ost, oer = oracle.read_gz(data, chunk_size=chunk, threads=4)
+    assert rc == 0, oer.msg
+    out, st = gpu_decode(decoder, data, chunk, seg, lut_bits)
+    got = b"".join(out)
+    want = b"".join(ochunks)
+    if expect is not None:
+        assert want == expect
+    assert len(got) == len(want)
+    assert got == want
+    assert [len(c) for c in out] == [len(c) for c in ochunks]
+    assert st.uncompressed_bytes == ost.uncompressed_bytes
+    assert st.chunks_decoded == ost.chunks_decoded
+    assert st.speculation_failures == ost.speculation_failures
+    return st
+
+
+def test_corpus_fixtures(decoder):
+    sums = json.load(open(os.path.join(GOLDEN, "corpus_sums.json")))
+    for name, meta in sorted(sums.items()):
+        _say("fixture", name)
+        data = open(os.path.join(GOLDEN, "corpus", name), "rb").read()
+        for chunk in (65536, 4 << 20):
+            out, st = gpu_decode(decoder, data, chunk)
+            got = b"".join(out)
+            assert hashlib.sha256(got).hexdigest() == meta["sha256"], name
+            check_same(decoder, data, chunk)
+
+
+@pytest.mark.parametrize("seg", [32 << 10, 128 << 10, 0])
+@pytest.mark.parametrize("lut_bits", [8, 9, 10])
+def test_fastq_small(decoder, seg, lut_bits):
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(6 << 20)
+    g = synth.gzip(raw, 6, piece=2 << 20).tobytes()
+    _say("fastq", seg, lut_bits, len(g))
+    check_same(decoder, g, 1 << 20, seg, lut_bits, expect=raw.tobytes())
+
+
+@pytest.mark.parametrize("level", [1, 6, 9])
+def test_text_levels(decoder, level):
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.text(5 << 20)
+    g = synth.gzip(raw, level, piece=2 << 20).tobytes()
+    _say("text", level, len(g))
+    check_same(decoder, g, 512 << 10, 48 << 10, expect=raw.tobytes())
+
+
+def test_lcg_levels_matrix(decoder):
+    for level in range(1, 10):
+        p = lcg_payload(1 << 20, 1000 + level)
+        _say("lcg level", level)
+        check_same(decoder, gz(p, level), 65536, expect=p)
+
+
+def test_mixed_entropy(decoder):
+    from rusty_rapidgzip_amd import synth
+
+    raw, lv, ends = synth.mixed(24 << 20)
+    g = synth.gzip_segments(raw, lv, ends, piece=4 << 20).tobytes()
+    _say("mixed", len(g))
+    check_same(decoder, g, 1 << 20, 64 << 10, expect=raw.tobytes())
+
+
+def test_multi_member(decoder):
+    parts = [lcg_payload(n, 100 + n) for n in (0, 1, 5000, 300_000, 0, 70_000, 1 << 20)]
+    g = b"".join(gz(p, 6) for p in parts)
+    check_same(decoder, g, 65536, expect=b"".join(parts))
+    tiny = b"".join(gz(bytes([65 + i % 26]) * (i % 7), 6) for i in range(500))
+    check_same(decoder, tiny, 65536)
+
+
+def test_nested_gzip_speculation_failures(decoder):
+    inner = gz(lcg_payload(2 << 20, 42), 6)
+    outer = gz(inner, 1)
+    st = check_same(decoder, outer, 65536, expect=inner)
+    _say("nested spec failures", st.speculation_failures)
+
+
+def test_bgzf(decoder):
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(5 << 20)
+    bg = synth.bgzf(raw, 6).tobytes()
+    check_same(decoder, bg, 1 << 20, expect=raw.tobytes())
+    # BGZF followed by a plain gzip member (the non-BGZF remainder path)
+    tail = lcg_payload(200_000, 9)
+    check_same(decoder, bg[:-28] + gz(tail, 6), 1 << 20, expect=raw.tobytes() + tail)
+
+
+def test_errors_not_faults(decoder):
+    import rusty_rapidgzip_amd as rrg
+
+    p = lcg_payload(400_000, 5)
+    good = gz(p, 6)
+    bad = bytearray(good)
+    bad[-6] ^= 1
+    with pytest.raises(rrg.GzipError) as ei:
+        gpu_decode(decoder, bytes(bad), 65536)
+    assert ei.value.kind == "CrcMismatch"
+    bad = bytearray(good)
+    bad[-2] ^= 1
+    with pytest.raises(rrg.GzipError) as ei:
+        gpu_decode(decoder, bytes(bad), 65536)
+    assert ei.value.kind == "IsizeMismatch"
+    with pytest.raises(rrg.RapidgzipError):
+        gpu_decode(decoder, good[: len(good) // 2], 65536)
+    with pytest.raises(rrg.RapidgzipError):
+        gpu_decode(decoder, good + b"junk", 65536)
+    with pytest.raises(rrg.GzipError) as ei:
+        gpu_decode(decoder, b"\x1f\x8c" + good[2:], 65536)
+    assert ei.value.kind == "BadMagic"
+    with pytest.raises(rrg.IoError):
+        gpu_decode(decoder, b"", 65536)
+    # random corruption inside the deflate body: an error or (if the damage is
+    # undetectable by deflate) a CRC error -- never a fault
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        b = bytearray(good)
+        for pos in rng.integers(20, len(b) - 8, 3):
+            b[pos] ^= int(rng.integers(1, 256))
+        with pytest.raises(rrg.RapidgzipError):
+            gpu_decode(decoder, bytes(b), 65536)
+    # decoder still healthy afterwards
+    check_same(decoder, good, 65536, expect=p)
+
+
+def test_device_api(decoder):
+    import ctypes
+
+    import torch
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(4 << 20)
+    g = synth.gzip(raw, 6, piece=1 << 20).tobytes()
+    d = torch.frombuffer(bytearray(g), dtype=torch.uint8).to("cuda:0")
+    torch.cuda.synchronize()
+    ptr, n, ends, st = decoder.decode_device(d.data_ptr(), len(g), g, rrg.Config(chunk_size_bytes=1 << 20))
+    assert n == raw.size and ends[-1] == n
+    rc, ochunks, _, _ = oracle.read_gz(g, chunk_size=1 << 20, threads=4)
+    assert np.diff([0] + ends).tolist() == [len(c) for c in ochunks]
+    host = np.empty(n, dtype=np.uint8)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(host.ctypes.data, ctypes.c_void_p(ptr), n, 2) == 0  # device -> host
+    assert host.tobytes() == raw.tobytes()
+
+
+def test_sink_closed(decoder):
+    import rusty_rapidgzip_amd as rrg
+
+    p = lcg_payload(1 << 20, 77)
+    g = gz(p, 6)
+    got = []
+
+    def sink(b):
+        got.append(b)
+        if len(got) == 2:
+            raise rrg.SinkClosed()
+
+    st = decoder.decode_bytes(g, sink, rrg.Config(chunk_size_bytes=65536))
+    assert st.chunks_decoded == 1  # the third send never happens; the second is not counted
+    assert b"".join(got) == p[: sum(len(x) for x in got)]
+
+
+def test_read_gz_file(decoder, tmp_path):
+    import rusty_rapidgzip_amd as rrg
+
+    p = lcg_payload(700_000, 3)
+    f = tmp_path / "x.gz"
+    f.write_bytes(gz(p, 6))
+    out = []
+    st = rrg.read_gz(f, out, rrg.Config(chunk_size_bytes=65536))
+    assert b"".join(out) == p
+    assert st.compressed_bytes == f.stat().st_size
+    assert zlib.crc32(b"".join(out)) == zlib.crc32(p)
