@@ -285,59 +285,53 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     return G;
 }
 
-// Run the inflate kernel over `work`, growing cell / member capacity for
-// segments that overflow, until every listed segment has a final status.
+// Give segment j a larger output region (cells or member records) after it
+// overflowed; the old region is abandoned.
+void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& mb_used) {
+    if (status == ST_MB_OVERFLOW) {
+        d.mb_base = (uint32_t)mb_used;
+        d.mb_cap = d.mb_cap * 16 + 16;
+        mb_used += d.mb_cap;
+    } else {
+        const uint64_t cap = d.cells_cap * 8 + 65536;
+        d.cells_base = (cells_used + 7) & ~7ull;
+        d.cells_cap = cap;
+        cells_used = d.cells_base + cap;
+    }
+}
+
+// One launch of the inflate kernel over the segment indices in `work`; results
+// for every segment are copied back into `res`.
 int run_inflate(Ctx& c, const Geometry& G, uint64_t nbits, uint64_t in_len, std::vector<SegDesc>& desc,
-                std::vector<SegResult>& res, uint64_t& cells_used, uint64_t& mb_used, std::vector<uint32_t> work,
+                std::vector<SegResult>& res, uint64_t cells_used, uint64_t mb_used, const std::vector<uint32_t>& work,
                 rrg_error* err) {
     const size_t S = desc.size();
+    if (work.empty()) return RRG_OK;
     HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * 320 * sizeof(uint16_t)));
-    for (int round = 0; !work.empty(); ++round) {
-        if (round > 12) {
-            set_err(err, RRG_E_ARG, "decoded output exceeds the decoder's capacity");
-            return RRG_E_ARG;
-        }
-        HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), cells_used * sizeof(uint16_t), c.s));
-        HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), mb_used * sizeof(MemberRec), c.s));
-        HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));
-        HIPCHK(c.res.ensure(S * sizeof(SegResult)));
-        HIPCHK(c.work.ensure(work.size() * sizeof(uint32_t)));
-        HIPCHK(c.counter.ensure(16));
-        HIPCHK(hipMemcpyAsync(c.segs.p, desc.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice, c.s));
-        HIPCHK(hipMemcpyAsync(c.work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c.s));
-        HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
-        {
-            StageTimer t(c, &c.prof.ms_inflate);
-            HIPCHK(launch_inflate(c.in.as<uint64_t>(), nbits, in_len, c.segs.as<SegDesc>(), c.res.as<SegResult>(),
-                                  c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
-                                  c.counter.as<uint32_t>(), c.cells.as<uint16_t>(), c.sorted.as<uint16_t>(), G.g,
-                                  c.s));
-            t.stop();
-        }
-        c.prof.n_inflate_launches++;
-        res.resize(S);
-        HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
-        HIPCHK(hipStreamSynchronize(c.s));
-        std::vector<uint32_t> again;
-        for (uint32_t j : work) {
-            SegResult& r = res[j];
-            if (r.status == ST_OVERFLOW) {
-                const uint64_t cap = desc[j].cells_cap * 8 + 65536;
-                desc[j].cells_base = (cells_used + 7) & ~7ull;
-                desc[j].cells_cap = cap;
-                cells_used = desc[j].cells_base + cap;
-                again.push_back(j);
-            } else if (r.status == ST_MB_OVERFLOW) {
-                desc[j].mb_base = (uint32_t)mb_used;
-                desc[j].mb_cap = desc[j].mb_cap * 16 + 16;
-                mb_used += desc[j].mb_cap;
-                again.push_back(j);
-            }
-        }
-        work.swap(again);
+    HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), c.cells.cap, c.s));
+    HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
+    HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));
+    HIPCHK(c.res.ensure(S * sizeof(SegResult)));
+    HIPCHK(c.work.ensure(work.size() * sizeof(uint32_t)));
+    HIPCHK(c.counter.ensure(16));
+    HIPCHK(hipMemcpyAsync(c.segs.p, desc.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
+    {
+        StageTimer t(c, &c.prof.ms_inflate);
+        HIPCHK(launch_inflate(c.in.as<uint64_t>(), nbits, in_len, c.segs.as<SegDesc>(), c.res.as<SegResult>(),
+                              c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
+                              c.counter.as<uint32_t>(), c.cells.as<uint16_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
+        t.stop();
     }
+    c.prof.n_inflate_launches++;
+    res.resize(S);
+    HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
     return RRG_OK;
 }
+
+inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
 
 // Geometry of the accepted chain, shared by the resolve / CRC stages.
 struct ChainOut {
@@ -569,8 +563,9 @@ int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len,
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
     int rc = run_inflate(c, G, nbits, file_len, desc, res, cells_used, mb_used, all, err);
     if (rc) return rc;
+    // a capacity overflow is not a decode failure (the region is grown below)
     std::vector<uint8_t> first_ok(S);
-    for (size_t j = 0; j < S; ++j) first_ok[j] = res[j].status == ST_OK;
+    for (size_t j = 0; j < S; ++j) first_ok[j] = res[j].status == ST_OK || overflowed(res[j].status);
 
     // anchored chain (pipeline.rs:774-809) with batched re-decode rounds
     size_t n_chain = 0;
@@ -593,6 +588,13 @@ int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len,
                 }
                 continue;
             }
+            if (desc[j].start_bit == expected && overflowed(r.status)) {
+                // verified start, output region too small: grow and decode again
+                grow_segment(desc[j], r.status, cells_used, mb_used);
+                redo.push_back((uint32_t)j);
+                if (j + 1 < S) expected = desc[j + 1].start_bit;
+                continue;
+            }
             if (desc[j].start_bit == expected) {  // genuine error at a true boundary
                 if (redo.empty()) fatal = (long)j;
                 break;
@@ -602,7 +604,7 @@ int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len,
             if (j + 1 < S) expected = desc[j + 1].start_bit;  // optimistic; verified next round
         }
         if (redo.empty()) break;
-        if (round > (int)S + 2) {
+        if (round > (int)S + 64) {
             set_err(err, RRG_E_DEVICE, "verification chain did not settle");
             return RRG_E_DEVICE;
         }
@@ -772,6 +774,17 @@ int bgzf_path(Ctx& c, const Geometry& G, const uint8_t* h, uint64_t len, const r
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
     int rc = run_inflate(c, G, len * 8, len, desc, res, cells_used, mb_used, all, err);
     if (rc) return rc;
+    for (int round = 0; round < 6; ++round) {
+        std::vector<uint32_t> again;
+        for (size_t j = 0; j < S; ++j)
+            if (overflowed(res[j].status)) {
+                grow_segment(desc[j], res[j].status, cells_used, mb_used);
+                again.push_back((uint32_t)j);
+            }
+        if (again.empty()) break;
+        rc = run_inflate(c, G, len * 8, len, desc, res, cells_used, mb_used, again, err);
+        if (rc) return rc;
+    }
     std::vector<uint64_t> off(S + 1, 0);
     for (size_t j = 0; j < S; ++j) off[j + 1] = off[j] + res[j].n_cells;
     const uint64_t U = off[S];

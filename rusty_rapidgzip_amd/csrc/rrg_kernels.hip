@@ -985,9 +985,11 @@ __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ 
     const uint64_t off = piece_off[blockIdx.x];
     const uint64_t end = off + piece_len[blockIdx.x];
     // sub-block [sb, sb+1024): sb = end - 1024*(256 - tid), clipped at off
+    // (signed: for a short piece most sub-blocks lie wholly before `off`)
     const int64_t sb = (int64_t)end - 1024ll * (int64_t)(256 - tid);
+    const int64_t es = sb + 1024;
     uint64_t b = sb < (int64_t)off ? off : (uint64_t)sb;
-    const uint64_t e = (uint64_t)(sb + 1024);
+    const uint64_t e = es < (int64_t)b ? b : (uint64_t)es;
     uint32_t c = 0;
     // head bytes up to 8-byte alignment
     while (b < e && (b & 7)) {
