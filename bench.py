@@ -19,7 +19,6 @@ import json
 import os
 import sys
 import time
-import zlib
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -50,10 +49,9 @@ def main() -> int:
         return 2
 
     import numpy as np
-    import torch
 
     import rusty_rapidgzip_amd as rrg
-    from rusty_rapidgzip_amd import synth
+    from rusty_rapidgzip_amd import hipmem, synth
 
     threads = min(16, os.cpu_count() or 1)
     n = int(args.gib * (1 << 30))
@@ -64,33 +62,22 @@ def main() -> int:
     gz = synth.gzip(raw, 6, threads=threads, piece=64 << 20)
     C = int(gz.size)
     log(f"gzip -6: {C} bytes ({n / C:.2f}:1) in {time.time() - t0:.1f}s")
-    crc_raw = 0
-    for off in range(0, n, 1 << 30):
-        crc_raw = zlib.crc32(raw[off:off + (1 << 30)], crc_raw)
-
-    dev = torch.device("cuda:0")
-    d_in = torch.from_numpy(gz).to(dev)
-    torch.cuda.synchronize()
+    d_in = hipmem.DeviceBuffer.from_host(gz)
+    hipmem.synchronize()
     cfg = rrg.Config(chunk_size_bytes=args.chunk_mib << 20, segment_bytes=args.segment_kib << 10,
                      lut_bits=args.lut_bits, lanes_per_block=args.lanes)
     dec = rrg.Decoder(0)
     host = gz.tobytes()
 
     def step():
-        return dec.decode_device(d_in.data_ptr(), C, host, cfg)
+        return dec.decode_device(d_in.ptr, C, host, cfg)
 
     # warmup (first call also checks the output bytes against the generator)
     for w in range(max(1, args.warmup)):
         ptr, out_len, ends, st = step()
         if w == 0:
             assert out_len == n, (out_len, n)
-            import ctypes
-
-            hip = ctypes.CDLL("libamdhip64.so")
-            hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-            check = np.empty(n, dtype=np.uint8)
-            if hip.hipMemcpy(check.ctypes.data, ctypes.c_void_p(ptr), n, 2) != 0:
-                raise RuntimeError("hipMemcpy failed")
+            check = hipmem.to_host(ptr, n)
             if not np.array_equal(check, raw):
                 raise RuntimeError("GPU output differs from the generated bytes")
             del check
@@ -101,14 +88,14 @@ def main() -> int:
 
     lat = []
     prof = []
-    torch.cuda.synchronize()
+    hipmem.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t1 = time.perf_counter()
         step()
         lat.append((time.perf_counter() - t1) * 1e3)
         prof.append(dec.profile().as_dict())
-    torch.cuda.synchronize()
+    hipmem.synchronize()
     elapsed = time.perf_counter() - t_start
     value = n * args.steps / elapsed / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
@@ -162,8 +149,7 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
-    del d_in
-    torch.cuda.synchronize()
+    d_in.free()
 
     if not args.no_cpu:
         from oracle import oracle

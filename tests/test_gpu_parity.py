@@ -171,26 +171,19 @@ def test_errors_not_faults(decoder):
 
 
 def test_device_api(decoder):
-    import ctypes
-
-    import torch
-
     import rusty_rapidgzip_amd as rrg
-    from rusty_rapidgzip_amd import synth
+    from rusty_rapidgzip_amd import hipmem, synth
 
     raw = synth.fastq(4 << 20)
     g = synth.gzip(raw, 6, piece=1 << 20).tobytes()
-    d = torch.frombuffer(bytearray(g), dtype=torch.uint8).to("cuda:0")
-    torch.cuda.synchronize()
-    ptr, n, ends, st = decoder.decode_device(d.data_ptr(), len(g), g, rrg.Config(chunk_size_bytes=1 << 20))
-    assert n == raw.size and ends[-1] == n
-    rc, ochunks, _, _ = oracle.read_gz(g, chunk_size=1 << 20, threads=4)
-    assert np.diff([0] + ends).tolist() == [len(c) for c in ochunks]
-    host = np.empty(n, dtype=np.uint8)
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    assert hip.hipMemcpy(host.ctypes.data, ctypes.c_void_p(ptr), n, 2) == 0  # device -> host
-    assert host.tobytes() == raw.tobytes()
+    d = hipmem.DeviceBuffer.from_host(g)
+    for host in (g, None):  # with and without the optional host copy
+        ptr, n, ends, st = decoder.decode_device(d.ptr, len(g), host, rrg.Config(chunk_size_bytes=1 << 20))
+        assert n == raw.size and ends[-1] == n
+        rc, ochunks, _, _ = oracle.read_gz(g, chunk_size=1 << 20, threads=4)
+        assert np.diff([0] + ends).tolist() == [len(c) for c in ochunks]
+        assert hipmem.to_host(ptr, n).tobytes() == raw.tobytes()
+    d.free()
 
 
 def test_sink_closed(decoder):
