@@ -360,43 +360,23 @@ int resolve_chain(Ctx& c, const std::vector<SegDesc>& desc, const std::vector<Se
     for (uint32_t j = 0; j < n; ++j) any_spec |= cs[j].spec != 0;
     if (any_spec && tz) {
         StageTimer t(c, &c.prof.ms_tail);
-        HIPCHK(c.tz.ensure(tz * sizeof(uint32_t)));
-        uint64_t cap = std::max<uint64_t>(1 << 20, tz / 16);
-        uint32_t host_ctr[2];
-        for (int pass = 0; pass < 2; ++pass) {
-            HIPCHK(c.lself.ensure(cap * 8));
-            HIPCHK(c.lptr.ensure(cap * 8));
-            HIPCHK(c.lcell.ensure(cap * 8));
-            const uint32_t init_ctr[2] = {0u, UINT32_MAX};
-            HIPCHK(hipMemcpyAsync(c.counters.p, init_ctr, sizeof init_ctr, hipMemcpyHostToDevice, c.s));
-            HIPCHK(launch_tail_init(c.chain.as<ChainSeg>(), n, cap, c.cells.as<uint16_t>(), c.tz.as<uint32_t>(),
-                                    c.lself.as<uint64_t>(), c.lptr.as<uint64_t>(), c.lcell.as<uint64_t>(),
-                                    c.counters.as<uint32_t>(), c.s));
-            HIPCHK(hipMemcpyAsync(host_ctr, c.counters.p, sizeof host_ctr, hipMemcpyDeviceToHost, c.s));
-            HIPCHK(hipStreamSynchronize(c.s));
-            if (host_ctr[0] <= cap) break;
-            cap = host_ctr[0];
-        }
-        const uint64_t n_list = host_ctr[0];
-        if (host_ctr[1] != UINT32_MAX) *marker_err_seg = host_ctr[1];
-        c.prof.n_markers_tail += n_list;
-        for (int round = 0; n_list; ++round) {
-            if (round > 64) {
-                set_err(err, RRG_E_DEVICE, "tail chain did not converge");
-                return RRG_E_DEVICE;
-            }
-            uint32_t rem = 0;
-            HIPCHK(hipMemsetAsync(c.counters.as<uint32_t>() + 2, 0, 4, c.s));
-            HIPCHK(launch_tail_jump(n_list, c.tz.as<uint32_t>(), c.lself.as<uint64_t>(), c.lptr.as<uint64_t>(),
-                                    c.counters.as<uint32_t>() + 2, c.s));
-            HIPCHK(hipMemcpyAsync(&rem, c.counters.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, c.s));
-            HIPCHK(hipStreamSynchronize(c.s));
-            c.prof.n_tail_rounds++;
-            if (rem == 0) break;
-        }
-        HIPCHK(launch_tail_patch(n_list, c.tz.as<uint32_t>(), c.lself.as<uint64_t>(), c.lcell.as<uint64_t>(),
-                                 c.cells.as<uint16_t>(), c.s));
+        // ~4 blocks per CU keeps phase 1 parallel while phase 2 stays short
+        const uint32_t G = std::max<uint32_t>(1, (uint32_t)(n / std::max(1, 4 * c.n_cu)));
+        const uint32_t nb = (n + G - 1) / G;
+        HIPCHK(c.tz.ensure((size_t)nb * 32768 * 2 * sizeof(uint16_t)));
+        HIPCHK(c.counters.ensure(16));
+        const uint32_t none = UINT32_MAX;
+        HIPCHK(hipMemcpyAsync(c.counters.p, &none, 4, hipMemcpyHostToDevice, c.s));
+        uint16_t* exitb = c.tz.as<uint16_t>();
+        uint16_t* entry = exitb + (size_t)nb * 32768;
+        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), exitb, entry,
+                           c.counters.as<uint32_t>(), c.s));
+        uint32_t e1 = UINT32_MAX;
+        HIPCHK(hipMemcpyAsync(&e1, c.counters.p, 4, hipMemcpyDeviceToHost, c.s));
         t.stop();
+        HIPCHK(hipStreamSynchronize(c.s));
+        *marker_err_seg = e1;
+        c.prof.n_tail_rounds = nb;
     }
     // resolve tiles: 64 KiB of output each, aligned in output coordinates
     const uint64_t TILE = 65536;

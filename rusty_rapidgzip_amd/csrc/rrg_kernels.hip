@@ -213,36 +213,42 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
     uint64_t result = ~0ull;
     uint8_t* lens = s_lens + lane * 340;
     uint16_t* cnt = s_cnt + lane * 32;
-    uint64_t base = cur;
-    bool done = (cur + 13 > limit);
-    while (!done) {
-        const uint64_t p = base + (uint64_t)lane;
-        const bool inb = p + 13 <= limit;
-        uint32_t step = 255;
+    bool end = (cur + 13 > limit);
+    while (!end) {
+        // 1. walk the chain, collecting up to 64 visited candidates (lane k holds the k-th)
+        uint64_t mycand = 0;
+        int ncand = 0;
+        while (ncand < 64 && !end) {
+            const uint64_t base = cur;
+            const uint64_t p = base + (uint64_t)lane;
+            uint32_t step = 255;
+            if (p + 13 <= limit) step = s_next[(uint32_t)(bits64_at(in64, p) & 0x1fffu)];
+            const uint64_t cm = __ballot(step == 0);
+            const int nx = (step == 0) ? 1 : (int)step;
+            while (ncand < 64) {
+                const uint64_t rel = cur - base;
+                if (rel >= 64) break;
+                if (cur + 13 > limit) {
+                    end = true;
+                    break;
+                }
+                if ((cm >> rel) & 1ull) {
+                    if (lane == ncand) mycand = cur;
+                    ++ncand;
+                }
+                cur += (uint64_t)__builtin_amdgcn_readlane(nx, (int)rel);
+            }
+        }
+        // 2. verify the visited candidates in parallel; the first that passes wins
         bool ver = false;
-        if (inb) {
-            const uint32_t w = (uint32_t)(bits64_at(in64, p) & 0x1fffu);
-            step = s_next[w];
-            if (step == 0) ver = verify_candidate(in64, nbits, p, lens, cnt);
-        }
+        if (lane < ncand) ver = verify_candidate(in64, nbits, mycand, lens, cnt);
         const uint64_t vm = __ballot(ver);
-        const int nx = (step == 0) ? 1 : (int)step;
-        // follow the chain through this 64-offset window (uniform control)
-        while (true) {
-            const uint64_t rel = cur - base;
-            if (rel >= 64) break;
-            if (cur + 13 > limit) {
-                done = true;
-                break;
-            }
-            if ((vm >> rel) & 1ull) {
-                result = cur;
-                done = true;
-                break;
-            }
-            cur += (uint64_t)__builtin_amdgcn_readlane(nx, (int)rel);
+        if (vm) {
+            const int first = __ffsll((unsigned long long)vm) - 1;
+            result = (uint64_t)__builtin_amdgcn_readlane((int)(uint32_t)mycand, first) |
+                     ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mycand >> 32), first) << 32);
+            break;
         }
-        base += 64;
     }
     if (lane == 0) found[t] = result;
 }
@@ -749,12 +755,28 @@ hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_le
 // ===========================================================================
 // Tail chain.  A segment's markers can only reference the 32 Ki cells before
 // it, which always lie in the last min(len, 32 Ki) cells of earlier segments
-// (their "tail zones").  Every marker in a tail zone gets a list entry whose
-// pointer is the tail-zone index it copies; pointer jumping then resolves all
-// of them in O(log #segments) rounds, and the patch kernel writes the bytes
-// back into the cells as literals.  This replaces the reference's serial
-// build_prev_tail_fast sweep (pipeline.rs:1043-1078).
+// (their "tail zones").  The reference carries that window serially from chunk
+// to chunk (build_prev_tail_fast, pipeline.rs:1043-1078).  Here it is a
+// three-phase scan over blocks of G consecutive segments:
+//
+//   1. k_tail_block (one workgroup per block, in parallel): sweep the block's
+//      segments with the window held in LDS in *symbolic* form -- a cell is a
+//      literal byte or a marker relative to the block's entry window (the 32 Ki
+//      bytes before the block).  The window starts as the identity (cell i =
+//      marker for entry byte i); each segment's tail cells are rewritten in
+//      place to symbolic form and the final window is the block's exit tail.
+//   2. k_tail_entry (one workgroup): entry window of block B+1 = exit tail of
+//      B resolved against entry window of B -- one 32 Ki step per block.
+//   3. k_tail_patch (parallel): every symbolic marker in a tail zone becomes
+//      the literal byte of its block's entry window.
+//
+// After phase 3 every tail-zone cell is a literal, so k_resolve needs one
+// gather per marker.  Entry-window cells carry 0x100 for bytes before the
+// start of the output (a marker reaching there is the reference's
+// "marker references bytes outside the available prefix window").
 // ===========================================================================
+constexpr uint32_t WIN = 32768;
+
 __device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, uint32_t c, uint32_t off,
                                               uint32_t* s_out, uint64_t* local_out) {
     uint64_t rem = (uint64_t)off + 1;
@@ -769,111 +791,96 @@ __device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, u
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_tail_init(const ChainSeg* __restrict__ cs, uint32_t n_seg,
-                                                   const uint16_t* __restrict__ cells, uint32_t* __restrict__ tz_state,
-                                                   uint64_t* __restrict__ lst_self, uint64_t* __restrict__ lst_ptr,
-                                                   uint64_t* __restrict__ lst_cell, uint64_t lst_cap,
-                                                   uint32_t* __restrict__ counters) {
-    const uint32_t c = blockIdx.x;
-    if (c >= n_seg) return;
-    const ChainSeg me = cs[c];
-    const uint64_t tl = me.len < 32768 ? me.len : 32768;
-    const uint64_t first = me.len - tl;
-    for (uint64_t j = threadIdx.x; j < ((tl + 255) & ~255ull); j += 256) {
-        bool is_marker = false;
-        uint64_t k = 0, tgt = 0, cell_addr = 0;
-        if (j < tl) {
-            cell_addr = me.cells_base + first + j;
-            const uint16_t cell = cells[cell_addr];
-            k = me.tz_base + j;
-            if (!(cell & 0x8000u)) {
-                tz_state[k] = 0x80000000u | (cell & 0xffu);
+__global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict__ cs, uint32_t n_seg, uint32_t G,
+                                                     uint16_t* __restrict__ cells, uint16_t* __restrict__ exit_tail) {
+    __shared__ uint16_t win[WIN];
+    const uint32_t B = blockIdx.x, tid = threadIdx.x;
+    for (uint32_t i = tid; i < WIN; i += 1024) win[i] = (uint16_t)(0x8000u | (WIN - 1 - i));
+    __syncthreads();
+    const uint32_t first = B * G, end = min(n_seg, first + G);
+    for (uint32_t s = first; s < end; ++s) {
+        const uint64_t len = cs[s].len;
+        const uint32_t tl = (uint32_t)(len < WIN ? len : WIN);
+        if (tl == 0) continue;
+        const uint64_t tz0 = cs[s].cells_base + len - tl;  // first tail cell
+        const uint32_t keep = WIN - tl;                      // window cells shifted down
+        uint16_t v[WIN / 1024];
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) {
+            const uint32_t i = tid + 1024u * r;
+            uint16_t c;
+            if (i < keep) {
+                c = win[i + tl];
             } else {
-                uint32_t s;
-                uint64_t local;
-                if (!marker_target(cs, c, cell & 0x7fffu, &s, &local)) {
-                    atomicMin(&counters[1], c);
-                    tz_state[k] = 0x80000000u;
-                } else {
-                    const uint64_t tls = cs[s].len < 32768 ? cs[s].len : 32768;
-                    tgt = cs[s].tz_base + (local - (cs[s].len - tls));
-                    is_marker = true;
-                }
+                c = cells[tz0 + (i - keep)];
+                if (c & 0x8000u) c = win[WIN - 1 - (c & 0x7fffu)];
             }
+            v[r] = c;
         }
-        // wave-aggregated list append
-        const uint64_t m = __ballot(is_marker);
-        if (m) {
-            const uint32_t lane = threadIdx.x & 63u;
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&counters[0], (uint32_t)__popcll(m));
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
-            if (is_marker) {
-                const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-                if (idx >= lst_cap) continue;  // host re-runs with a larger list
-                tz_state[k] = idx;
-                lst_self[idx] = k;
-                lst_ptr[idx] = tgt;
-                lst_cell[idx] = cell_addr;
-            }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) {
+            const uint32_t i = tid + 1024u * r;
+            win[i] = v[r];
+            if (i >= keep) cells[tz0 + (i - keep)] = v[r];
+        }
+        __syncthreads();
+    }
+    uint16_t* out = exit_tail + (size_t)B * WIN;
+    for (uint32_t i = tid; i < WIN; i += 1024) out[i] = win[i];
+}
+
+__global__ __launch_bounds__(1024) void k_tail_entry(const uint16_t* __restrict__ exit_tail, uint32_t n_blocks,
+                                                     uint16_t* __restrict__ entry) {
+    __shared__ uint16_t ent[WIN];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < WIN; i += 1024) ent[i] = 0x100;  // before the output start
+    __syncthreads();
+    for (uint32_t B = 0; B < n_blocks; ++B) {
+        uint16_t v[WIN / 1024];
+        const uint16_t* ex = exit_tail + (size_t)B * WIN;
+        uint16_t* en = entry + (size_t)B * WIN;
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) {
+            const uint32_t i = tid + 1024u * r;
+            en[i] = ent[i];
+            uint16_t c = ex[i];
+            if (c & 0x8000u) c = ent[WIN - 1 - (c & 0x7fffu)];
+            v[r] = c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) ent[tid + 1024u * r] = v[r];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tail_patch(const ChainSeg* __restrict__ cs, uint32_t n_seg, uint32_t G,
+                                                    const uint16_t* __restrict__ entry, uint16_t* __restrict__ cells,
+                                                    uint32_t* __restrict__ err) {
+    const uint32_t s = blockIdx.x;
+    if (s >= n_seg) return;
+    const uint64_t len = cs[s].len;
+    const uint32_t tl = (uint32_t)(len < WIN ? len : WIN);
+    const uint64_t tz0 = cs[s].cells_base + len - tl;
+    const uint16_t* en = entry + (size_t)(s / G) * WIN;
+    for (uint32_t k = threadIdx.x; k < tl; k += 256) {
+        const uint16_t c = cells[tz0 + k];
+        if (c & 0x8000u) {
+            const uint16_t b = en[WIN - 1 - (c & 0x7fffu)];
+            if (b & 0x100u) atomicMin(err, s);
+            cells[tz0 + k] = (uint16_t)(b & 0xffu);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_tail_jump(uint64_t n_list, uint32_t* __restrict__ tz_state,
-                                                   const uint64_t* __restrict__ lst_self, uint64_t* lst_ptr,
-                                                   uint32_t* __restrict__ remaining) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool pending = false;
-    if (i < n_list) {
-        const uint64_t self = lst_self[i];
-        if (!(tz_state[self] & 0x80000000u)) {
-            const uint64_t p = lst_ptr[i];
-            const uint32_t sp = __atomic_load_n(&tz_state[p], __ATOMIC_RELAXED);
-            if (sp & 0x80000000u) {
-                __atomic_store_n(&tz_state[self], sp, __ATOMIC_RELAXED);
-            } else {
-                const uint64_t np = __atomic_load_n(&lst_ptr[sp], __ATOMIC_RELAXED);
-                __atomic_store_n(&lst_ptr[i], np, __ATOMIC_RELAXED);
-                pending = true;
-            }
-        }
-    }
-    const uint64_t m = __ballot(pending);
-    if (m && (threadIdx.x & 63u) == (uint32_t)__ffsll((unsigned long long)m) - 1)
-        atomicAdd(remaining, (uint32_t)__popcll(m));
-}
-
-__global__ __launch_bounds__(256) void k_tail_patch(uint64_t n_list, const uint32_t* __restrict__ tz_state,
-                                                    const uint64_t* __restrict__ lst_self,
-                                                    const uint64_t* __restrict__ lst_cell, uint16_t* __restrict__ cells) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_list) cells[lst_cell[i]] = (uint16_t)(tz_state[lst_self[i]] & 0xffu);
-}
-
-hipError_t launch_tail_init(const ChainSeg* d_cs, uint32_t n_seg, uint64_t lst_cap, const uint16_t* d_cells,
-                            uint32_t* d_tz_state, uint64_t* d_lst_self, uint64_t* d_lst_ptr, uint64_t* d_lst_cell,
-                            uint32_t* d_counters, hipStream_t s) {
+hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_exit,
+                       uint16_t* d_entry, uint32_t* d_err, hipStream_t s) {
     if (n_seg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tail_init, dim3(n_seg), dim3(256), 0, s, d_cs, n_seg, d_cells, d_tz_state, d_lst_self,
-                       d_lst_ptr, d_lst_cell, lst_cap, d_counters);
-    return hipGetLastError();
-}
-
-hipError_t launch_tail_jump(uint64_t n_list, uint32_t* d_tz_state, const uint64_t* d_lst_self, uint64_t* d_lst_ptr,
-                            uint32_t* d_remaining, hipStream_t s) {
-    if (n_list == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tail_jump, dim3((unsigned)((n_list + 255) / 256)), dim3(256), 0, s, n_list, d_tz_state,
-                       d_lst_self, d_lst_ptr, d_remaining);
-    return hipGetLastError();
-}
-
-hipError_t launch_tail_patch(uint64_t n_list, const uint32_t* d_tz_state, const uint64_t* d_lst_self,
-                             const uint64_t* d_lst_cell, uint16_t* d_cells, hipStream_t s) {
-    if (n_list == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tail_patch, dim3((unsigned)((n_list + 255) / 256)), dim3(256), 0, s, n_list, d_tz_state,
-                       d_lst_self, d_lst_cell, d_cells);
+    const uint32_t nb = (n_seg + G - 1) / G;
+    hipLaunchKernelGGL(k_tail_block, dim3(nb), dim3(1024), 0, s, d_cs, n_seg, G, d_cells, d_exit);
+    hipLaunchKernelGGL(k_tail_entry, dim3(1), dim3(1024), 0, s, d_exit, nb, d_entry);
+    hipLaunchKernelGGL(k_tail_patch, dim3(n_seg), dim3(256), 0, s, d_cs, n_seg, G, d_entry, d_cells, d_err);
     return hipGetLastError();
 }
 

@@ -25,15 +25,11 @@ hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_le
                           uint32_t* d_counter, uint16_t* d_cells, uint16_t* d_sorted, const InflateGeom& g,
                           hipStream_t s);
 
-// Tail chain by pointer jumping over the markers in every segment's last
-// 32 Ki cells (the only cells later segments can reference).
-hipError_t launch_tail_init(const ChainSeg* d_cs, uint32_t n_seg, uint64_t lst_cap, const uint16_t* d_cells,
-                            uint32_t* d_tz_state, uint64_t* d_lst_self, uint64_t* d_lst_ptr,
-                            uint64_t* d_lst_cell, uint32_t* d_counters, hipStream_t s);
-hipError_t launch_tail_jump(uint64_t n_list, uint32_t* d_tz_state, const uint64_t* d_lst_self,
-                            uint64_t* d_lst_ptr, uint32_t* d_remaining, hipStream_t s);
-hipError_t launch_tail_patch(uint64_t n_list, const uint32_t* d_tz_state, const uint64_t* d_lst_self,
-                             const uint64_t* d_lst_cell, uint16_t* d_cells, hipStream_t s);
+// Tail chain: blocked three-phase scan over blocks of G chain segments.
+// d_exit / d_entry hold ceil(n_seg/G) * 32768 u16 each; *d_err receives the
+// lowest segment whose marker reaches before the start of the output.
+hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_exit,
+                       uint16_t* d_entry, uint32_t* d_err, hipStream_t s);
 
 // Resolve markers and narrow u16 cells to u8 output.
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs,
