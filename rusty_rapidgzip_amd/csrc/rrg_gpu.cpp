@@ -499,6 +499,13 @@ int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len,
         t.stop();
         HIPCHK(hipStreamSynchronize(c.s));
     }
+    // a find is a bit offset inside the body; anything else is a device fault
+    for (size_t i = 0; i < found.size(); ++i)
+        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= nbits)) {
+            set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu",
+                    (unsigned long long)found[i], (unsigned long long)targets[i]);
+            return RRG_E_DEVICE;
+        }
     // chunk boundaries exactly as the reference (strictly increasing finds)
     std::vector<uint64_t> B{base};
     for (size_t i = 0; i < targets.size(); ++i)

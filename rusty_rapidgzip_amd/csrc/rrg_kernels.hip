@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
                     if (lane == ncand) mycand = cur;
                     ++ncand;
                 }
-                cur += (uint64_t)__builtin_amdgcn_readlane(nx, (int)rel);
+                cur += (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel);
             }
         }
         // 2. verify the visited candidates in parallel; the first that passes wins
@@ -245,8 +245,10 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
         const uint64_t vm = __ballot(ver);
         if (vm) {
             const int first = __ffsll((unsigned long long)vm) - 1;
-            result = (uint64_t)__builtin_amdgcn_readlane((int)(uint32_t)mycand, first) |
-                     ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mycand >> 32), first) << 32);
+            // readlane returns int: go through uint32_t so bit 31 does not sign-extend
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mycand, first);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mycand >> 32), first);
+            result = ((uint64_t)hi << 32) | lo;
             break;
         }
     }

@@ -170,6 +170,17 @@ def test_errors_not_faults(decoder):
     check_same(decoder, good, 65536, expect=p)
 
 
+def test_large_offsets(decoder):
+    # compressed offsets beyond 2^31 bits (256 MiB): 64-bit positions end to end
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(1100 << 20)
+    g = synth.gzip(raw, 6, piece=64 << 20)
+    assert g.size * 8 > (1 << 31) + (8 << 20)
+    _say("large", g.size)
+    check_same(decoder, g.tobytes(), 4 << 20, expect=raw.tobytes())
+
+
 def test_device_api(decoder):
     import rusty_rapidgzip_amd as rrg
     from rusty_rapidgzip_amd import hipmem, synth
