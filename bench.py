@@ -67,7 +67,7 @@ def main() -> int:
     cfg = rrg.Config(chunk_size_bytes=args.chunk_mib << 20, segment_bytes=args.segment_kib << 10,
                      lut_bits=args.lut_bits, lanes_per_block=args.lanes)
     dec = rrg.Decoder(0)
-    host = gz.tobytes()
+    host = gz  # numpy array, passed by pointer
 
     def step():
         return dec.decode_device(d_in.ptr, C, host, cfg)

@@ -324,11 +324,18 @@ class Decoder:
         out_len = ctypes.c_uint64()
         n = ctypes.c_size_t()
         st, er = _Stats(), _Err()
-        hbuf = None
+        # host copy of the file (framing walks only), passed without copying
+        hptr = None
         if host is not None:
-            hbuf = (ctypes.c_uint8 * len(host)).from_buffer_copy(host)
-        rc = self._lib.rrg_gpu_decode_device(self._ctx, ctypes.c_void_p(d_ptr), length,
-                                             ctypes.addressof(hbuf) if hbuf is not None else None,
+            if isinstance(host, bytes):
+                hptr = ctypes.cast(ctypes.c_char_p(host), ctypes.c_void_p).value
+            else:
+                import numpy as np
+
+                arr = np.ascontiguousarray(host, dtype=np.uint8)
+                host = arr  # keep alive for the call
+                hptr = arr.ctypes.data
+        rc = self._lib.rrg_gpu_decode_device(self._ctx, ctypes.c_void_p(d_ptr), length, hptr,
                                              ctypes.byref(config._c()), ctypes.byref(d_out), ctypes.byref(out_len),
                                              ends, max_chunks, ctypes.byref(n), ctypes.byref(st), ctypes.byref(er))
         if rc != RRG_OK:

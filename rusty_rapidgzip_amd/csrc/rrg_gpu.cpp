@@ -307,7 +307,7 @@ int run_inflate(Ctx& c, const Geometry& G, uint64_t nbits, uint64_t in_len, std:
                 rrg_error* err) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
-    HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * 320 * sizeof(uint16_t)));
+    HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * 480 * sizeof(uint16_t)));
     HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), c.cells.cap, c.s));
     HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
     HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));

@@ -66,14 +66,33 @@ __device__ __forceinline__ bool is_cand13(uint32_t b) {
 }
 
 // Strict verification (block_finder.rs:175-301) of a candidate at `p`.
-// `lens` is this lane's 340-byte LDS scratch, `cnt` 32 u16.
+// Bits are staged per lane in LDS (`buf`, 32 words): the first fill takes 4
+// words, which covers the header fields and the precode (where all but ~0.5% of
+// candidates on random bits stop); longer reads refill 32 words at a time so
+// the rare full-header parse waits on memory once per 2 Kbit.
 struct VerifyBits {
     const uint64_t* in64;
     uint64_t nbits;
     uint64_t pos;
+    uint64_t* buf;
+    uint64_t bw;      // first word held in buf
+    uint32_t nheld;   // words held
+    uint64_t nvalid;  // words that may be read from in64
+    __device__ __forceinline__ void fill(uint64_t w, uint32_t n) {
+        bw = w;
+        nheld = n;
+        for (uint32_t k = 0; k < n; ++k) buf[k] = (w + k < nvalid) ? in64[w + k] : 0ull;
+    }
+    __device__ __forceinline__ uint64_t peek64() {
+        const uint64_t q = pos >> 6;
+        if (q < bw || q + 1 >= bw + nheld) fill(q, 32);
+        const uint64_t lo = buf[q - bw], hi = buf[q - bw + 1];
+        const uint32_t sh = (uint32_t)(pos & 63);
+        return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    }
     __device__ __forceinline__ bool read(uint32_t n, uint32_t* v) {
         if (pos + n > nbits) return false;
-        *v = n ? (uint32_t)(bits64_at(in64, pos) & ((1ull << n) - 1)) : 0u;
+        *v = n ? (uint32_t)(peek64() & ((1ull << n) - 1)) : 0u;
         pos += n;
         return true;
     }
@@ -84,14 +103,13 @@ struct VerifyBits {
 // LUT probe, 15 for codes longer than 10.
 __device__ int verify_decode(VerifyBits& b, const uint8_t* lens, int n, const uint16_t* cnt) {
     if (b.pos + 10 > b.nbits) return -1;
-    const uint64_t w = bits64_at(b.in64, b.pos);
+    const uint64_t w = b.peek64();
     int code = 0, first = 0, index = 0;
     for (int l = 1; l <= 15; ++l) {
         code |= (int)((w >> (l - 1)) & 1u);
         const int count = cnt[l];
         if (code - count < first) {
             if (l > 10 && b.pos + 15 > b.nbits) return -1;
-            // symbol = the (code-first)-th symbol of length l in symbol order
             int want = code - first;  // rank within length l
             for (int s = 0; s < n; ++s) {
                 if (lens[s] == l) {
@@ -126,8 +144,10 @@ __device__ bool verify_kraft(const uint8_t* lens, int n, uint16_t* cnt) {
     return true;
 }
 
-__device__ bool verify_candidate(const uint64_t* in64, uint64_t nbits, uint64_t p, uint8_t* lens, uint16_t* cnt) {
-    VerifyBits b{in64, nbits, p};
+__device__ bool verify_candidate(const uint64_t* in64, uint64_t nbits, uint64_t p, uint8_t* lens, uint16_t* cnt,
+                                 uint64_t* bitbuf) {
+    VerifyBits b{in64, nbits, p, bitbuf, 0, 0, (nbits >> 6) + 2};
+    b.fill(p >> 6, 4);
     uint32_t v;
     if (!b.read(3, &v)) return false;
     if (!b.read(5, &v)) return false;
@@ -190,12 +210,23 @@ __device__ bool verify_candidate(const uint64_t* in64, uint64_t nbits, uint64_t 
     return true;
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One wavefront per target.  The wave keeps a 4096-bit region of the stream
+// in registers (lane k holds word rbase+k) and walks the reference's chain
+// 64 offsets at a time; visited candidates (up to 64, lane k holds the k-th)
+// are then verified in parallel and the first that passes is the answer.
 __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t nbits,
                                                     const uint64_t* __restrict__ targets,
                                                     uint64_t* __restrict__ found, uint32_t n_targets) {
     __shared__ uint8_t s_next[8192];
     __shared__ uint8_t s_lens[64 * 340];
     __shared__ uint16_t s_cnt[64 * 32];
+    __shared__ uint64_t s_bits[64 * 32];
     const int lane = (int)threadIdx.x;
     for (int i = lane; i < 8192; i += 64) {
         uint32_t w = (uint32_t)i, step = 0;
@@ -210,19 +241,32 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
     if (t >= n_targets) return;
     uint64_t cur = targets[t];
     const uint64_t limit = nbits;
+    const uint64_t nvalid = (nbits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
     uint8_t* lens = s_lens + lane * 340;
     uint16_t* cnt = s_cnt + lane * 32;
+    uint64_t* vbuf = s_bits + lane * 32;
+    uint64_t rbase = cur >> 6;
+    uint64_t rw = (rbase + lane < nvalid) ? in64[rbase + lane] : 0ull;
     bool end = (cur + 13 > limit);
     while (!end) {
-        // 1. walk the chain, collecting up to 64 visited candidates (lane k holds the k-th)
+        // 1. walk the chain, collecting up to 64 visited candidates
         uint64_t mycand = 0;
         int ncand = 0;
         while (ncand < 64 && !end) {
             const uint64_t base = cur;
+            if ((base >> 6) + 2 >= rbase + 64) {
+                rbase = base >> 6;
+                rw = (rbase + lane < nvalid) ? in64[rbase + lane] : 0ull;
+            }
+            const int j = (int)((base >> 6) - rbase);
+            const uint64_t w0 = readlane64(rw, j), w1 = readlane64(rw, j + 1), w2 = readlane64(rw, j + 2);
+            const uint32_t sh = (uint32_t)(base & 63) + (uint32_t)lane;  // 0..126
+            uint64_t v;
+            if (sh < 64) v = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+            else v = (sh > 64) ? (w1 >> (sh - 64)) | (w2 << (128 - sh)) : w1;
             const uint64_t p = base + (uint64_t)lane;
-            uint32_t step = 255;
-            if (p + 13 <= limit) step = s_next[(uint32_t)(bits64_at(in64, p) & 0x1fffu)];
+            const uint32_t step = (p + 13 <= limit) ? (uint32_t)s_next[(uint32_t)(v & 0x1fffu)] : 255u;
             const uint64_t cm = __ballot(step == 0);
             const int nx = (step == 0) ? 1 : (int)step;
             while (ncand < 64) {
@@ -241,14 +285,10 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
         }
         // 2. verify the visited candidates in parallel; the first that passes wins
         bool ver = false;
-        if (lane < ncand) ver = verify_candidate(in64, nbits, mycand, lens, cnt);
+        if (lane < ncand) ver = verify_candidate(in64, nbits, mycand, lens, cnt, vbuf);
         const uint64_t vm = __ballot(ver);
         if (vm) {
-            const int first = __ffsll((unsigned long long)vm) - 1;
-            // readlane returns int: go through uint32_t so bit 31 does not sign-extend
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mycand, first);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mycand >> 32), first);
-            result = ((uint64_t)hi << 32) | lo;
+            result = readlane64(mycand, __ffsll((unsigned long long)vm) - 1);
             break;
         }
     }
@@ -284,8 +324,10 @@ constexpr int DIST_LB = 7;
 template <int LB>
 struct LaneTables {
     static constexpr int kLit = 1 << LB;
-    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) * 2 + 320 + 5 * 32;
+    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) * 2 + 5 * 32;
 };
+// per-lane global scratch, in u16: 320 length-sorted symbols + 160 (= 320 B of code lengths)
+constexpr int LANE_SCRATCH_U16 = 480;
 
 int inflate_lds_per_lane(int lut_bits) {
     switch (lut_bits) {
@@ -398,14 +440,16 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     uint8_t* my = lds + tid * LaneTables<LB>::kBytes;
     uint16_t* litlut = reinterpret_cast<uint16_t*>(my);
     uint16_t* distlut = litlut + (1 << LB);
-    uint8_t* lens = reinterpret_cast<uint8_t*>(distlut + (1 << DIST_LB));
-    uint16_t* cntL = reinterpret_cast<uint16_t*>(lens + 320);
+    uint16_t* cntL = distlut + (1 << DIST_LB);
     uint16_t* cntD = cntL + 16;
     uint16_t* tnext = cntD + 16;
     uint16_t* toffs = tnext + 16;
     uint16_t* cltmp = toffs + 16;  // 16 u16 = 32 B: precode lengths (19 B) while a header is read
-    uint16_t* sortedL = sorted_scratch + ((size_t)blockIdx.x * blockDim.x + tid) * 320;
+    // code lengths and length-sorted symbols live in per-lane global scratch: they
+    // are touched once per block header, not per symbol
+    uint16_t* sortedL = sorted_scratch + ((size_t)blockIdx.x * blockDim.x + tid) * LANE_SCRATCH_U16;
     uint16_t* sortedD = sortedL + 288;
+    uint8_t* lens = reinterpret_cast<uint8_t*>(sortedL + 320);
 
     int state = S_FETCH;
     uint32_t seg = 0;
@@ -589,18 +633,18 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 const int hlit = (int)a + 257, hdist = (int)b2 + 1, hclen = (int)c2 + 4;
                 if (hlit > 286 || hdist > 30) RRG_FAIL(ST_INVALID, DET_HLIT_HDIST);
                 uint8_t* cl = reinterpret_cast<uint8_t*>(cltmp);
-                for (int i = 0; i < 19; ++i) lens[i] = 0;
+                for (int i = 0; i < 19; ++i) cl[i] = 0;
                 for (int i = 0; i < hclen; ++i) {
                     uint32_t x;
                     RRG_GETBITS(x, 3);
-                    lens[c_cl_order[i]] = (uint8_t)x;
+                    cl[c_cl_order[i]] = (uint8_t)x;
                 }
-                for (int i = 0; i < 19; ++i) cl[i] = lens[i];
                 // precode LUT (7 bits) lives in the distance LUT while lengths are read
                 if (!build_table(cl, 19, distlut, DIST_LB, cntD, tnext, toffs, nullptr))
                     RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
                 const int total = hlit + hdist;
                 int i = 0;
+                uint8_t last = 0;  // previous code length (for code 16)
                 while (i < total) {
                     if (pos + 10 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                     if (bitcnt < 32) RRG_REFILL();
@@ -613,15 +657,15 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     RRG_CONSUME(l);
                     const uint32_t sym = e >> 4;
                     if (sym <= 15) {
-                        lens[i++] = (uint8_t)sym;
+                        last = (uint8_t)sym;
+                        lens[i++] = last;
                     } else if (sym == 16) {
                         if (i == 0) RRG_FAIL(ST_INVALID, DET_CODE16_FIRST);
                         uint32_t x;
                         RRG_GETBITS(x, 2);
                         const int n = (int)x + 3;
                         if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE16_OVERRUN);
-                        const uint8_t prev = lens[i - 1];
-                        for (int j = 0; j < n; ++j) lens[i + j] = prev;
+                        for (int j = 0; j < n; ++j) lens[i + j] = last;
                         i += n;
                     } else if (sym == 17) {
                         uint32_t x;
@@ -629,6 +673,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                         const int n = (int)x + 3;
                         if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE17_OVERRUN);
                         for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                        last = 0;
                         i += n;
                     } else {
                         uint32_t x;
@@ -636,6 +681,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                         const int n = (int)x + 11;
                         if (i + n > total) RRG_FAIL(ST_INVALID, DET_CODE18_OVERRUN);
                         for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                        last = 0;
                         i += n;
                     }
                 }
