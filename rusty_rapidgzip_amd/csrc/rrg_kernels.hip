@@ -65,62 +65,59 @@ __device__ __forceinline__ bool is_cand13(uint32_t b) {
     return !(b & 1u) && ((b >> 1) & 3u) == 2u && ((b >> 3) & 31u) <= 29u && ((b >> 8) & 31u) <= 29u;
 }
 
-// Strict verification (block_finder.rs:175-301) of a candidate at `p`.
-// Bits are staged per lane in LDS (`buf`, 32 words): the first fill takes 4
-// words, which covers the header fields and the precode (where all but ~0.5% of
-// candidates on random bits stop); longer reads refill 32 words at a time so
-// the rare full-header parse waits on memory once per 2 Kbit.
-struct VerifyBits {
-    const uint64_t* in64;
-    uint64_t nbits;
-    uint64_t pos;
-    uint64_t* buf;
-    uint64_t bw;      // first word held in buf
-    uint32_t nheld;   // words held
-    uint64_t nvalid;  // words that may be read from in64
-    __device__ __forceinline__ void fill(uint64_t w, uint32_t n) {
-        bw = w;
-        nheld = n;
-        for (uint32_t k = 0; k < n; ++k) buf[k] = (w + k < nvalid) ? in64[w + k] : 0ull;
-    }
-    __device__ __forceinline__ uint64_t peek64() {
-        const uint64_t q = pos >> 6;
-        if (q < bw || q + 1 >= bw + nheld) fill(q, 32);
-        const uint64_t lo = buf[q - bw], hi = buf[q - bw + 1];
-        const uint32_t sh = (uint32_t)(pos & 63);
-        return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
-    }
-    __device__ __forceinline__ bool read(uint32_t n, uint32_t* v) {
-        if (pos + n > nbits) return false;
-        *v = n ? (uint32_t)(peek64() & ((1ull << n) - 1)) : 0u;
-        pos += n;
-        return true;
-    }
+// ---------------------------------------------------------------------------
+// Finder layout: one wavefront per target, FIND_WAVES wavefronts per workgroup
+// sharing the NEXT_LUT.  Each wave keeps a 66-word (4224-bit) region of the
+// stream in LDS.  Per 64-offset window every lane evaluates NEXT_LUT and, at
+// a jump of 0, the cheap precode test (header fields + strict Kraft over the
+// 19 precode lengths); the wave then follows the reference's chain and, at a
+// visited offset that passed the cheap test, runs the full strict header parse
+// and the 64-symbol trial decode as wave-uniform code.  The first offset that
+// passes everything is the answer -- the same order as the reference.
+// ---------------------------------------------------------------------------
+constexpr int FIND_WAVES = 4;
+constexpr int REGION_WORDS = 66;
+
+struct FindWave {
+    uint64_t reg[REGION_WORDS];
+    uint8_t lens[320];
+    uint8_t clut[128];     // precode LUT: (symbol << 3) | length
+    uint16_t cnt[32];      // code-length counts: literal/length [0..16), distance [16..32)
+    uint16_t sorted[320];  // length-sorted symbols: literal/length [0..288), distance [288..320)
 };
 
-// canonical decode (RFC 1951 3.2.2) over lengths lens[0..n): returns symbol or
-// -1.  Follows the reference's lookup rules: 10 bits must be available for the
-// LUT probe, 15 for codes longer than 10.
-__device__ int verify_decode(VerifyBits& b, const uint8_t* lens, int n, const uint16_t* cnt) {
-    if (b.pos + 10 > b.nbits) return -1;
-    const uint64_t w = b.peek64();
+// 64 bits at stream position `pos` for wave-uniform code: from the LDS region
+// when it holds them, else from global memory.
+__device__ __forceinline__ uint64_t ubits(const uint64_t* __restrict__ in64, const FindWave& fw, uint64_t rbase,
+                                          uint64_t pos) {
+    const uint64_t q = pos >> 6;
+    uint64_t lo, hi;
+    if (q >= rbase && q + 1 < rbase + REGION_WORDS) {
+        lo = fw.reg[q - rbase];
+        hi = fw.reg[q - rbase + 1];
+    } else {
+        lo = in64[q];
+        hi = in64[q + 1];
+    }
+    const uint32_t sh = (uint32_t)(pos & 63);
+    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+}
+
+// Canonical decode over a length-sorted symbol table (RFC 1951 3.2.2), with
+// the reference's lookup rules (10 bits available for the LUT probe, 15 for
+// codes longer than 10).  Returns the symbol or -1.
+__device__ int ucanon(const uint64_t* __restrict__ in64, const FindWave& fw, uint64_t rbase, uint64_t nbits,
+                      uint64_t& pos, const uint16_t* cnt, const uint16_t* sorted) {
+    if (pos + 10 > nbits) return -1;
+    const uint64_t w = ubits(in64, fw, rbase, pos);
     int code = 0, first = 0, index = 0;
     for (int l = 1; l <= 15; ++l) {
         code |= (int)((w >> (l - 1)) & 1u);
         const int count = cnt[l];
         if (code - count < first) {
-            if (l > 10 && b.pos + 15 > b.nbits) return -1;
-            int want = code - first;  // rank within length l
-            for (int s = 0; s < n; ++s) {
-                if (lens[s] == l) {
-                    if (want == 0) {
-                        b.pos += (uint64_t)l;
-                        return s;
-                    }
-                    --want;
-                }
-            }
-            return -1;
+            if (l > 10 && pos + 15 > nbits) return -1;
+            pos += (uint64_t)l;
+            return sorted[index + (code - first)];
         }
         index += count;
         first += count;
@@ -130,105 +127,161 @@ __device__ int verify_decode(VerifyBits& b, const uint8_t* lens, int n, const ui
     return -1;
 }
 
-// Kraft check over lens[0..n) at 15 bits; strict: complete when >= 2 symbols.
-__device__ bool verify_kraft(const uint8_t* lens, int n, uint16_t* cnt) {
+// counts + length-sorted symbols of lens[0..n) into cnt[0..16) / sorted[0..n)
+__device__ void sort_lengths(const uint8_t* lens, int n, uint16_t* cnt, uint16_t* sorted) {
     for (int l = 0; l < 16; ++l) cnt[l] = 0;
     for (int s = 0; s < n; ++s) cnt[lens[s]]++;
     cnt[0] = 0;
-    uint32_t total = 0, nsym = 0;
-    for (int l = 1; l <= 15; ++l) {
-        total += (uint32_t)cnt[l] << (15 - l);
-        nsym += cnt[l];
-    }
-    if (nsym >= 2 && total != 32768u) return false;
-    return true;
+    uint16_t off[16];
+    off[1] = 0;
+    for (int l = 1; l < 15; ++l) off[l + 1] = (uint16_t)(off[l] + cnt[l]);
+    for (int s = 0; s < n; ++s)
+        if (lens[s]) sorted[off[lens[s]]++] = (uint16_t)s;
 }
 
-__device__ bool verify_candidate(const uint64_t* in64, uint64_t nbits, uint64_t p, uint8_t* lens, uint16_t* cnt,
-                                 uint64_t* bitbuf) {
-    VerifyBits b{in64, nbits, p, bitbuf, 0, 0, (nbits >> 6) + 2};
-    b.fill(p >> 6, 4);
-    uint32_t v;
-    if (!b.read(3, &v)) return false;
-    if (!b.read(5, &v)) return false;
-    const int hlit = (int)v + 257;
-    if (!b.read(5, &v)) return false;
-    const int hdist = (int)v + 1;
-    if (!b.read(4, &v)) return false;
-    const int hclen = (int)v + 4;
-    if (hlit > 286 || hdist > 30) return false;
-    // precode lengths live in lens[320..339) while the litlen/dist lengths fill lens[0..316)
-    uint8_t* cl = lens + 320;
-    for (int i = 0; i < 19; ++i) cl[i] = 0;
-    for (int i = 0; i < hclen; ++i) {
-        if (!b.read(3, &v)) return false;
-        cl[c_cl_order[i]] = (uint8_t)v;
-    }
-    if (!verify_kraft(cl, 19, cnt)) return false;
-    const int total = hlit + hdist;
-    int i = 0;
-    while (i < total) {
-        const int sym = verify_decode(b, cl, 19, cnt);
-        if (sym < 0) return false;
-        if (sym <= 15) {
-            lens[i++] = (uint8_t)sym;
-        } else if (sym == 16) {
-            if (i == 0) return false;
-            if (!b.read(2, &v)) return false;
-            const int n = (int)v + 3;
-            if (i + n > total) return false;
-            const uint8_t prev = lens[i - 1];
-            for (int j = 0; j < n; ++j) lens[i + j] = prev;
-            i += n;
-        } else if (sym == 17 || sym == 18) {
-            if (!b.read(sym == 17 ? 3 : 7, &v)) return false;
-            const int n = (int)v + (sym == 17 ? 3 : 11);
-            if (i + n > total) return false;
-            for (int j = 0; j < n; ++j) lens[i + j] = 0;
-            i += n;
-        } else {
-            return false;
+// Cheap stage (block_finder.rs:196-213): header fields + strict precode Kraft.
+// `lo`/`hi` hold the 128 bits at p.
+__device__ __forceinline__ bool precode_ok(uint64_t lo, uint64_t hi, uint64_t p, uint64_t nbits) {
+    const uint32_t hclen = 4u + (uint32_t)((lo >> 13) & 15u);
+    if (p + 17 + 3ull * hclen > nbits) return false;
+    uint32_t sum = 0, nsym = 0;
+    for (uint32_t i = 0; i < hclen; ++i) {
+        const uint32_t at = 17 + 3 * i;
+        const uint32_t l = (uint32_t)((at < 62 ? lo >> at : at >= 64 ? hi >> (at - 64) : (lo >> at) | (hi << (64 - at))) & 7u);
+        if (l) {
+            sum += 1u << (7 - l);
+            ++nsym;
         }
     }
-    if (lens[256] == 0) return false;
-    // lens[0..hlit) literal/length, lens[hlit..total) distance; both strict.
-    uint16_t* cntL = cnt + 16;
-    if (!verify_kraft(lens, hlit, cntL)) return false;
-    if (!verify_kraft(lens + hlit, hdist, cnt)) return false;
+    return nsym == 1 || (nsym >= 2 && sum == 128u);
+}
+
+// Full strict verification (block_finder.rs:175-301) of `p` as wave-uniform
+// code; the lanes cooperate only to build the precode LUT.
+__device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
+                            uint64_t p, int lane) {
+    const uint64_t w0 = ubits(in64, fw, rbase, p);
+    const uint32_t hlit = 257u + (uint32_t)((w0 >> 3) & 31u);
+    const uint32_t hdist = 1u + (uint32_t)((w0 >> 8) & 31u);
+    const uint32_t hclen = 4u + (uint32_t)((w0 >> 13) & 15u);
+    // precode lengths (in symbol order) and canonical codes; every lane builds
+    // the two LUT entries lane and lane+64
+    uint8_t cl[19];
+    for (int i = 0; i < 19; ++i) cl[i] = 0;
+    const uint64_t w1 = ubits(in64, fw, rbase, p + 17);
+    for (uint32_t i = 0; i < hclen; ++i) cl[c_cl_order[i]] = (uint8_t)((w1 >> (3 * i)) & 7u);
+    {
+        uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 19; ++i) cnt[cl[i]]++;
+        cnt[0] = 0;
+        uint32_t next[8];
+        uint32_t code = 0;
+        for (int l = 1; l < 8; ++l) {
+            code = (code + cnt[l - 1]) << 1;
+            next[l] = code;
+        }
+        uint8_t e0 = 0, e1 = 0;
+        for (int sy = 0; sy < 19; ++sy) {
+            const uint32_t l = cl[sy];
+            if (!l) continue;
+            const uint32_t rev = __brev(next[l]++) >> (32 - l);
+            const uint32_t m = (1u << l) - 1;
+            if (((uint32_t)lane & m) == rev) e0 = (uint8_t)((sy << 3) | l);
+            if (((uint32_t)(lane + 64) & m) == rev) e1 = (uint8_t)((sy << 3) | l);
+        }
+        fw.clut[lane] = e0;
+        fw.clut[lane + 64] = e1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint64_t pos = p + 17 + 3ull * hclen;
+    const uint32_t total = hlit + hdist;
+    uint32_t i = 0, last = 0;
+    uint32_t sumL = 0, nL = 0, sumD = 0, nD = 0;
+    auto put = [&](uint32_t L, uint32_t n) -> bool {
+        for (uint32_t k = 0; k < n; ++k, ++i) {
+            if (lane == 0) fw.lens[i] = (uint8_t)L;
+            if (L) {
+                if (i < hlit) {
+                    sumL += 1u << (15 - L);
+                    ++nL;
+                } else {
+                    sumD += 1u << (15 - L);
+                    ++nD;
+                }
+            }
+        }
+        // oversubscription only grows: reject as soon as it appears
+        return !((nL >= 2 && sumL > 32768u) || (nD >= 2 && sumD > 32768u));
+    };
+    while (i < total) {
+        if (pos + 10 > nbits) return false;
+        const uint64_t w = ubits(in64, fw, rbase, pos);
+        const uint32_t e = fw.clut[(uint32_t)(w & 127u)];
+        const uint32_t l = e & 7u;
+        if (!l) return false;
+        pos += l;
+        const uint32_t sym = e >> 3;
+        const uint64_t x = w >> l;
+        if (sym <= 15) {
+            last = sym;
+            if (!put(sym, 1)) return false;
+        } else if (sym == 16) {
+            if (i == 0 || pos + 2 > nbits) return false;
+            const uint32_t n = 3u + (uint32_t)(x & 3u);
+            pos += 2;
+            if (i + n > total || !put(last, n)) return false;
+        } else if (sym == 17) {
+            if (pos + 3 > nbits) return false;
+            const uint32_t n = 3u + (uint32_t)(x & 7u);
+            pos += 3;
+            if (i + n > total) return false;
+            put(0, n);
+            last = 0;
+        } else {
+            if (pos + 7 > nbits) return false;
+            const uint32_t n = 11u + (uint32_t)(x & 127u);
+            pos += 7;
+            if (i + n > total) return false;
+            put(0, n);
+            last = 0;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (fw.lens[256] == 0) return false;
+    if ((nL >= 2 && sumL != 32768u) || (nD >= 2 && sumD != 32768u)) return false;
     // trial decode of 64 symbols (block_finder.rs:268-301)
+    if (lane == 0) {
+        sort_lengths(fw.lens, (int)hlit, fw.cnt, fw.sorted);
+        sort_lengths(fw.lens + hlit, (int)hdist, fw.cnt + 16, fw.sorted + 288);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int k = 0; k < 64; ++k) {
-        const int sym = verify_decode(b, lens, hlit, cntL);
+        const int sym = ucanon(in64, fw, rbase, nbits, pos, fw.cnt, fw.sorted);
         if (sym < 0) return false;
         if (sym <= 255) continue;
         if (sym == 256) return true;
         if (sym > 285) return false;
-        if (!b.read(len_extra((uint32_t)sym - 257), &v)) return false;
-        const int ds = verify_decode(b, lens + hlit, hdist, cnt);
+        const uint32_t lx = len_extra((uint32_t)sym - 257);
+        if (pos + lx > nbits) return false;
+        pos += lx;
+        const int ds = ucanon(in64, fw, rbase, nbits, pos, fw.cnt + 16, fw.sorted + 288);
         if (ds < 0 || ds >= 30) return false;
-        if (!b.read(dist_extra((uint32_t)ds), &v)) return false;
+        const uint32_t dx = dist_extra((uint32_t)ds);
+        if (pos + dx > nbits) return false;
+        pos += dx;
     }
     return true;
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// One wavefront per target.  The wave keeps a 4096-bit region of the stream
-// in registers (lane k holds word rbase+k) and walks the reference's chain
-// 64 offsets at a time; visited candidates (up to 64, lane k holds the k-th)
-// are then verified in parallel and the first that passes is the answer.
-__global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t nbits,
-                                                    const uint64_t* __restrict__ targets,
-                                                    uint64_t* __restrict__ found, uint32_t n_targets) {
+__global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t nbits,
+                                                                const uint64_t* __restrict__ targets,
+                                                                uint64_t* __restrict__ found, uint32_t n_targets) {
     __shared__ uint8_t s_next[8192];
-    __shared__ uint8_t s_lens[64 * 340];
-    __shared__ uint16_t s_cnt[64 * 32];
-    __shared__ uint64_t s_bits[64 * 32];
-    const int lane = (int)threadIdx.x;
-    for (int i = lane; i < 8192; i += 64) {
+    __shared__ FindWave s_wave[FIND_WAVES];
+    for (int i = (int)threadIdx.x; i < 8192; i += 64 * FIND_WAVES) {
         uint32_t w = (uint32_t)i, step = 0;
         while (step < 13 && !is_cand13(w)) {
             w >>= 1;
@@ -237,59 +290,54 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
         s_next[i] = (uint8_t)step;
     }
     __syncthreads();
-    const uint32_t t = blockIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t t = blockIdx.x * FIND_WAVES + (threadIdx.x >> 6);
     if (t >= n_targets) return;
+    FindWave& fw = s_wave[threadIdx.x >> 6];
     uint64_t cur = targets[t];
     const uint64_t limit = nbits;
     const uint64_t nvalid = (nbits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
-    uint8_t* lens = s_lens + lane * 340;
-    uint16_t* cnt = s_cnt + lane * 32;
-    uint64_t* vbuf = s_bits + lane * 32;
-    uint64_t rbase = cur >> 6;
-    uint64_t rw = (rbase + lane < nvalid) ? in64[rbase + lane] : 0ull;
+    uint64_t rbase = ~0ull;
     bool end = (cur + 13 > limit);
     while (!end) {
-        // 1. walk the chain, collecting up to 64 visited candidates
-        uint64_t mycand = 0;
-        int ncand = 0;
-        while (ncand < 64 && !end) {
-            const uint64_t base = cur;
-            if ((base >> 6) + 2 >= rbase + 64) {
-                rbase = base >> 6;
-                rw = (rbase + lane < nvalid) ? in64[rbase + lane] : 0ull;
+        const uint64_t base = cur;
+        // the window needs bits [base, base + 63 + 128)
+        if (rbase == ~0ull || (base >> 6) + 3 >= rbase + REGION_WORDS) {
+            rbase = base >> 6;
+            for (int k = lane; k < REGION_WORDS; k += 64) fw.reg[k] = (rbase + k < nvalid) ? in64[rbase + k] : 0ull;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        const uint64_t p = base + (uint64_t)lane;
+        const uint32_t wi = (uint32_t)((p >> 6) - rbase);
+        const uint32_t sh = (uint32_t)(p & 63);
+        const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
+        const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
+        const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
+        uint32_t step = 255;
+        bool cand = false;
+        if (p + 13 <= limit) {
+            step = s_next[(uint32_t)(lo & 0x1fffu)];
+            if (step == 0) cand = precode_ok(lo, hi, p, nbits);
+        }
+        const uint64_t cm = __ballot(cand);
+        const int nx = (step == 0) ? 1 : (int)step;
+        while (true) {
+            const uint64_t rel = cur - base;
+            if (rel >= 64) break;
+            if (cur + 13 > limit) {
+                end = true;
+                break;
             }
-            const int j = (int)((base >> 6) - rbase);
-            const uint64_t w0 = readlane64(rw, j), w1 = readlane64(rw, j + 1), w2 = readlane64(rw, j + 2);
-            const uint32_t sh = (uint32_t)(base & 63) + (uint32_t)lane;  // 0..126
-            uint64_t v;
-            if (sh < 64) v = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
-            else v = (sh > 64) ? (w1 >> (sh - 64)) | (w2 << (128 - sh)) : w1;
-            const uint64_t p = base + (uint64_t)lane;
-            const uint32_t step = (p + 13 <= limit) ? (uint32_t)s_next[(uint32_t)(v & 0x1fffu)] : 255u;
-            const uint64_t cm = __ballot(step == 0);
-            const int nx = (step == 0) ? 1 : (int)step;
-            while (ncand < 64) {
-                const uint64_t rel = cur - base;
-                if (rel >= 64) break;
-                if (cur + 13 > limit) {
+            if ((cm >> rel) & 1ull) {
+                if (deep_verify(in64, fw, rbase, nbits, cur, lane)) {
+                    result = cur;
                     end = true;
                     break;
                 }
-                if ((cm >> rel) & 1ull) {
-                    if (lane == ncand) mycand = cur;
-                    ++ncand;
-                }
-                cur += (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel);
             }
-        }
-        // 2. verify the visited candidates in parallel; the first that passes wins
-        bool ver = false;
-        if (lane < ncand) ver = verify_candidate(in64, nbits, mycand, lens, cnt, vbuf);
-        const uint64_t vm = __ballot(ver);
-        if (vm) {
-            result = readlane64(mycand, __ffsll((unsigned long long)vm) - 1);
-            break;
+            cur += (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel);
         }
     }
     if (lane == 0) found[t] = result;
@@ -298,8 +346,8 @@ __global__ __launch_bounds__(64) void k_find_blocks(const uint64_t* __restrict__
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t nbits, const uint64_t* d_targets, uint64_t* d_found,
                               uint32_t n_targets, hipStream_t s) {
     if (n_targets == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_find_blocks, dim3(n_targets), dim3(64), 0, s, d_in64, nbits, d_targets, d_found,
-                       n_targets);
+    hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
+                       d_in64, nbits, d_targets, d_found, n_targets);
     return hipGetLastError();
 }
 
