@@ -76,6 +76,13 @@ __device__ __forceinline__ bool is_cand13(uint32_t b) {
 // passes everything is the answer -- the same order as the reference.
 // ---------------------------------------------------------------------------
 constexpr int FIND_WAVES = 4;
+
+// Values every lane holds identically, re-declared as wave-uniform so the
+// compiler keeps the chain walk in scalar registers with scalar branches.
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
 constexpr int REGION_WORDS = 66;
 
 struct FindWave {
@@ -100,7 +107,7 @@ __device__ __forceinline__ uint64_t ubits(const uint64_t* __restrict__ in64, con
         hi = in64[q + 1];
     }
     const uint32_t sh = (uint32_t)(pos & 63);
-    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    return uni64(sh ? (lo >> sh) | (hi << (64 - sh)) : lo);
 }
 
 // Canonical decode over a length-sorted symbol table (RFC 1951 3.2.2), with
@@ -113,11 +120,11 @@ __device__ int ucanon(const uint64_t* __restrict__ in64, const FindWave& fw, uin
     int code = 0, first = 0, index = 0;
     for (int l = 1; l <= 15; ++l) {
         code |= (int)((w >> (l - 1)) & 1u);
-        const int count = cnt[l];
+        const int count = (int)uni32(cnt[l]);
         if (code - count < first) {
             if (l > 10 && pos + 15 > nbits) return -1;
             pos += (uint64_t)l;
-            return sorted[index + (code - first)];
+            return (int)uni32(sorted[index + (code - first)]);
         }
         index += count;
         first += count;
@@ -140,18 +147,18 @@ __device__ void sort_lengths(const uint8_t* lens, int n, uint16_t* cnt, uint16_t
 }
 
 // Cheap stage (block_finder.rs:196-213): header fields + strict precode Kraft.
-// `lo`/`hi` hold the 128 bits at p.
+// `lo`/`hi` hold the 128 bits at p; branch-free over the 19 possible fields.
 __device__ __forceinline__ bool precode_ok(uint64_t lo, uint64_t hi, uint64_t p, uint64_t nbits) {
     const uint32_t hclen = 4u + (uint32_t)((lo >> 13) & 15u);
     if (p + 17 + 3ull * hclen > nbits) return false;
+    const uint64_t pc = (lo >> 17) | (hi << 47);  // precode lengths, 3 bits each
     uint32_t sum = 0, nsym = 0;
-    for (uint32_t i = 0; i < hclen; ++i) {
-        const uint32_t at = 17 + 3 * i;
-        const uint32_t l = (uint32_t)((at < 62 ? lo >> at : at >= 64 ? hi >> (at - 64) : (lo >> at) | (hi << (64 - at))) & 7u);
-        if (l) {
-            sum += 1u << (7 - l);
-            ++nsym;
-        }
+#pragma unroll
+    for (uint32_t i = 0; i < 19; ++i) {
+        uint32_t l = (uint32_t)(pc >> (3 * i)) & 7u;
+        l = i < hclen ? l : 0u;
+        sum += (128u >> l) & (0u - (uint32_t)(l != 0));
+        nsym += (uint32_t)(l != 0);
     }
     return nsym == 1 || (nsym >= 2 && sum == 128u);
 }
@@ -217,7 +224,7 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     while (i < total) {
         if (pos + 10 > nbits) return false;
         const uint64_t w = ubits(in64, fw, rbase, pos);
-        const uint32_t e = fw.clut[(uint32_t)(w & 127u)];
+        const uint32_t e = uni32(fw.clut[(uint32_t)(w & 127u)]);
         const uint32_t l = e & 7u;
         if (!l) return false;
         pos += l;
@@ -249,7 +256,7 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (fw.lens[256] == 0) return false;
+    if (uni32(fw.lens[256]) == 0) return false;
     if ((nL >= 2 && sumL != 32768u) || (nD >= 2 && sumD != 32768u)) return false;
     // trial decode of 64 symbols (block_finder.rs:268-301)
     if (lane == 0) {
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     const uint32_t t = blockIdx.x * FIND_WAVES + (threadIdx.x >> 6);
     if (t >= n_targets) return;
     FindWave& fw = s_wave[threadIdx.x >> 6];
-    uint64_t cur = targets[t];
+    uint64_t cur = uni64(targets[t]);
     const uint64_t limit = nbits;
     const uint64_t nvalid = (nbits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
@@ -331,13 +338,13 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
                 break;
             }
             if ((cm >> rel) & 1ull) {
-                if (deep_verify(in64, fw, rbase, nbits, cur, lane)) {
+                if (uni32(deep_verify(in64, fw, rbase, nbits, cur, lane) ? 1u : 0u)) {
                     result = cur;
                     end = true;
                     break;
                 }
             }
-            cur += (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel);
+            cur = uni64(cur + (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel));
         }
     }
     if (lane == 0) found[t] = result;
