@@ -57,11 +57,13 @@ _synth = None
 
 
 def gpu_lib() -> ctypes.CDLL:
+    """The decoder library (RRG_GPU_LIB overrides the in-tree path)."""
     global _gpu
     if _gpu is None:
-        if not os.path.exists(GPU_LIB):
-            raise RuntimeError(f"{GPU_LIB} is not built: run __graft_entry__.build()")
-        _gpu = ctypes.CDLL(GPU_LIB)
+        path = os.environ.get("RRG_GPU_LIB", GPU_LIB)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+        _gpu = ctypes.CDLL(path)
     return _gpu
 
 

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     bool end = (cur + 13 > limit);
     while (!end) {
         const uint64_t base = cur;
+        if (base + 13 > limit) break;  // the reference's `while pos + 13 <= limit`
         // the window needs bits [base, base + 63 + 128)
         if (rbase == ~0ull || (base >> 6) + 3 >= rbase + REGION_WORDS) {
             rbase = base >> 6;
@@ -330,22 +331,25 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         }
         const uint64_t cm = __ballot(cand);
         const int nx = (step == 0) ? 1 : (int)step;
-        while (true) {
-            const uint64_t rel = cur - base;
-            if (rel >= 64) break;
-            if (cur + 13 > limit) {
+        // chain walk in window-relative 32-bit offsets: scalar compares only
+        const uint64_t room = limit - 13 - base;  // base + 13 <= limit here
+        const uint32_t rel_max = room < 64 ? (uint32_t)room : 64u;
+        uint32_t rel = 0;
+        while (rel < 64) {
+            if (rel > rel_max) {
                 end = true;
                 break;
             }
             if ((cm >> rel) & 1ull) {
-                if (uni32(deep_verify(in64, fw, rbase, nbits, cur, lane) ? 1u : 0u)) {
-                    result = cur;
+                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
+                    result = base + rel;
                     end = true;
                     break;
                 }
             }
-            cur = uni64(cur + (uint64_t)(uint32_t)__builtin_amdgcn_readlane(nx, (int)rel));
+            rel = uni32(rel + (uint32_t)__builtin_amdgcn_readlane(nx, (int)rel));
         }
+        cur = base + rel;
     }
     if (lane == 0) found[t] = result;
 }
