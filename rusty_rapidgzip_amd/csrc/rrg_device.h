@@ -15,6 +15,7 @@ struct SegDesc {
     uint64_t hint_bit;
     uint64_t cells_base;  // first u16 cell of this segment's output region
     uint64_t cells_cap;   // capacity in cells
+    uint64_t end_byte;    // end of the segment's file in the input buffer (bits past it are not read)
     uint32_t flags;       // SEG_SPEC | SEG_SINGLE_MEMBER
     uint32_t mb_base;     // first MemberRec slot
     uint32_t mb_cap;      // MemberRec slots
@@ -89,7 +90,7 @@ struct ChainSeg {
     uint64_t len;         // cells
     uint64_t tz_base;     // index of the segment's tail zone in the dense tail arrays
     uint32_t spec;        // first member may hold markers
-    uint32_t pad;
+    uint32_t first;       // first segment of a file's output: nothing before it is reachable
 };
 
 }  // namespace rrg

@@ -216,16 +216,16 @@ struct Ctx {
     int device = 0;
     hipStream_t s = nullptr;
     int n_cu = 0;
-    DevBuf in, targets, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, lself, lptr, lcell, counters,
-        tiles, out, poff, plen, pcrc, err;
+    DevBuf in, targets, limits, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
+        pcrc, err;
+    size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     uint8_t* pin = nullptr;
     size_t pin_cap = 0;
     rrg_profile prof{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Ctx() {
-        DevBuf* all[] = {&in,      &targets, &found, &segs, &res,  &mrec,  &work,     &counter,
-                         &cells,   &sorted,  &chain, &tz,   &lself, &lptr, &lcell,    &counters,
-                         &tiles,   &out,     &poff,  &plen, &pcrc, &err};
+        DevBuf* all[] = {&in,    &targets, &limits, &found, &segs, &res,  &mrec, &work, &counter, &cells,
+                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err};
         for (DevBuf* b : all) b->release();
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -258,9 +258,10 @@ struct StageTimer {
     }
 };
 
-// Result of one decode, in device output coordinates.
+// Result of one file's decode.
 struct Decoded {
-    std::vector<uint64_t> chunk_ends;  // emitted chunk end offsets (absolute in c.out)
+    std::vector<uint64_t> chunk_ends;  // output chunk end offsets, relative to out_base
+    uint64_t out_base = 0;             // the file's first output byte in the batch output buffer
     uint64_t spec_failures = 0;
     bool failed = false;
     rrg_error error{};
@@ -274,7 +275,7 @@ struct Geometry {
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
     G.g.lut_bits = (cfg.lut_bits == 8 || cfg.lut_bits == 10) ? cfg.lut_bits : 9;
-    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 32;
+    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 16;
     if (lanes > 64) lanes = 64;
     while (lanes > 1 && (size_t)lanes * inflate_lds_per_lane(G.g.lut_bits) > 65536) lanes /= 2;
     G.g.lanes_per_block = lanes;
@@ -285,7 +286,7 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     return G;
 }
 
-// Give segment j a larger output region (cells or member records) after it
+// Give a segment a larger output region (cells or member records) after it
 // overflowed; the old region is abandoned.
 void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& mb_used) {
     if (status == ST_MB_OVERFLOW) {
@@ -302,9 +303,8 @@ void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& m
 
 // One launch of the inflate kernel over the segment indices in `work`; results
 // for every segment are copied back into `res`.
-int run_inflate(Ctx& c, const Geometry& G, uint64_t nbits, uint64_t in_len, std::vector<SegDesc>& desc,
-                std::vector<SegResult>& res, uint64_t cells_used, uint64_t mb_used, const std::vector<uint32_t>& work,
-                rrg_error* err) {
+int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vector<SegResult>& res,
+                uint64_t cells_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
     HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * 480 * sizeof(uint16_t)));
@@ -319,7 +319,7 @@ int run_inflate(Ctx& c, const Geometry& G, uint64_t nbits, uint64_t in_len, std:
     HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
     {
         StageTimer t(c, &c.prof.ms_inflate);
-        HIPCHK(launch_inflate(c.in.as<uint64_t>(), nbits, in_len, c.segs.as<SegDesc>(), c.res.as<SegResult>(),
+        HIPCHK(launch_inflate(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(),
                               c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
                               c.counter.as<uint32_t>(), c.cells.as<uint16_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
         t.stop();
@@ -333,61 +333,48 @@ int run_inflate(Ctx& c, const Geometry& G, uint64_t nbits, uint64_t in_len, std:
 
 inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
 
-// Geometry of the accepted chain, shared by the resolve / CRC stages.
-struct ChainOut {
-    std::vector<uint64_t> off;  // output offset (relative to the path's base) per chain segment, plus total
-};
-
-// Tail chain + resolve + narrow for chain segments [0, n) into c.out at out_base.
-int resolve_chain(Ctx& c, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res, uint32_t n,
-                  uint64_t out_base, const std::vector<uint64_t>& off, uint32_t* marker_err_seg, rrg_error* err) {
-    *marker_err_seg = UINT32_MAX;
+// Tail chain + resolve + narrow for the chain segments `cs` into `out`;
+// out_off[k] is chain segment k's first output byte.  flags[k] != 0 when a
+// marker of segment k reaches before the start of its file's output.
+int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>& out_off, DevBuf& out,
+                  std::vector<uint32_t>& flags, rrg_error* err) {
+    const uint32_t n = (uint32_t)cs.size();
+    flags.assign(n, 0);
     if (n == 0) return RRG_OK;
-    std::vector<ChainSeg> cs(n);
     uint64_t tz = 0;
+    bool any_spec = false;
     for (uint32_t j = 0; j < n; ++j) {
-        cs[j].cells_base = desc[j].cells_base;
-        cs[j].len = res[j].n_cells;
         cs[j].tz_base = tz;
-        cs[j].spec = (desc[j].flags & SEG_SPEC) ? 1u : 0u;
-        cs[j].pad = 0;
-        tz += std::min<uint64_t>(res[j].n_cells, 32768);
+        tz += std::min<uint64_t>(cs[j].len, 32768);
+        any_spec |= cs[j].spec != 0;
     }
     HIPCHK(c.chain.ensure(n * sizeof(ChainSeg)));
     HIPCHK(hipMemcpyAsync(c.chain.p, cs.data(), n * sizeof(ChainSeg), hipMemcpyHostToDevice, c.s));
-    HIPCHK(c.counters.ensure(64));
-    bool any_spec = false;
-    for (uint32_t j = 0; j < n; ++j) any_spec |= cs[j].spec != 0;
+    HIPCHK(c.err.ensure((size_t)n * 4));
+    HIPCHK(hipMemsetAsync(c.err.p, 0, (size_t)n * 4, c.s));
     if (any_spec && tz) {
         StageTimer t(c, &c.prof.ms_tail);
         // ~4 blocks per CU keeps phase 1 parallel while phase 2 stays short
         const uint32_t G = std::max<uint32_t>(1, (uint32_t)(n / std::max(1, 4 * c.n_cu)));
         const uint32_t nb = (n + G - 1) / G;
         HIPCHK(c.tz.ensure((size_t)nb * 32768 * 2 * sizeof(uint16_t)));
-        HIPCHK(c.counters.ensure(16));
-        const uint32_t none = UINT32_MAX;
-        HIPCHK(hipMemcpyAsync(c.counters.p, &none, 4, hipMemcpyHostToDevice, c.s));
         uint16_t* exitb = c.tz.as<uint16_t>();
         uint16_t* entry = exitb + (size_t)nb * 32768;
-        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), exitb, entry,
-                           c.counters.as<uint32_t>(), c.s));
-        uint32_t e1 = UINT32_MAX;
-        HIPCHK(hipMemcpyAsync(&e1, c.counters.p, 4, hipMemcpyDeviceToHost, c.s));
+        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), exitb, entry, c.err.as<uint32_t>(),
+                           c.s));
         t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
-        *marker_err_seg = e1;
-        c.prof.n_tail_rounds = nb;
+        c.prof.n_tail_rounds += nb;
     }
     // resolve tiles: 64 KiB of output each, aligned in output coordinates
     const uint64_t TILE = 65536;
     std::vector<ResolveTile> tiles;
     for (uint32_t j = 0; j < n; ++j) {
-        const uint64_t ob = out_base + off[j], oe = ob + res[j].n_cells;
+        const uint64_t ob = out_off[j], oe = ob + cs[j].len;
         uint64_t p = ob;
         while (p < oe) {
             const uint64_t q = std::min<uint64_t>(oe, (p / TILE + 1) * TILE);
             ResolveTile t;
-            t.cell_addr = desc[j].cells_base + (p - ob);
+            t.cell_addr = cs[j].cells_base + (p - ob);
             t.out_off = p;
             t.seg = j;
             t.n = (uint32_t)(q - p);
@@ -398,25 +385,20 @@ int resolve_chain(Ctx& c, const std::vector<SegDesc>& desc, const std::vector<Se
     if (!tiles.empty()) {
         StageTimer t(c, &c.prof.ms_resolve);
         HIPCHK(c.tiles.ensure(tiles.size() * sizeof(ResolveTile)));
-        HIPCHK(c.err.ensure(16));
         HIPCHK(hipMemcpyAsync(c.tiles.p, tiles.data(), tiles.size() * sizeof(ResolveTile), hipMemcpyHostToDevice,
                               c.s));
-        const uint32_t none = UINT32_MAX;
-        HIPCHK(hipMemcpyAsync(c.err.p, &none, 4, hipMemcpyHostToDevice, c.s));
         HIPCHK(launch_resolve(c.tiles.as<ResolveTile>(), (uint32_t)tiles.size(), c.chain.as<ChainSeg>(),
-                              c.cells.as<uint16_t>(), c.out.as<uint8_t>(), c.err.as<uint32_t>(), c.s));
-        uint32_t e2 = UINT32_MAX;
-        HIPCHK(hipMemcpyAsync(&e2, c.err.p, 4, hipMemcpyDeviceToHost, c.s));
+                              c.cells.as<uint16_t>(), out.as<uint8_t>(), c.err.as<uint32_t>(), c.s));
         t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
-        *marker_err_seg = std::min(*marker_err_seg, e2);
     }
+    HIPCHK(hipMemcpyAsync(flags.data(), c.err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
     return RRG_OK;
 }
 
-// CRC pieces for byte ranges [lo, hi) of c.out; returns the raw CRC of each range.
-int crc_ranges(Ctx& c, const std::vector<std::pair<uint64_t, uint64_t>>& ranges, std::vector<uint32_t>& raw,
-               rrg_error* err) {
+// CRC pieces for byte ranges [lo, hi) of `out`; returns the raw CRC of each range.
+int crc_ranges(Ctx& c, const DevBuf& out, const std::vector<std::pair<uint64_t, uint64_t>>& ranges,
+               std::vector<uint32_t>& raw, rrg_error* err) {
     const uint64_t PIECE = 256 * 1024;
     std::vector<uint64_t> po;
     std::vector<uint32_t> pl;
@@ -437,8 +419,8 @@ int crc_ranges(Ctx& c, const std::vector<std::pair<uint64_t, uint64_t>>& ranges,
         HIPCHK(c.pcrc.ensure(pc.size() * 4));
         HIPCHK(hipMemcpyAsync(c.poff.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, c.s));
         HIPCHK(hipMemcpyAsync(c.plen.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, c.s));
-        HIPCHK(launch_crc_pieces(c.out.as<uint8_t>(), c.poff.as<uint64_t>(), c.plen.as<uint32_t>(),
-                                 (uint32_t)po.size(), c.pcrc.as<uint32_t>(), c.s));
+        HIPCHK(launch_crc_pieces(out.as<uint8_t>(), c.poff.as<uint64_t>(), c.plen.as<uint32_t>(), (uint32_t)po.size(),
+                                 c.pcrc.as<uint32_t>(), c.s));
         HIPCHK(hipMemcpyAsync(pc.data(), c.pcrc.p, pc.size() * 4, hipMemcpyDeviceToHost, c.s));
         t.stop();
         HIPCHK(hipStreamSynchronize(c.s));
@@ -455,243 +437,243 @@ int crc_ranges(Ctx& c, const std::vector<std::pair<uint64_t, uint64_t>>& ranges,
 
 inline uint32_t std_crc(uint32_t raw, uint64_t len) { return raw ^ gf2_mul(x8n(len), 0xffffffffu) ^ 0xffffffffu; }
 
-// parallel_decode_member (pipeline.rs:468-1041) over the body at file offset
-// body_off of the file in c.in; output appended to c.out at out_base.
-int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len, const rrg_cfg& cfg,
-                uint64_t out_base, Decoded& dec, rrg_error* err) {
-    const uint64_t nbits = file_len * 8;
-    const uint64_t base = body_off * 8;
-    const uint64_t total = nbits - base;
-    const uint64_t chunk_bits = (uint64_t)std::max<size_t>(cfg.chunk_size_bytes, 65536) * 8;
-    // segment size: enough segments to keep every lane busy twice over
-    uint64_t seg_bytes = cfg.segment_bytes;
-    if (!seg_bytes) {
-        seg_bytes = (total / 8) / std::max<uint64_t>(1, 2 * G.lanes_in_flight);
-        seg_bytes = std::max<uint64_t>(seg_bytes, 32 * 1024);
-    }
-    uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
-    const uint64_t seg_bits = chunk_bits / D;
+// ---------------------------------------------------------------------------
+// batch engine
+//
+// A batch is a set of gzip files resident in c.in.  Every stage runs once for
+// the whole batch: one finder launch over the chunk/segment targets of every
+// file, one persistent inflate launch over all segments (BGZF members and
+// speculative segments of uneven size share the work queue), host chain walks
+// whose re-decodes are again gathered into one launch per round, one tail
+// chain + resolve over the concatenated chains, one CRC launch over every
+// member of every file.  A single-file decode is a batch of one.
+// ---------------------------------------------------------------------------
+struct Job {
+    // input
+    uint64_t in_off = 0;         // the file's first byte in c.in (multiple of 8)
+    uint64_t len = 0;            // file length
+    const uint8_t* h = nullptr;  // host copy of the file: headers and BGZF framing are read here
+    size_t chunk_size = 4u << 20;
+    size_t segment_bytes = 0;
+    // host plan
+    struct Mb {
+        uint64_t off, size;  // file offsets
+    };
+    std::vector<Mb> mem;           // BGZF members
+    std::vector<uint8_t> hdr_err;  // per BGZF member
+    bool bgzf = false, walk_trunc = false;
+    bool pre_failed = false;  // header error found while planning (reported after any BGZF output)
+    rrg_error pre_err{};
+    uint64_t body = UINT64_MAX;  // file offset of the speculative (member-path) deflate body; none if MAX
+    // targets and segments (index ranges in the batch arrays)
+    size_t t_lo = 0, t_hi = 0;
+    size_t bg_lo = 0, bg_hi = 0, mp_lo = 0, mp_hi = 0;
+    std::vector<uint8_t> seg_is_chunk;  // per member-path segment
+    std::vector<uint64_t> orig_start;
+    std::vector<uint8_t> first_ok;
+    size_t n_chain = 0;
+    long fatal = -1;
+    bool saw_final = false;
+    // output
+    std::vector<uint64_t> off;  // job-relative output offset: BGZF members, then chain segments, then the end
+    uint64_t U = 0;
+    size_t c_lo = 0;  // first chain segment of this job in the batch chain
+    size_t r_lo = 0;  // first CRC range
+    Decoded dec;
 
-    // targets: chunk targets (pipeline.rs:497-503) and segment targets between them
-    std::vector<uint64_t> targets;
-    std::vector<uint8_t> is_chunk;
+    uint64_t abs_bits(uint64_t file_byte) const { return (in_off + file_byte) * 8; }
+    uint64_t end_bits() const { return (in_off + len) * 8; }
+};
+
+// Host planning: BGZF framing walk (pipeline.rs:1357-1420) or the first header
+// (lib.rs:463-472, gzip.rs:252-300).
+void plan_job(Job& J) {
+    const uint8_t* h = J.h;
+    const uint64_t len = J.len;
+    uint64_t rest = 0;
+    if (bgzf_block_size(h, len)) {
+        J.bgzf = true;
+        uint64_t pos = 0;
+        while (pos < len) {
+            const uint32_t sz = bgzf_block_size(h + pos, len - pos);
+            if (!sz) break;
+            if (pos + sz > len) {
+                J.walk_trunc = true;
+                break;
+            }
+            J.mem.push_back({pos, sz});
+            pos += sz;
+        }
+        if (J.walk_trunc || pos >= len) return;
+        rest = pos;  // remainder: the first non-BGZF member onwards (pipeline.rs:1622-1633)
+    }
+    uint32_t meth = 0;
+    const long hl = parse_header(h + rest, len - rest, &meth);
+    if (hl < 0) {
+        J.pre_failed = true;
+        status_err(&J.pre_err, (uint32_t)-hl, 0);
+        J.pre_err.method = meth;
+        return;
+    }
+    J.body = rest + (uint64_t)hl;
+}
+
+// Targets of one member-path body: chunk targets (pipeline.rs:497-503) and
+// segment targets between them.
+void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& targets, std::vector<uint64_t>& limits,
+                  std::vector<uint8_t>& is_chunk) {
+    const uint64_t base = J.abs_bits(J.body);
+    const uint64_t total = J.end_bits() - base;
+    const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
+    uint64_t seg_bytes = J.segment_bytes;
+    if (!seg_bytes) {
+        // 64 KiB segments when the file keeps every lane busy with them; down to
+        // 32 KiB for smaller files
+        seg_bytes = 2 * (total / 8) / std::max<uint64_t>(1, G.lanes_in_flight);
+        seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 32 * 1024), 64 * 1024);
+    }
+    const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
+    const uint64_t seg_bits = chunk_bits / D;
     for (uint64_t k = 0; k * chunk_bits < total; ++k) {
         if (k) {
             targets.push_back(base + k * chunk_bits);
+            limits.push_back(J.end_bits());
             is_chunk.push_back(1);
         }
         for (uint64_t j = 1; j < D; ++j) {
             const uint64_t t = k * chunk_bits + j * seg_bits;
             if (t >= total) break;
             targets.push_back(base + t);
+            limits.push_back(J.end_bits());
             is_chunk.push_back(0);
         }
     }
-    std::vector<uint64_t> found(targets.size());
-    if (!targets.empty()) {
-        HIPCHK(c.targets.ensure(targets.size() * 8));
-        HIPCHK(c.found.ensure(targets.size() * 8));
-        HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), targets.size() * 8, hipMemcpyHostToDevice, c.s));
-        StageTimer t(c, &c.prof.ms_find);
-        HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), nbits, c.targets.as<uint64_t>(), c.found.as<uint64_t>(),
-                                  (uint32_t)targets.size(), c.s));
-        HIPCHK(hipMemcpyAsync(found.data(), c.found.p, found.size() * 8, hipMemcpyDeviceToHost, c.s));
-        t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
-    }
-    // a find is a bit offset inside the body; anything else is a device fault
-    for (size_t i = 0; i < found.size(); ++i)
-        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= nbits)) {
-            set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu",
-                    (unsigned long long)found[i], (unsigned long long)targets[i]);
-            return RRG_E_DEVICE;
-        }
-    // chunk boundaries exactly as the reference (strictly increasing finds)
-    std::vector<uint64_t> B{base};
-    for (size_t i = 0; i < targets.size(); ++i)
-        if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
-    std::vector<uint64_t> M(B);
-    for (size_t i = 0; i < targets.size(); ++i)
-        if (!is_chunk[i] && found[i] != UINT64_MAX) M.push_back(found[i]);
-    std::sort(M.begin(), M.end());
-    M.erase(std::unique(M.begin(), M.end()), M.end());
-    const size_t S = M.size();
-    c.prof.n_segments += S;
-    std::vector<uint8_t> seg_is_chunk(S, 0);
-    {
-        size_t bi = 0;
-        for (size_t j = 0; j < S && bi < B.size(); ++j)
-            if (M[j] == B[bi]) {
-                seg_is_chunk[j] = 1;
-                ++bi;
-            }
-    }
-    // segment descriptors
-    std::vector<SegDesc> desc(S);
-    uint64_t cells_used = 0, mb_used = 0;
-    for (size_t j = 0; j < S; ++j) {
-        const uint64_t hint = j + 1 < S ? M[j + 1] : UINT64_MAX;
-        const uint64_t span = ((j + 1 < S ? M[j + 1] : nbits) - M[j]) / 8;
-        SegDesc& d = desc[j];
-        d.start_bit = M[j];
-        d.hint_bit = hint;
-        d.cells_base = cells_used;
-        d.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
-        cells_used += d.cells_cap;
-        d.flags = j ? SEG_SPEC : 0u;
-        d.mb_base = (uint32_t)mb_used;
-        d.mb_cap = 4;
-        mb_used += 4;
-        d.pad = 0;
-    }
-    std::vector<uint64_t> orig_start(M);
-    std::vector<SegResult> res;
-    std::vector<uint32_t> all(S);
-    for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
-    int rc = run_inflate(c, G, nbits, file_len, desc, res, cells_used, mb_used, all, err);
-    if (rc) return rc;
-    // a capacity overflow is not a decode failure (the region is grown below)
-    std::vector<uint8_t> first_ok(S);
-    for (size_t j = 0; j < S; ++j) first_ok[j] = res[j].status == ST_OK || overflowed(res[j].status);
+}
 
-    // anchored chain (pipeline.rs:774-809) with batched re-decode rounds
-    size_t n_chain = 0;
-    long fatal = -1;
-    bool saw_final = false;
-    for (int round = 0;; ++round) {
-        uint64_t expected = base;
-        std::vector<uint32_t> redo;
-        n_chain = 0;
-        fatal = -1;
-        saw_final = false;
-        for (size_t j = 0; j < S; ++j) {
-            const SegResult& r = res[j];
-            if (desc[j].start_bit == expected && r.status == ST_OK) {
-                ++n_chain;
-                expected = r.end_bit;
-                if (r.final_block) {
-                    saw_final = true;
-                    break;
-                }
-                continue;
-            }
-            if (desc[j].start_bit == expected && overflowed(r.status)) {
-                // verified start, output region too small: grow and decode again
-                grow_segment(desc[j], r.status, cells_used, mb_used);
-                redo.push_back((uint32_t)j);
-                if (j + 1 < S) expected = desc[j + 1].start_bit;
-                continue;
-            }
-            if (desc[j].start_bit == expected) {  // genuine error at a true boundary
-                if (redo.empty()) fatal = (long)j;
+// Anchored verification chain of one job (pipeline.rs:774-809): accept a
+// segment iff it decoded OK from where its predecessor ended; otherwise queue
+// it for re-decode from there (optimistically continuing with the next
+// segment's own start, verified next round).  Returns nonzero when the chain
+// settled (nothing queued).
+bool chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult>& res, uint64_t& cells_used,
+                 uint64_t& mb_used, std::vector<uint32_t>& redo) {
+    const size_t before = redo.size();
+    uint64_t expected = J.abs_bits(J.body);
+    J.n_chain = 0;
+    J.fatal = -1;
+    J.saw_final = false;
+    for (size_t j = J.mp_lo; j < J.mp_hi; ++j) {
+        const SegResult& r = res[j];
+        if (desc[j].start_bit == expected && r.status == ST_OK) {
+            ++J.n_chain;
+            expected = r.end_bit;
+            if (r.final_block) {
+                J.saw_final = true;
                 break;
             }
-            desc[j].start_bit = expected;
+            continue;
+        }
+        if (desc[j].start_bit == expected && overflowed(r.status)) {
+            // verified start, output region too small: grow and decode again
+            grow_segment(desc[j], r.status, cells_used, mb_used);
             redo.push_back((uint32_t)j);
-            if (j + 1 < S) expected = desc[j + 1].start_bit;  // optimistic; verified next round
+            if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
+            continue;
         }
-        if (redo.empty()) break;
-        if (round > (int)S + 64) {
-            set_err(err, RRG_E_DEVICE, "verification chain did not settle");
-            return RRG_E_DEVICE;
+        if (desc[j].start_bit == expected) {  // genuine error at a true boundary
+            if (redo.size() == before) J.fatal = (long)(j - J.mp_lo);
+            break;
         }
-        rc = run_inflate(c, G, nbits, file_len, desc, res, cells_used, mb_used, redo, err);
-        if (rc) return rc;
+        desc[j].start_bit = expected;
+        redo.push_back((uint32_t)j);
+        if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
     }
-    // chunk index of each segment (reference chunk k starts at B[k])
+    return redo.size() == before;
+}
+
+// Errors, CRC/ISIZE checks and chunk emission of the member-path part of a
+// job (pipeline.rs:783-809, 858-952).  `base` is the job-relative output
+// offset of the part; `off[k]` the part-relative offset of chain segment k.
+void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res,
+                        const std::vector<MemberRec>& recs, const std::vector<uint32_t>& flags,
+                        const std::vector<uint32_t>& raw, size_t r_lo, uint64_t base, const std::vector<uint64_t>& off) {
+    const size_t n_chain = J.n_chain;
+    const size_t S = J.mp_hi - J.mp_lo;
+    Decoded& dec = J.dec;
     std::vector<uint32_t> chunk_of(S);
     {
         uint32_t k = 0;
         for (size_t j = 0; j < S; ++j) {
-            if (j && seg_is_chunk[j]) ++k;
+            if (j && J.seg_is_chunk[j]) ++k;
             chunk_of[j] = k;
         }
     }
     // speculation failure = a chunk whose guessed start was not where the chain
-    // arrived, or whose speculative decode failed (pipeline.rs:783-809)
-    uint64_t spec = 0;
+    // arrived, or whose speculative decode failed
     for (size_t j = 1; j < n_chain; ++j)
-        if (seg_is_chunk[j] && (orig_start[j] != desc[j].start_bit || !first_ok[j])) ++spec;
-    dec.spec_failures += spec;
-
-    // output offsets of the accepted chain
-    std::vector<uint64_t> off(n_chain + 1, 0);
-    for (size_t j = 0; j < n_chain; ++j) off[j + 1] = off[j] + res[j].n_cells;
-    const uint64_t U = off[n_chain];
-    HIPCHK(c.out.grow_keep(out_base + U + 64, out_base, c.s));
-    uint32_t merr = UINT32_MAX;
-    rc = resolve_chain(c, desc, res, (uint32_t)n_chain, out_base, off, &merr, err);
-    if (rc) return rc;
-
-    // member CRC / ISIZE (pipeline.rs:867-916)
-    std::vector<MemberRec> recs(mb_used);
-    if (mb_used) {
-        HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
-        HIPCHK(hipStreamSynchronize(c.s));
-    }
-    struct Mem {
-        uint64_t lo, hi;
-        const MemberRec* rec;
-        uint32_t seg;
-    };
-    std::vector<Mem> mems;
-    uint64_t mstart = 0;
-    for (size_t j = 0; j < n_chain; ++j)
-        for (uint32_t k = 0; k < res[j].n_mb; ++k) {
-            const MemberRec& r = recs[desc[j].mb_base + k];
-            const uint64_t e = off[j] + r.out_off;
-            mems.push_back({mstart, e, &r, (uint32_t)j});
-            mstart = e;
-        }
-    std::vector<std::pair<uint64_t, uint64_t>> ranges;
-    for (const Mem& m : mems) ranges.push_back({out_base + m.lo, out_base + m.hi});
-    std::vector<uint32_t> raw;
-    rc = crc_ranges(c, ranges, raw, err);
-    if (rc) return rc;
-
+        if (J.seg_is_chunk[j] && (J.orig_start[j] != desc[J.mp_lo + j].start_bit || !J.first_ok[j]))
+            ++dec.spec_failures;
+    const uint64_t body_abs = J.in_off + J.body;
     // earliest failure, as the chunk index it stops the stream at
     uint32_t fail_chunk = UINT32_MAX;
     rrg_error ferr{};
-    if (fatal >= 0) {
-        fail_chunk = chunk_of[fatal];
-        status_err(&ferr, res[fatal].status, res[fatal].detail);
+    if (J.fatal >= 0) {
+        fail_chunk = chunk_of[J.fatal];
+        status_err(&ferr, res[J.mp_lo + J.fatal].status, res[J.mp_lo + J.fatal].detail);
     }
-    if (merr != UINT32_MAX && chunk_of[merr] < fail_chunk) {
-        fail_chunk = chunk_of[merr];
-        set_err(&ferr, RRG_E_INVALID, "invalid deflate stream: %s", detail_msg(DET_MARKER_RANGE));
-    }
-    for (size_t i = 0; i < mems.size(); ++i) {
-        const Mem& m = mems[i];
-        const uint64_t len = m.hi - m.lo;
-        const uint32_t got = std_crc(raw[i], len);
-        const uint32_t ck = chunk_of[m.seg];
-        if (ck >= fail_chunk) break;
-        if (got != m.rec->crc) {
-            fail_chunk = ck;
-            set_err(&ferr, RRG_E_CRC,
-                    "CRC32 mismatch in member #%zu (uncompressed bytes: %llu, trailer at byte %llu): expected "
-                    "%#010x, got %#010x",
-                    i, (unsigned long long)len, (unsigned long long)(m.rec->trailer_byte - body_off),
-                    m.rec->crc, got);
-            ferr.expected = m.rec->crc;
-            ferr.got = got;
-            ferr.got64 = len;
-            ferr.member = (uint32_t)i;
-            ferr.trailer_byte = m.rec->trailer_byte - body_off;
+    for (size_t k = 0; k < n_chain; ++k)
+        if (flags[J.c_lo + (J.bg_hi - J.bg_lo) + k]) {
+            if (chunk_of[k] < fail_chunk) {
+                fail_chunk = chunk_of[k];
+                set_err(&ferr, RRG_E_INVALID, "invalid deflate stream: %s", detail_msg(DET_MARKER_RANGE));
+            }
             break;
         }
-        if ((uint32_t)len != m.rec->isize) {
-            fail_chunk = ck;
-            set_err(&ferr, RRG_E_ISIZE, "ISIZE mismatch in member #%zu (trailer at byte %llu): expected %u, got %llu",
-                    i, (unsigned long long)(m.rec->trailer_byte - body_off), m.rec->isize, (unsigned long long)len);
-            ferr.expected = m.rec->isize;
-            ferr.got64 = len;
-            ferr.member = (uint32_t)i;
-            ferr.trailer_byte = m.rec->trailer_byte - body_off;
-            break;
+    // member records in chain order, with their output ranges
+    size_t mi = 0;
+    uint64_t mstart = 0;
+    bool stop = false;
+    for (size_t j = 0; j < n_chain && !stop; ++j) {
+        for (uint32_t k = 0; k < res[J.mp_lo + j].n_mb; ++k, ++mi) {
+            const MemberRec& m = recs[desc[J.mp_lo + j].mb_base + k];
+            const uint64_t e = off[j] + m.out_off;
+            const uint64_t len = e - mstart;
+            mstart = e;
+            const uint32_t ck = chunk_of[j];
+            if (ck >= fail_chunk) {
+                stop = true;
+                break;
+            }
+            const uint32_t got = std_crc(raw[r_lo + mi], len);
+            const uint64_t tb = m.trailer_byte - body_abs;
+            if (got != m.crc) {
+                fail_chunk = ck;
+                set_err(&ferr, RRG_E_CRC,
+                        "CRC32 mismatch in member #%zu (uncompressed bytes: %llu, trailer at byte %llu): expected "
+                        "%#010x, got %#010x",
+                        mi, (unsigned long long)len, (unsigned long long)tb, m.crc, got);
+                ferr.expected = m.crc;
+                ferr.got = got;
+                ferr.got64 = len;
+                ferr.member = (uint32_t)mi;
+                ferr.trailer_byte = tb;
+                stop = true;
+                break;
+            }
+            if ((uint32_t)len != m.isize) {
+                fail_chunk = ck;
+                set_err(&ferr, RRG_E_ISIZE, "ISIZE mismatch in member #%zu (trailer at byte %llu): expected %u, got %llu",
+                        mi, (unsigned long long)tb, m.isize, (unsigned long long)len);
+                ferr.expected = m.isize;
+                ferr.got64 = len;
+                ferr.member = (uint32_t)mi;
+                ferr.trailer_byte = tb;
+                stop = true;
+                break;
+            }
         }
     }
-    if (fail_chunk == UINT32_MAX && !saw_final) {
+    if (fail_chunk == UINT32_MAX && !J.saw_final) {
         // the chain ran past every segment without reaching the end of the file
         fail_chunk = n_chain ? chunk_of[n_chain - 1] + 1 : 0;
         set_err(&ferr, RRG_E_IO, "parallel decode produced no final block");
@@ -699,177 +681,294 @@ int member_path(Ctx& c, const Geometry& G, uint64_t body_off, uint64_t file_len,
     // emit chunks: [start of chunk k, start of chunk k+1) over the chain
     const uint32_t n_chunks_chain = n_chain ? chunk_of[n_chain - 1] + 1 : 0;
     const uint32_t emit = std::min(n_chunks_chain, fail_chunk);
-    for (size_t j = 1; j <= n_chain && dec.chunk_ends.size() < emit; ++j)
-        if (j == n_chain || chunk_of[j] != chunk_of[j - 1]) dec.chunk_ends.push_back(out_base + off[j]);
+    uint32_t emitted = 0;
+    for (size_t j = 1; j <= n_chain && emitted < emit; ++j)
+        if (j == n_chain || chunk_of[j] != chunk_of[j - 1]) {
+            dec.chunk_ends.push_back(base + off[j]);
+            ++emitted;
+        }
     if (fail_chunk != UINT32_MAX) {
         dec.failed = true;
         dec.error = ferr;
     }
-    return RRG_OK;
 }
 
-// parallel_decode_bgzf (pipeline.rs:1357-1636)
-int bgzf_path(Ctx& c, const Geometry& G, const uint8_t* h, uint64_t len, const rrg_cfg& cfg, Decoded& dec,
-              rrg_error* err) {
-    struct Mb {
-        uint64_t off, size;
-    };
-    std::vector<Mb> mem;
-    uint64_t pos = 0;
-    bool walk_trunc = false;
-    while (pos < len) {
-        const uint32_t sz = bgzf_block_size(h + pos, len - pos);
-        if (!sz) break;
-        if (pos + sz > len) {
-            walk_trunc = true;
-            break;
-        }
-        mem.push_back({pos, sz});
-        pos += sz;
+// Decode every job of the batch (inputs already in c.in) into `out`.
+int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out, rrg_error* err) {
+    const uint64_t total_bits = (uint64_t)c.in_used * 8;
+    // 1. host plan
+    for (Job& J : jobs) plan_job(J);
+    // 2. block finder over every member-path body
+    std::vector<uint64_t> targets, limits;
+    std::vector<uint8_t> is_chunk;
+    for (Job& J : jobs) {
+        J.t_lo = targets.size();
+        if (J.body != UINT64_MAX) make_targets(J, G, targets, limits, is_chunk);
+        J.t_hi = targets.size();
     }
-    const size_t S = mem.size();
-    std::vector<SegDesc> desc(S);
-    std::vector<int> hdr_err(S, 0);
+    std::vector<uint64_t> found(targets.size());
+    if (!targets.empty()) {
+        HIPCHK(c.targets.ensure(targets.size() * 8));
+        HIPCHK(c.limits.ensure(limits.size() * 8));
+        HIPCHK(c.found.ensure(targets.size() * 8));
+        HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), targets.size() * 8, hipMemcpyHostToDevice, c.s));
+        HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), limits.size() * 8, hipMemcpyHostToDevice, c.s));
+        StageTimer t(c, &c.prof.ms_find);
+        HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), total_bits, c.targets.as<uint64_t>(), c.limits.as<uint64_t>(),
+                                  c.found.as<uint64_t>(), (uint32_t)targets.size(), c.s));
+        HIPCHK(hipMemcpyAsync(found.data(), c.found.p, found.size() * 8, hipMemcpyDeviceToHost, c.s));
+        t.stop();
+        HIPCHK(hipStreamSynchronize(c.s));
+    }
+    // a find is a bit offset inside the target's file; anything else is a device fault
+    for (size_t i = 0; i < found.size(); ++i)
+        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= limits[i])) {
+            set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu", (unsigned long long)found[i],
+                    (unsigned long long)targets[i]);
+            return RRG_E_DEVICE;
+        }
+    // 3. segment descriptors
+    std::vector<SegDesc> desc;
     uint64_t cells_used = 0, mb_used = 0;
-    for (size_t j = 0; j < S; ++j) {
-        const uint8_t* p = h + mem[j].off;
-        const long hl = parse_header(p, mem[j].size, nullptr);
-        uint64_t isz = 0;
-        if (mem[j].size >= 18)
-            isz = (uint64_t)p[mem[j].size - 4] | ((uint64_t)p[mem[j].size - 3] << 8) |
-                  ((uint64_t)p[mem[j].size - 2] << 16) | ((uint64_t)p[mem[j].size - 1] << 24);
-        SegDesc& d = desc[j];
-        if (hl < 0 || mem[j].size < 18) {
-            hdr_err[j] = 1;
-            d.start_bit = d.hint_bit = 0;  // empty decode
-        } else {
-            d.start_bit = (mem[j].off + (uint64_t)hl) * 8;
-            d.hint_bit = UINT64_MAX;
+    for (Job& J : jobs) {
+        // BGZF members: one single-member segment each, output sized from ISIZE
+        J.bg_lo = desc.size();
+        J.hdr_err.assign(J.mem.size(), 0);
+        for (size_t j = 0; j < J.mem.size(); ++j) {
+            const uint8_t* p = J.h + J.mem[j].off;
+            const uint64_t sz = J.mem[j].size;
+            const long hl = parse_header(p, sz, nullptr);
+            uint64_t isz = 0;
+            if (sz >= 18)
+                isz = (uint64_t)p[sz - 4] | ((uint64_t)p[sz - 3] << 8) | ((uint64_t)p[sz - 2] << 16) |
+                      ((uint64_t)p[sz - 1] << 24);
+            SegDesc d{};
+            if (hl < 0 || sz < 18) {
+                J.hdr_err[j] = 1;
+                d.start_bit = d.hint_bit = 0;  // empty decode
+                d.end_byte = 0;
+            } else {
+                d.start_bit = J.abs_bits(J.mem[j].off + (uint64_t)hl);
+                d.hint_bit = UINT64_MAX;
+                d.end_byte = J.in_off + J.len;
+            }
+            d.cells_base = cells_used;
+            d.cells_cap = ((std::min<uint64_t>(isz, 1u << 22) + 64) + 7) & ~7ull;
+            cells_used += d.cells_cap;
+            d.flags = SEG_SINGLE_MEMBER;
+            d.mb_base = (uint32_t)mb_used;
+            d.mb_cap = 1;
+            mb_used += 1;
+            desc.push_back(d);
         }
-        d.cells_base = cells_used;
-        d.cells_cap = ((std::min<uint64_t>(isz, 1u << 22) + 64) + 7) & ~7ull;
-        cells_used += d.cells_cap;
-        d.flags = SEG_SINGLE_MEMBER;
-        d.mb_base = (uint32_t)mb_used;
-        d.mb_cap = 1;
-        mb_used += 1;
-        d.pad = 0;
+        J.bg_hi = desc.size();
+        // member path: chunk boundaries exactly as the reference (strictly
+        // increasing finds), segments at every distinct find
+        J.mp_lo = J.mp_hi = desc.size();
+        if (J.body == UINT64_MAX) continue;
+        const uint64_t base = J.abs_bits(J.body);
+        std::vector<uint64_t> B{base};
+        for (size_t i = J.t_lo; i < J.t_hi; ++i)
+            if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
+        std::vector<uint64_t> M(B);
+        for (size_t i = J.t_lo; i < J.t_hi; ++i)
+            if (!is_chunk[i] && found[i] != UINT64_MAX) M.push_back(found[i]);
+        std::sort(M.begin(), M.end());
+        M.erase(std::unique(M.begin(), M.end()), M.end());
+        const size_t S = M.size();
+        J.seg_is_chunk.assign(S, 0);
+        for (size_t j = 0, bi = 0; j < S && bi < B.size(); ++j)
+            if (M[j] == B[bi]) {
+                J.seg_is_chunk[j] = 1;
+                ++bi;
+            }
+        for (size_t j = 0; j < S; ++j) {
+            SegDesc d{};
+            d.start_bit = M[j];
+            d.hint_bit = j + 1 < S ? M[j + 1] : UINT64_MAX;
+            d.end_byte = J.in_off + J.len;
+            const uint64_t span = ((j + 1 < S ? M[j + 1] : J.end_bits()) - M[j]) / 8;
+            d.cells_base = cells_used;
+            d.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
+            cells_used += d.cells_cap;
+            d.flags = j ? SEG_SPEC : 0u;
+            d.mb_base = (uint32_t)mb_used;
+            d.mb_cap = 4;
+            mb_used += 4;
+            desc.push_back(d);
+        }
+        J.mp_hi = desc.size();
+        J.orig_start = M;
     }
+    const size_t S = desc.size();
     c.prof.n_segments += S;
+    // 4. one inflate launch over every segment of the batch
     std::vector<SegResult> res;
     std::vector<uint32_t> all(S);
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
-    int rc = run_inflate(c, G, len * 8, len, desc, res, cells_used, mb_used, all, err);
+    int rc = run_inflate(c, G, desc, res, cells_used, mb_used, all, err);
     if (rc) return rc;
-    for (int round = 0; round < 6; ++round) {
-        std::vector<uint32_t> again;
-        for (size_t j = 0; j < S; ++j)
-            if (overflowed(res[j].status)) {
-                grow_segment(desc[j], res[j].status, cells_used, mb_used);
-                again.push_back((uint32_t)j);
-            }
-        if (again.empty()) break;
-        rc = run_inflate(c, G, len * 8, len, desc, res, cells_used, mb_used, again, err);
+    for (Job& J : jobs) {
+        // a capacity overflow is not a decode failure (the region is grown below)
+        J.first_ok.resize(J.mp_hi - J.mp_lo);
+        for (size_t j = J.mp_lo; j < J.mp_hi; ++j)
+            J.first_ok[j - J.mp_lo] = res[j].status == ST_OK || overflowed(res[j].status);
+    }
+    // 5. settle: chain walks and capacity retries, re-decodes batched per round
+    for (int round = 0;; ++round) {
+        std::vector<uint32_t> redo;
+        for (Job& J : jobs) {
+            if (round < 6)
+                for (size_t j = J.bg_lo; j < J.bg_hi; ++j)
+                    if (overflowed(res[j].status)) {
+                        grow_segment(desc[j], res[j].status, cells_used, mb_used);
+                        redo.push_back((uint32_t)j);
+                    }
+            if (J.body != UINT64_MAX) chain_round(J, desc, res, cells_used, mb_used, redo);
+        }
+        if (redo.empty()) break;
+        if (round > (int)S + 64) {
+            set_err(err, RRG_E_DEVICE, "verification chain did not settle");
+            return RRG_E_DEVICE;
+        }
+        rc = run_inflate(c, G, desc, res, cells_used, mb_used, redo, err);
         if (rc) return rc;
     }
-    std::vector<uint64_t> off(S + 1, 0);
-    for (size_t j = 0; j < S; ++j) off[j + 1] = off[j] + res[j].n_cells;
-    const uint64_t U = off[S];
-    HIPCHK(c.out.ensure(U + 64));
-    uint32_t merr = UINT32_MAX;
-    rc = resolve_chain(c, desc, res, (uint32_t)S, 0, off, &merr, err);
+    // 6. output layout (each file at a 256-byte aligned offset) and the chain
+    std::vector<ChainSeg> cs;
+    std::vector<uint64_t> cs_out;
+    uint64_t out_total = 0;
+    for (Job& J : jobs) {
+        J.dec.out_base = out_total;
+        J.c_lo = cs.size();
+        J.off.clear();
+        uint64_t o = 0;
+        for (size_t j = J.bg_lo; j < J.bg_hi; ++j) {
+            J.off.push_back(o);
+            cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, 0u, 1u});
+            cs_out.push_back(out_total + o);
+            o += res[j].n_cells;
+        }
+        for (size_t k = 0; k < J.n_chain; ++k) {
+            const size_t j = J.mp_lo + k;
+            J.off.push_back(o);
+            cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
+                                  k == 0 ? 1u : 0u});
+            cs_out.push_back(out_total + o);
+            o += res[j].n_cells;
+        }
+        J.off.push_back(o);
+        J.U = o;
+        out_total = (out_total + o + 255) & ~255ull;
+    }
+    HIPCHK(out.ensure(out_total + 64));
+    std::vector<uint32_t> flags;
+    rc = resolve_chain(c, cs, cs_out, out, flags, err);
     if (rc) return rc;
+    // 7. CRC of every member of every file (BGZF members, then member-path members)
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
         HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
         HIPCHK(hipStreamSynchronize(c.s));
     }
-    std::vector<std::pair<uint64_t, uint64_t>> ranges(S);
-    for (size_t j = 0; j < S; ++j) ranges[j] = {off[j], off[j + 1]};
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    for (Job& J : jobs) {
+        J.r_lo = ranges.size();
+        const uint64_t ob = J.dec.out_base;
+        const size_t nb = J.bg_hi - J.bg_lo;
+        for (size_t k = 0; k < nb; ++k) ranges.push_back({ob + J.off[k], ob + J.off[k + 1]});
+        uint64_t mstart = J.off[nb];
+        for (size_t k = 0; k < J.n_chain; ++k) {
+            const size_t j = J.mp_lo + k;
+            for (uint32_t q = 0; q < res[j].n_mb; ++q) {
+                const uint64_t e = J.off[nb + k] + recs[desc[j].mb_base + q].out_off;
+                ranges.push_back({ob + mstart, ob + e});
+                mstart = e;
+            }
+        }
+    }
     std::vector<uint32_t> raw;
-    rc = crc_ranges(c, ranges, raw, err);
+    rc = crc_ranges(c, out, ranges, raw, err);
     if (rc) return rc;
-    // first failing member (libdeflate_gzip_decompress semantics: header, deflate, CRC, ISIZE)
-    size_t bad = S;
-    for (size_t j = 0; j < S; ++j) {
-        const SegResult& r = res[j];
-        bool ok = !hdr_err[j] && r.status == ST_OK && r.n_mb == 1;
-        if (ok) {
-            const MemberRec& m = recs[desc[j].mb_base];
-            ok = std_crc(raw[j], r.n_cells) == m.crc && (uint32_t)r.n_cells == m.isize &&
-                 m.trailer_byte + 8 <= mem[j].off + mem[j].size;
+    // 8. per-file verdicts and chunk boundaries
+    for (Job& J : jobs) {
+        Decoded& dec = J.dec;
+        const size_t nb = J.bg_hi - J.bg_lo;
+        if (J.bgzf) {
+            // first failing member (libdeflate_gzip_decompress semantics: header, deflate, CRC, ISIZE)
+            size_t bad = nb;
+            for (size_t k = 0; k < nb; ++k) {
+                const size_t j = J.bg_lo + k;
+                const SegResult& r = res[j];
+                bool ok = !J.hdr_err[k] && r.status == ST_OK && r.n_mb == 1;
+                if (ok) {
+                    const MemberRec& m = recs[desc[j].mb_base];
+                    ok = std_crc(raw[J.r_lo + k], r.n_cells) == m.crc && (uint32_t)r.n_cells == m.isize &&
+                         m.trailer_byte + 8 <= J.in_off + J.mem[k].off + J.mem[k].size;
+                }
+                if (!ok) {
+                    bad = k;
+                    break;
+                }
+            }
+            // batches ramp 64 KiB -> chunk size (pipeline.rs:1488-1551)
+            const uint64_t tmax = std::max<size_t>(J.chunk_size, 65536);
+            uint64_t target = 65536, cur = 0;
+            size_t first = 0;
+            for (size_t k = 0; k < nb; ++k) {
+                cur += J.mem[k].size;
+                if (cur >= target || k + 1 == nb) {
+                    if (bad >= first && bad <= k) break;  // this batch holds the failing member
+                    dec.chunk_ends.push_back(J.off[k + 1]);
+                    if (cur >= target) target = std::min(target * 2, tmax);
+                    cur = 0;
+                    first = k + 1;
+                }
+            }
+            if (bad < nb) {
+                dec.failed = true;
+                set_err(&dec.error, RRG_E_INVALID, "invalid deflate stream: libdeflate gzip decode failed");
+                continue;
+            }
+            if (J.walk_trunc) {
+                dec.failed = true;
+                set_err(&dec.error, RRG_E_TRUNCATED, "truncated gzip stream");
+                continue;
+            }
         }
-        if (!ok) {
-            bad = j;
-            break;
-        }
-    }
-    // batches ramp 64 KiB -> chunk size (pipeline.rs:1488-1551)
-    const uint64_t tmax = std::max<size_t>(cfg.chunk_size_bytes, 65536);
-    uint64_t target = 65536, cur = 0;
-    size_t first = 0;
-    for (size_t j = 0; j < S; ++j) {
-        cur += mem[j].size;
-        if (cur >= target || j + 1 == S) {
-            if (bad >= first && bad <= j) break;  // this batch holds the failing member
-            dec.chunk_ends.push_back(off[j + 1]);
-            if (cur >= target) target = std::min(target * 2, tmax);
-            cur = 0;
-            first = j + 1;
-        }
-    }
-    if (bad < S) {
-        dec.failed = true;
-        set_err(&dec.error, RRG_E_INVALID, "invalid deflate stream: libdeflate gzip decode failed");
-        return RRG_OK;
-    }
-    if (walk_trunc) {
-        dec.failed = true;
-        set_err(&dec.error, RRG_E_TRUNCATED, "truncated gzip stream");
-        return RRG_OK;
-    }
-    if (pos < len) {
-        // remainder: the first non-BGZF member onwards (pipeline.rs:1622-1633)
-        uint32_t meth = 0;
-        const long hl = parse_header(h + pos, len - pos, &meth);
-        if (hl < 0) {
+        if (J.pre_failed) {
             dec.failed = true;
-            status_err(&dec.error, (uint32_t)-hl, 0);
-            dec.error.method = meth;
-            return RRG_OK;
+            dec.error = J.pre_err;
+            continue;
         }
-        Decoded rest;
-        rc = member_path(c, G, pos + (uint64_t)hl, len, cfg, U, rest, err);
-        if (rc) return rc;
-        for (uint64_t e : rest.chunk_ends) dec.chunk_ends.push_back(e);
-        if (rest.failed) {
-            dec.failed = true;
-            dec.error = rest.error;
-        }
+        if (J.body == UINT64_MAX) continue;
+        std::vector<uint64_t> off_mp(J.off.begin() + nb, J.off.end());
+        for (uint64_t& v : off_mp) v -= J.off[nb];
+        finish_member_path(J, desc, res, recs, flags, raw, J.r_lo + nb, J.off[nb], off_mp);
     }
-    (void)G;
     return RRG_OK;
 }
 
-// Decode the file already in c.in (and, for framing walks, at host pointer h).
+// Decode the single file already in c.in (and at host pointer h).
 int decode_loaded(Ctx& c, const uint8_t* h, uint64_t len, const rrg_cfg& cfg, Decoded& dec, rrg_error* err) {
     const Geometry G = pick_geometry(c, cfg);
-    if (bgzf_block_size(h, len)) return bgzf_path(c, G, h, len, cfg, dec, err);
-    uint32_t meth = 0;
-    const long hl = parse_header(h, len, &meth);
-    if (hl < 0) {
-        dec.failed = true;
-        status_err(&dec.error, (uint32_t)-hl, 0);
-        dec.error.method = meth;
-        return RRG_OK;
-    }
-    return member_path(c, G, (uint64_t)hl, len, cfg, 0, dec, err);
+    std::vector<Job> jobs(1);
+    jobs[0].in_off = 0;
+    jobs[0].len = len;
+    jobs[0].h = h;
+    jobs[0].chunk_size = cfg.chunk_size_bytes;
+    jobs[0].segment_bytes = cfg.segment_bytes;
+    const int rc = decode_batch(c, G, jobs, c.out, err);
+    if (rc) return rc;
+    dec = jobs[0].dec;
+    return RRG_OK;
 }
 
 int load_host(Ctx& c, const uint8_t* file, size_t len, rrg_error* err) {
     HIPCHK(c.in.ensure(len + 64));
     HIPCHK(hipMemcpyAsync(c.in.p, file, len, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemsetAsync(c.in.as<uint8_t>() + len, 0, 64, c.s));
+    c.in_used = len + 64;
     return RRG_OK;
 }
 
@@ -902,7 +1001,7 @@ void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);
     cfg->chunk_size_bytes = 4u << 20;
     cfg->lut_bits = 9;
-    cfg->lanes_per_block = 32;
+    cfg->lanes_per_block = 16;
 }
 
 void* rrg_gpu_create(int device) {
@@ -955,6 +1054,7 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
     HIPCHK(c.in.ensure(len + 64));
     HIPCHK(hipMemcpyAsync(c.in.p, d_file, len, hipMemcpyDeviceToDevice, c.s));
     HIPCHK(hipMemsetAsync(c.in.as<uint8_t>() + len, 0, 64, c.s));
+    c.in_used = len + 64;
     std::vector<uint8_t> hcopy;
     const uint8_t* h = h_file;
     if (!h) {

@@ -283,8 +283,9 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     return true;
 }
 
-__global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t nbits,
+__global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
                                                                 const uint64_t* __restrict__ targets,
+                                                                const uint64_t* __restrict__ limits,
                                                                 uint64_t* __restrict__ found, uint32_t n_targets) {
     __shared__ uint8_t s_next[8192];
     __shared__ FindWave s_wave[FIND_WAVES];
@@ -302,8 +303,10 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     if (t >= n_targets) return;
     FindWave& fw = s_wave[threadIdx.x >> 6];
     uint64_t cur = uni64(targets[t]);
-    const uint64_t limit = nbits;
-    const uint64_t nvalid = (nbits >> 6) + 2;  // words that exist (the input is padded)
+    // search limit: the end of the target's file (several files may share the buffer)
+    const uint64_t limit = uni64(limits[t]);
+    const uint64_t nbits = limit;
+    const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
     uint64_t rbase = ~0ull;
     bool end = (cur + 13 > limit);
@@ -354,11 +357,11 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     if (lane == 0) found[t] = result;
 }
 
-hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t nbits, const uint64_t* d_targets, uint64_t* d_found,
-                              uint32_t n_targets, hipStream_t s) {
+hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
+                              const uint64_t* d_limits, uint64_t* d_found, uint32_t n_targets, hipStream_t s) {
     if (n_targets == 0) return hipSuccess;
     hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
-                       d_in64, nbits, d_targets, d_found, n_targets);
+                       d_in64, total_bits, d_targets, d_limits, d_found, n_targets);
     return hipGetLastError();
 }
 
@@ -488,8 +491,7 @@ __device__ int parse_gz_header(const uint64_t* __restrict__ in64, uint64_t len, 
 enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_STORED, S_BLOCK_END, S_EXIT };
 
 template <int LB>
-__global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in64, uint64_t nbits, uint64_t in_len,
-                                                const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
+__global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
                                                 MemberRec* __restrict__ mrec, const uint32_t* __restrict__ work,
                                                 uint32_t n_work, uint32_t* __restrict__ counter,
                                                 uint16_t* __restrict__ cells, uint16_t* __restrict__ sorted_scratch) {
@@ -513,6 +515,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     int state = S_FETCH;
     uint32_t seg = 0;
     uint64_t pos = 0, hint = 0, bitbuf = 0;
+    uint64_t nbits = 0, in_len = 0;  // end of the segment's file in the input buffer
     uint32_t bitcnt = 0;
     uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
     uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
@@ -557,6 +560,8 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 const SegDesc d = segs[seg];
                 pos = d.start_bit;
                 hint = d.hint_bit;
+                in_len = d.end_byte;
+                nbits = d.end_byte * 8;
                 out_base = d.cells_base;
                 out_cap = d.cells_cap;
                 out_n = 0;
@@ -836,24 +841,23 @@ int inflate_blocks_per_cu(int lut_bits, int lanes) {
     return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_len, const SegDesc* d_segs,
-                          SegResult* d_res, MemberRec* d_mrec, const uint32_t* d_work, uint32_t n_work,
-                          uint32_t* d_counter, uint16_t* d_cells, uint16_t* d_sorted, const InflateGeom& g,
-                          hipStream_t s) {
+hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                          const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
+                          uint16_t* d_sorted, const InflateGeom& g, hipStream_t s) {
     if (n_work == 0) return hipSuccess;
     const size_t shm = (size_t)g.lanes_per_block * inflate_lds_per_lane(g.lut_bits);
     switch (g.lut_bits) {
         case 8:
-            hipLaunchKernelGGL(k_inflate<8>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits, in_len,
-                               d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_inflate<8>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
             break;
         case 10:
-            hipLaunchKernelGGL(k_inflate<10>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits,
-                               in_len, d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_inflate<10>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
             break;
         default:
-            hipLaunchKernelGGL(k_inflate<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, nbits, in_len,
-                               d_segs, d_res, d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_inflate<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
             break;
     }
     return hipGetLastError();
@@ -878,9 +882,11 @@ hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_le
 //      the literal byte of its block's entry window.
 //
 // After phase 3 every tail-zone cell is a literal, so k_resolve needs one
-// gather per marker.  Entry-window cells carry 0x100 for bytes before the
-// start of the output (a marker reaching there is the reference's
-// "marker references bytes outside the available prefix window").
+// gather per marker.  Window cells carry 0x100 for bytes before the start of
+// the output (a marker reaching there is the reference's "marker references
+// bytes outside the available prefix window").  Several files can share one
+// chain: a segment flagged `first` starts a new output, and the window is
+// reset to 0x100 in front of it, so no marker reaches into another file.
 // ===========================================================================
 constexpr uint32_t WIN = 32768;
 
@@ -889,6 +895,7 @@ __device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, u
     uint64_t rem = (uint64_t)off + 1;
     int64_t s = (int64_t)c - 1;
     while (s >= 0 && rem > cs[s].len) {
+        if (cs[s].first) return false;  // before the start of this file's output
         rem -= cs[s].len;
         --s;
     }
@@ -902,10 +909,15 @@ __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict_
                                                      uint16_t* __restrict__ cells, uint16_t* __restrict__ exit_tail) {
     __shared__ uint16_t win[WIN];
     const uint32_t B = blockIdx.x, tid = threadIdx.x;
-    for (uint32_t i = tid; i < WIN; i += 1024) win[i] = (uint16_t)(0x8000u | (WIN - 1 - i));
-    __syncthreads();
     const uint32_t first = B * G, end = min(n_seg, first + G);
+    const bool fresh = first < n_seg && cs[first].first;
+    for (uint32_t i = tid; i < WIN; i += 1024) win[i] = fresh ? (uint16_t)0x100 : (uint16_t)(0x8000u | (WIN - 1 - i));
+    __syncthreads();
     for (uint32_t s = first; s < end; ++s) {
+        if (cs[s].first && s != first) {  // a new file: nothing before it is reachable
+            for (uint32_t i = tid; i < WIN; i += 1024) win[i] = 0x100;
+            __syncthreads();
+        }
         const uint64_t len = cs[s].len;
         const uint32_t tl = (uint32_t)(len < WIN ? len : WIN);
         if (tl == 0) continue;
@@ -973,9 +985,10 @@ __global__ __launch_bounds__(256) void k_tail_patch(const ChainSeg* __restrict__
     const uint16_t* en = entry + (size_t)(s / G) * WIN;
     for (uint32_t k = threadIdx.x; k < tl; k += 256) {
         const uint16_t c = cells[tz0 + k];
-        if (c & 0x8000u) {
-            const uint16_t b = en[WIN - 1 - (c & 0x7fffu)];
-            if (b & 0x100u) atomicMin(err, s);
+        if (c & 0x8100u) {
+            // a marker still relative to the block entry, or a byte before the output start
+            const uint16_t b = (c & 0x8000u) ? en[WIN - 1 - (c & 0x7fffu)] : c;
+            if (b & 0x100u) err[s] = 1u;
             cells[tz0 + k] = (uint16_t)(b & 0xffu);
         }
     }
@@ -1034,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__
                 if (marker_target(cs, t.seg, cell & 0x7fffu, &s, &local)) {
                     cell = cells[cs[s].cells_base + local];
                 } else {
-                    atomicMin(err, t.seg);
+                    err[t.seg] = 1u;
                     cell = 0;
                 }
             }

@@ -7,10 +7,11 @@
 
 namespace rrg {
 
-// Block finder: found[i] = first verified dynamic-block offset >= targets[i]
-// (UINT64_MAX when none).  One wavefront per target.
-hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t nbits, const uint64_t* d_targets,
-                              uint64_t* d_found, uint32_t n_targets, hipStream_t s);
+// Block finder: found[i] = first verified dynamic-block offset in
+// [targets[i], limits[i]) (UINT64_MAX when none).  One wavefront per target.
+// total_bits: size of the whole (padded) input buffer.
+hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
+                              const uint64_t* d_limits, uint64_t* d_found, uint32_t n_targets, hipStream_t s);
 
 // Persistent inflate over the segment indices in d_work[0..n_work).
 struct InflateGeom {
@@ -20,18 +21,18 @@ struct InflateGeom {
 };
 int inflate_lds_per_lane(int lut_bits);
 int inflate_blocks_per_cu(int lut_bits, int lanes);
-hipError_t launch_inflate(const uint64_t* d_in64, uint64_t nbits, uint64_t in_len, const SegDesc* d_segs,
-                          SegResult* d_res, MemberRec* d_mrec, const uint32_t* d_work, uint32_t n_work,
-                          uint32_t* d_counter, uint16_t* d_cells, uint16_t* d_sorted, const InflateGeom& g,
-                          hipStream_t s);
+hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                          const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
+                          uint16_t* d_sorted, const InflateGeom& g, hipStream_t s);
 
 // Tail chain: blocked three-phase scan over blocks of G chain segments.
-// d_exit / d_entry hold ceil(n_seg/G) * 32768 u16 each; *d_err receives the
-// lowest segment whose marker reaches before the start of the output.
+// d_exit / d_entry hold ceil(n_seg/G) * 32768 u16 each; d_err[s] is set to 1
+// for a chain segment s whose marker reaches before the start of its file's
+// output (d_err holds n_seg words, cleared by the caller).
 hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_exit,
                        uint16_t* d_entry, uint32_t* d_err, hipStream_t s);
 
-// Resolve markers and narrow u16 cells to u8 output.
+// Resolve markers and narrow u16 cells to u8 output (d_err as for launch_tail).
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs,
                           const uint16_t* d_cells, uint8_t* d_out, uint32_t* d_err, hipStream_t s);
 
