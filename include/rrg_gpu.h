@@ -19,6 +19,14 @@
  *   rrg_gpu_decode_device  no reference counterpart: the device-resident
  *                          form (input and output in HBM) used to measure
  *                          the kernels without PCIe transfers
+ *   rrg_gpu_decode_batch   many independent read_gz calls (lib.rs:366-496)
+ *                          served by one pass of every kernel; BGZF members
+ *                          (pipeline.rs:1357-1636) and speculative segments
+ *                          of all files share one persistent work queue
+ *   rrg_server_*           serving form: a request loop whose admission
+ *                          holds device bytes under a budget, the analogue
+ *                          of InflightCap (pipeline.rs:272-375), with
+ *                          continuous batching over rrg_gpu_decode_batch
  */
 #ifndef RRG_GPU_H
 #define RRG_GPU_H
@@ -29,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RRG_GPU_ABI_VERSION 1
+#define RRG_GPU_ABI_VERSION 2
 
 /* Error kinds; values match the variants of the reference error enums. */
 enum rrg_error_kind {
@@ -121,6 +129,75 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
                           size_t* n_chunks, rrg_stats* stats, rrg_error* err);
 
 void rrg_gpu_last_profile(void* ctx, rrg_profile* out);
+
+/* ---- batched form ----------------------------------------------------- */
+
+/* One file of a batch.  Inputs: d_file (device memory) or, when NULL, h_file
+ * (host memory, uploaded); h_file is optional when d_file is given (it spares
+ * a device-to-host copy of the headers).  Outputs are written by the call. */
+typedef struct {
+    const void* d_file;
+    const uint8_t* h_file;
+    size_t len;
+    uint64_t out_off;            /* first decompressed byte of this file in *d_out */
+    uint64_t out_len;            /* bytes decoded (the streamed prefix when error.kind != 0) */
+    const uint64_t* chunk_ends;  /* chunk end offsets relative to out_off; owned by ctx until its next call */
+    size_t n_chunks;
+    rrg_stats stats;
+    rrg_error error;             /* error.kind == RRG_OK: the file decoded and every trailer checked */
+} rrg_batch_item;
+
+/* Decode n independent gzip files with one launch of each kernel.  Each file
+ * gets exactly the result, chunk boundaries and error of a separate
+ * rrg_gpu_decode_device call.  Returns RRG_OK when the batch ran (per-file
+ * errors are in items[i].error) or an error kind for the batch as a whole
+ * (device failure, bad argument).  *d_out: the batch output, owned by ctx and
+ * valid until its next call. */
+int rrg_gpu_decode_batch(void* ctx, rrg_batch_item* items, size_t n, const rrg_cfg* cfg, const uint8_t** d_out,
+                         rrg_error* err);
+
+/* ---- serving form ----------------------------------------------------- */
+
+typedef struct {
+    size_t hbm_budget_bytes;  /* device bytes that admitted requests may hold (inputs in flight + outputs not yet
+                                 released); 0 = half of the free device memory at creation */
+    size_t max_batch_bytes;   /* compressed bytes per batch; 0 = 512 MiB */
+    int max_batch_items;      /* files per batch; 0 = 4096 */
+    rrg_cfg cfg;              /* decode configuration of every request */
+} rrg_server_opts;
+
+typedef struct {
+    uint64_t id;              /* from rrg_server_submit */
+    uint64_t tag;             /* caller's tag */
+    const uint8_t* d_out;     /* decompressed bytes in device memory, valid until rrg_server_release(id) */
+    uint64_t out_len;
+    uint64_t n_chunks;
+    uint64_t batch;           /* sequence number of the batch that decoded it */
+    uint32_t batch_items;     /* files in that batch */
+    double ms_queue;          /* submit -> start of its batch's decode */
+    double ms_total;          /* submit -> completion */
+    rrg_stats stats;
+    rrg_error error;          /* error.kind == RRG_OK on success */
+} rrg_completion;
+
+/* Start a server (one device context, a staging thread and a decode thread). */
+void* rrg_server_create(int device, const rrg_server_opts* opts);
+/* Queue a request; the input is copied, so `file` may be reused on return.
+ * Returns its id (>= 1), or 0 when the server is shutting down. */
+uint64_t rrg_server_submit(void* srv, const uint8_t* file, size_t len, uint64_t tag);
+/* Up to `max` completions in completion order; waits up to timeout_ms (< 0:
+ * forever) for the first one.  Returns the number written. */
+int rrg_server_poll(void* srv, rrg_completion* out, int max, int timeout_ms);
+/* Copy a completed request's chunk ends (relative to its output) into ends[0..cap). */
+size_t rrg_server_chunk_ends(void* srv, uint64_t id, uint64_t* ends, size_t cap);
+/* Copy a completed request's output to host memory (at most cap bytes); RRG_OK or an error kind. */
+int rrg_server_copy_out(void* srv, uint64_t id, uint8_t* host, size_t cap);
+/* Return a completed request's output memory to the budget. */
+int rrg_server_release(void* srv, uint64_t id);
+/* Requests submitted and not yet completed. */
+uint64_t rrg_server_pending(void* srv);
+/* Finish the queued requests, stop the threads, free device memory. */
+void rrg_server_destroy(void* srv);
 
 #ifdef __cplusplus
 }

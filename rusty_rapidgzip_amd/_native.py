@@ -37,9 +37,9 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_gpu_lib() -> str:
-    srcs = [os.path.join(CSRC, "rrg_kernels.hip"), os.path.join(CSRC, "rrg_gpu.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("rrg_kernels.hip", "rrg_gpu.cpp", "rrg_serve.cpp")]
     tmp = GPU_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO_DIR, "include"), *srcs, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO_DIR, "include"), *srcs, "-o", tmp, "-lpthread"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

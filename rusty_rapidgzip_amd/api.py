@@ -85,6 +85,45 @@ class Profile(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class _BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("d_file", ctypes.c_void_p),
+        ("h_file", ctypes.c_void_p),
+        ("len", ctypes.c_size_t),
+        ("out_off", ctypes.c_uint64),
+        ("out_len", ctypes.c_uint64),
+        ("chunk_ends", ctypes.POINTER(ctypes.c_uint64)),
+        ("n_chunks", ctypes.c_size_t),
+        ("stats", _Stats),
+        ("error", _Err),
+    ]
+
+
+class _ServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_size_t),
+        ("max_batch_bytes", ctypes.c_size_t),
+        ("max_batch_items", ctypes.c_int),
+        ("cfg", _Cfg),
+    ]
+
+
+class _Completion(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("tag", ctypes.c_uint64),
+        ("d_out", ctypes.c_void_p),
+        ("out_len", ctypes.c_uint64),
+        ("n_chunks", ctypes.c_uint64),
+        ("batch", ctypes.c_uint64),
+        ("batch_items", ctypes.c_uint32),
+        ("ms_queue", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("stats", _Stats),
+        ("error", _Err),
+    ]
+
+
 _CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
 
 # every symbol include/rrg_gpu.h declares
@@ -97,6 +136,15 @@ EXPORTED_SYMBOLS = (
     "rrg_gpu_decode",
     "rrg_gpu_decode_device",
     "rrg_gpu_last_profile",
+    "rrg_gpu_decode_batch",
+    "rrg_server_create",
+    "rrg_server_submit",
+    "rrg_server_poll",
+    "rrg_server_chunk_ends",
+    "rrg_server_copy_out",
+    "rrg_server_release",
+    "rrg_server_pending",
+    "rrg_server_destroy",
 )
 
 
@@ -117,6 +165,22 @@ def _lib() -> ctypes.CDLL:
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
             ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
         lib.rrg_gpu_last_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(Profile)]
+        lib.rrg_gpu_decode_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(_BatchItem), ctypes.c_size_t,
+                                             ctypes.POINTER(_Cfg), ctypes.POINTER(ctypes.c_void_p),
+                                             ctypes.POINTER(_Err)]
+        lib.rrg_server_create.restype = ctypes.c_void_p
+        lib.rrg_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(_ServerOpts)]
+        lib.rrg_server_submit.restype = ctypes.c_uint64
+        lib.rrg_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]
+        lib.rrg_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Completion), ctypes.c_int, ctypes.c_int]
+        lib.rrg_server_chunk_ends.restype = ctypes.c_size_t
+        lib.rrg_server_chunk_ends.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.c_size_t]
+        lib.rrg_server_copy_out.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t]
+        lib.rrg_server_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        lib.rrg_server_pending.restype = ctypes.c_uint64
+        lib.rrg_server_pending.argtypes = [ctypes.c_void_p]
+        lib.rrg_server_destroy.argtypes = [ctypes.c_void_p]
         lib._rrg_typed = True
     return lib
 
@@ -138,7 +202,7 @@ class Config:
     verbose: Verbosity = Verbosity.Off
     segment_bytes: int = 0
     lut_bits: int = 9
-    lanes_per_block: int = 32
+    lanes_per_block: int = 16
 
     def _c(self) -> _Cfg:
         c = _Cfg()
@@ -205,29 +269,63 @@ class SinkClosed(Exception):
     """Raised by a sink to end the stream (the reference's dropped Receiver)."""
 
 
-def _raise(rc: int, e: _Err) -> None:
+def _error(rc: int, e: _Err) -> RapidgzipError:
     msg = e.message.decode(errors="replace")
     if rc == RRG_E_EOF:
-        raise DeflateError("UnexpectedEof", msg)
+        return DeflateError("UnexpectedEof", msg)
     if rc == RRG_E_INVALID:
-        raise DeflateError("Invalid", msg)
+        return DeflateError("Invalid", msg)
     if rc == RRG_E_BADMAGIC:
-        raise GzipError("BadMagic", msg)
+        return GzipError("BadMagic", msg)
     if rc == RRG_E_METHOD:
-        raise GzipError("UnsupportedMethod", msg, method=e.method)
+        return GzipError("UnsupportedMethod", msg, method=e.method)
     if rc == RRG_E_RESERVED:
-        raise GzipError("ReservedFlags", msg, flags=e.method)
+        return GzipError("ReservedFlags", msg, flags=e.method)
     if rc == RRG_E_TRUNCATED:
-        raise GzipError("Truncated", msg)
+        return GzipError("Truncated", msg)
     if rc == RRG_E_CRC:
-        raise GzipError("CrcMismatch", msg, expected=e.expected, got=e.got, member=e.member,
-                        uncompressed=e.got64, trailer_byte=e.trailer_byte)
+        return GzipError("CrcMismatch", msg, expected=e.expected, got=e.got, member=e.member,
+                         uncompressed=e.got64, trailer_byte=e.trailer_byte)
     if rc == RRG_E_ISIZE:
-        raise GzipError("IsizeMismatch", msg, expected=e.expected, got=e.got64, member=e.member,
-                        trailer_byte=e.trailer_byte)
+        return GzipError("IsizeMismatch", msg, expected=e.expected, got=e.got64, member=e.member,
+                         trailer_byte=e.trailer_byte)
     if rc == RRG_E_IO:
-        raise IoError(msg)
-    raise DeviceError(msg or f"rrg_gpu error {rc}")
+        return IoError(msg)
+    return DeviceError(msg or f"rrg_gpu error {rc}")
+
+
+def _raise(rc: int, e: _Err) -> None:
+    raise _error(rc, e)
+
+
+def _host_ptr(data):
+    """(address, keep-alive object) of a bytes / bytearray / numpy buffer, without copying bytes objects."""
+    if data is None:
+        return None, None
+    if isinstance(data, bytes):
+        return ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p).value, data
+    import numpy as np
+
+    arr = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytearray, memoryview))
+                               else np.asarray(data, dtype=np.uint8))
+    return (arr.ctypes.data if arr.size else None), arr
+
+
+def _stats(st: _Stats) -> DecodeStats:
+    return DecodeStats(st.compressed_bytes, st.uncompressed_bytes, st.chunks_decoded, st.speculation_failures)
+
+
+@dataclass
+class BatchResult:
+    """One file of a batched decode: its output slice, chunk ends (relative to
+    the slice), stats, and the error it ended with (None on success)."""
+
+    d_out: int
+    out_len: int
+    chunk_ends: list
+    stats: DecodeStats
+    error: RapidgzipError | None
+    data: bytes | None = None
 
 
 def _sink_fn(sink):
@@ -325,16 +423,7 @@ class Decoder:
         n = ctypes.c_size_t()
         st, er = _Stats(), _Err()
         # host copy of the file (framing walks only), passed without copying
-        hptr = None
-        if host is not None:
-            if isinstance(host, bytes):
-                hptr = ctypes.cast(ctypes.c_char_p(host), ctypes.c_void_p).value
-            else:
-                import numpy as np
-
-                arr = np.ascontiguousarray(host, dtype=np.uint8)
-                host = arr  # keep alive for the call
-                hptr = arr.ctypes.data
+        hptr, _keep = _host_ptr(host)
         rc = self._lib.rrg_gpu_decode_device(self._ctx, ctypes.c_void_p(d_ptr), length, hptr,
                                              ctypes.byref(config._c()), ctypes.byref(d_out), ctypes.byref(out_len),
                                              ends, max_chunks, ctypes.byref(n), ctypes.byref(st), ctypes.byref(er))
@@ -342,6 +431,143 @@ class Decoder:
             _raise(rc, er)
         stats = DecodeStats(st.compressed_bytes, st.uncompressed_bytes, st.chunks_decoded, st.speculation_failures)
         return d_out.value or 0, out_len.value, [ends[i] for i in range(min(n.value, max_chunks))], stats
+
+    def decode_batch(self, files, config: Config | None = None, device_ptrs=None, device_lens=None,
+                     copy: bool = True):
+        """Decode independent gzip files with one pass of every kernel.
+
+        ``files``: host buffers (bytes / numpy), or None where ``device_ptrs[i]``
+        gives the file in device memory (then ``files[i]`` may still be given as
+        the host copy, or left None) with its length in ``device_lens[i]``.
+        Returns one :class:`BatchResult` per file;
+        with ``copy`` the decoded bytes are copied back into ``result.data``.
+        Device outputs stay valid until this decoder's next call."""
+        config = config or Config()
+        n = len(files)
+        items = (_BatchItem * max(1, n))()
+        keep = []
+        for i, f in enumerate(files):
+            hp, k = _host_ptr(f)
+            keep.append(k)
+            items[i].h_file = hp
+            dp = device_ptrs[i] if device_ptrs is not None else None
+            items[i].d_file = dp or None
+            if dp:
+                items[i].len = int(device_lens[i])
+            elif f is not None:
+                items[i].len = len(f) if not hasattr(f, "nbytes") else int(f.nbytes)
+            else:
+                raise ValueError(f"batch file {i} has neither host bytes nor a device pointer")
+        d_out = ctypes.c_void_p()
+        er = _Err()
+        rc = self._lib.rrg_gpu_decode_batch(self._ctx, items, n, ctypes.byref(config._c()), ctypes.byref(d_out),
+                                            ctypes.byref(er))
+        if rc != RRG_OK:
+            _raise(rc, er)
+        base = d_out.value or 0
+        out = []
+        if copy:
+            from . import hipmem
+        for i in range(n):
+            it = items[i]
+            ends = [it.chunk_ends[k] for k in range(it.n_chunks)]
+            r = BatchResult(base + it.out_off, it.out_len, ends, _stats(it.stats),
+                            _error(it.error.kind, it.error) if it.error.kind else None)
+            if copy:
+                r.data = hipmem.to_host(r.d_out, r.out_len).tobytes() if r.out_len else b""
+            out.append(r)
+        return out
+
+
+@dataclass
+class Completion:
+    """A finished server request."""
+
+    id: int
+    tag: int
+    d_out: int
+    out_len: int
+    n_chunks: int
+    batch: int
+    batch_items: int
+    ms_queue: float
+    ms_total: float
+    stats: DecodeStats
+    error: RapidgzipError | None
+
+
+class Server:
+    """Serving loop on one GPU (rrg_server_*): requests are queued, admitted
+    while the device bytes they hold stay under ``hbm_budget_bytes``, and
+    decoded in batches formed from whatever is queued when the previous batch
+    finishes.  Outputs stay in device memory until :meth:`release`."""
+
+    def __init__(self, device: int = 0, config: Config | None = None, hbm_budget_bytes: int = 0,
+                 max_batch_bytes: int = 0, max_batch_items: int = 0):
+        self._lib = _lib()
+        o = _ServerOpts()
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        o.max_batch_bytes = int(max_batch_bytes)
+        o.max_batch_items = int(max_batch_items)
+        o.cfg = (config or Config())._c()
+        self._srv = self._lib.rrg_server_create(int(device), ctypes.byref(o))
+        if not self._srv:
+            raise DeviceError(f"cannot start a server on HIP device {device}")
+
+    def submit(self, data, tag: int = 0) -> int:
+        hp, _k = _host_ptr(data)
+        n = len(data) if not hasattr(data, "nbytes") else int(data.nbytes)
+        rid = self._lib.rrg_server_submit(self._srv, hp, n, int(tag))
+        if not rid:
+            raise IoError("server is shutting down")
+        return rid
+
+    def poll(self, max_items: int = 64, timeout_ms: int = -1) -> list:
+        buf = (_Completion * max_items)()
+        k = self._lib.rrg_server_poll(self._srv, buf, max_items, int(timeout_ms))
+        out = []
+        for i in range(k):
+            c = buf[i]
+            out.append(Completion(c.id, c.tag, c.d_out or 0, c.out_len, c.n_chunks, c.batch, c.batch_items,
+                                  c.ms_queue, c.ms_total, _stats(c.stats),
+                                  _error(c.error.kind, c.error) if c.error.kind else None))
+        return out
+
+    def chunk_ends(self, rid: int) -> list:
+        n = self._lib.rrg_server_chunk_ends(self._srv, rid, None, 0)
+        arr = (ctypes.c_uint64 * max(1, n))()
+        self._lib.rrg_server_chunk_ends(self._srv, rid, arr, n)
+        return [arr[i] for i in range(n)]
+
+    def result(self, rid: int, out_len: int) -> bytes:
+        buf = (ctypes.c_uint8 * max(1, out_len))()
+        rc = self._lib.rrg_server_copy_out(self._srv, rid, ctypes.addressof(buf), out_len)
+        if rc != RRG_OK:
+            raise DeviceError(f"copy of request {rid} failed ({rc})")
+        return bytes(buf)[:out_len]
+
+    def release(self, rid: int) -> None:
+        self._lib.rrg_server_release(self._srv, rid)
+
+    def pending(self) -> int:
+        return self._lib.rrg_server_pending(self._srv)
+
+    def close(self) -> None:
+        if self._srv:
+            self._lib.rrg_server_destroy(self._srv)
+            self._srv = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default: dict[int, Decoder] = {}

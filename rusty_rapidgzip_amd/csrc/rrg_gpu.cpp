@@ -28,10 +28,10 @@
 #include <cstring>
 #include <vector>
 
-#include "rrg_launch.h"
+#include "rrg_engine.h"
 
-namespace {
-using namespace rrg;
+namespace rrg {
+namespace host {
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -168,109 +168,8 @@ double now_ms() {
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        const size_t c = n + n / 8 + 4096;
-        hipError_t e = hipMalloc(&p, c);
-        if (e == hipSuccess) cap = c;
-        return e;
-    }
-    // grow keeping the first `used` bytes
-    hipError_t grow_keep(size_t n, size_t used, hipStream_t s) {
-        if (n <= cap) return hipSuccess;
-        void* q = nullptr;
-        const size_t c = n + n / 4 + 4096;
-        hipError_t e = hipMalloc(&q, c);
-        if (e != hipSuccess) return e;
-        if (p && used) {
-            e = hipMemcpyAsync(q, p, used, hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) {
-                (void)hipFree(q);
-                return e;
-            }
-        }
-        if (p) (void)hipFree(p);
-        p = q;
-        cap = c;
-        return hipSuccess;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-struct Ctx {
-    int device = 0;
-    hipStream_t s = nullptr;
-    int n_cu = 0;
-    DevBuf in, targets, limits, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
-        pcrc, err;
-    size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
-    uint8_t* pin = nullptr;
-    size_t pin_cap = 0;
-    rrg_profile prof{};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    ~Ctx() {
-        DevBuf* all[] = {&in,    &targets, &limits, &found, &segs, &res,  &mrec, &work, &counter, &cells,
-                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err};
-        for (DevBuf* b : all) b->release();
-        if (pin) (void)hipHostFree(pin);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
-
-#define HIPCHK(expr)                                                                         \
-    do {                                                                                     \
-        hipError_t _e = (expr);                                                              \
-        if (_e != hipSuccess) {                                                              \
-            set_err(err, RRG_E_DEVICE, "HIP error %s at %s:%d", hipGetErrorString(_e), __FILE__, \
-                    __LINE__);                                                               \
-            return RRG_E_DEVICE;                                                             \
-        }                                                                                    \
-    } while (0)
-
-// Times one group of kernels on the context stream.
-struct StageTimer {
-    Ctx& c;
-    double* acc;
-    StageTimer(Ctx& cc, double* a) : c(cc), acc(a) { (void)hipEventRecord(c.ev0, c.s); }
-    void stop() {
-        (void)hipEventRecord(c.ev1, c.s);
-        (void)hipEventSynchronize(c.ev1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
-        *acc += ms;
-    }
-};
 
 // Result of one file's decode.
-struct Decoded {
-    std::vector<uint64_t> chunk_ends;  // output chunk end offsets, relative to out_base
-    uint64_t out_base = 0;             // the file's first output byte in the batch output buffer
-    uint64_t spec_failures = 0;
-    bool failed = false;
-    rrg_error error{};
-};
-
-struct Geometry {
-    InflateGeom g;
-    uint64_t lanes_in_flight;
-};
 
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
@@ -448,42 +347,6 @@ inline uint32_t std_crc(uint32_t raw, uint64_t len) { return raw ^ gf2_mul(x8n(l
 // chain + resolve over the concatenated chains, one CRC launch over every
 // member of every file.  A single-file decode is a batch of one.
 // ---------------------------------------------------------------------------
-struct Job {
-    // input
-    uint64_t in_off = 0;         // the file's first byte in c.in (multiple of 8)
-    uint64_t len = 0;            // file length
-    const uint8_t* h = nullptr;  // host copy of the file: headers and BGZF framing are read here
-    size_t chunk_size = 4u << 20;
-    size_t segment_bytes = 0;
-    // host plan
-    struct Mb {
-        uint64_t off, size;  // file offsets
-    };
-    std::vector<Mb> mem;           // BGZF members
-    std::vector<uint8_t> hdr_err;  // per BGZF member
-    bool bgzf = false, walk_trunc = false;
-    bool pre_failed = false;  // header error found while planning (reported after any BGZF output)
-    rrg_error pre_err{};
-    uint64_t body = UINT64_MAX;  // file offset of the speculative (member-path) deflate body; none if MAX
-    // targets and segments (index ranges in the batch arrays)
-    size_t t_lo = 0, t_hi = 0;
-    size_t bg_lo = 0, bg_hi = 0, mp_lo = 0, mp_hi = 0;
-    std::vector<uint8_t> seg_is_chunk;  // per member-path segment
-    std::vector<uint64_t> orig_start;
-    std::vector<uint8_t> first_ok;
-    size_t n_chain = 0;
-    long fatal = -1;
-    bool saw_final = false;
-    // output
-    std::vector<uint64_t> off;  // job-relative output offset: BGZF members, then chain segments, then the end
-    uint64_t U = 0;
-    size_t c_lo = 0;  // first chain segment of this job in the batch chain
-    size_t r_lo = 0;  // first CRC range
-    Decoded dec;
-
-    uint64_t abs_bits(uint64_t file_byte) const { return (in_off + file_byte) * 8; }
-    uint64_t end_bits() const { return (in_off + len) * 8; }
-};
 
 // Host planning: BGZF framing walk (pipeline.rs:1357-1420) or the first header
 // (lib.rs:463-472, gzip.rs:252-300).
@@ -988,7 +851,24 @@ void print_profile(const Ctx& c) {
                  (unsigned long long)p.n_segments);
 }
 
-}  // namespace
+bool init_ctx(Ctx& c, int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return false;
+    if (hipSetDevice(device) != hipSuccess) return false;
+    c.device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c.s) != hipSuccess ||
+        hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess)
+        return false;
+    c.n_cu = prop.multiProcessorCount;
+    return true;
+}
+
+}  // namespace host
+}  // namespace rrg
+
+using namespace rrg;
+using namespace rrg::host;
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1005,18 +885,11 @@ void rrg_cfg_default(rrg_cfg* cfg) {
 }
 
 void* rrg_gpu_create(int device) {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
-    if (hipSetDevice(device) != hipSuccess) return nullptr;
     Ctx* c = new Ctx();
-    c->device = device;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c->s) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (!init_ctx(*c, device)) {
         delete c;
         return nullptr;
     }
-    c->n_cu = prop.multiProcessorCount;
     return c;
 }
 
