@@ -95,11 +95,12 @@ typedef struct {
     double ms_resolve;    /* resolve + narrow kernel */
     double ms_crc;        /* CRC kernel */
     uint64_t n_segments;
-    uint64_t n_markers_tail;
+    uint64_t n_redecoded;     /* segments decoded again (false starts, grown output regions) */
     uint64_t n_inflate_launches;
     uint64_t n_tail_rounds;
     uint64_t compressed_bytes;
     uint64_t uncompressed_bytes;
+    uint64_t n_grown;         /* output regions grown after an overflow */
 } rrg_profile;
 
 int rrg_gpu_abi_version(void);

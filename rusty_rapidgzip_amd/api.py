@@ -74,11 +74,12 @@ class Profile(ctypes.Structure):
         ("ms_resolve", ctypes.c_double),
         ("ms_crc", ctypes.c_double),
         ("n_segments", ctypes.c_uint64),
-        ("n_markers_tail", ctypes.c_uint64),
+        ("n_redecoded", ctypes.c_uint64),
         ("n_inflate_launches", ctypes.c_uint64),
         ("n_tail_rounds", ctypes.c_uint64),
         ("compressed_bytes", ctypes.c_uint64),
         ("uncompressed_bytes", ctypes.c_uint64),
+        ("n_grown", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

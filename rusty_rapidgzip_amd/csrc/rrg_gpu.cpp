@@ -390,9 +390,10 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
     uint64_t seg_bytes = J.segment_bytes;
     if (!seg_bytes) {
-        // 64 KiB segments when the file keeps every lane busy with them; down to
-        // 32 KiB for smaller files
-        seg_bytes = 2 * (total / 8) / std::max<uint64_t>(1, G.lanes_in_flight);
+        // about one segment per lane in flight (a lane decodes one segment in
+        // roughly the time the launch takes), between 32 and 64 KiB: shorter
+        // segments cost finder work and speculation, longer ones idle lanes
+        seg_bytes = (total / 8) * 10 / std::max<uint64_t>(1, 11 * G.lanes_in_flight);
         seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 32 * 1024), 64 * 1024);
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
@@ -416,11 +417,12 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
 // Anchored verification chain of one job (pipeline.rs:774-809): accept a
 // segment iff it decoded OK from where its predecessor ended; otherwise queue
 // it for re-decode from there (optimistically continuing with the next
-// segment's own start, verified next round).  Returns nonzero when the chain
-// settled (nothing queued).
-bool chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult>& res, uint64_t& cells_used,
-                 uint64_t& mb_used, std::vector<uint32_t>& redo) {
+// segment's own start, verified next round).  Returns the number of output
+// regions grown.
+uint32_t chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult>& res, uint64_t& cells_used,
+                     uint64_t& mb_used, std::vector<uint32_t>& redo) {
     const size_t before = redo.size();
+    uint32_t grown = 0;
     uint64_t expected = J.abs_bits(J.body);
     J.n_chain = 0;
     J.fatal = -1;
@@ -439,6 +441,7 @@ bool chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult
         if (desc[j].start_bit == expected && overflowed(r.status)) {
             // verified start, output region too small: grow and decode again
             grow_segment(desc[j], r.status, cells_used, mb_used);
+            ++grown;
             redo.push_back((uint32_t)j);
             if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
             continue;
@@ -451,7 +454,7 @@ bool chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult
         redo.push_back((uint32_t)j);
         if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
     }
-    return redo.size() == before;
+    return grown;
 }
 
 // Errors, CRC/ISIZE checks and chunk emission of the member-path part of a
@@ -685,15 +688,17 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
                 for (size_t j = J.bg_lo; j < J.bg_hi; ++j)
                     if (overflowed(res[j].status)) {
                         grow_segment(desc[j], res[j].status, cells_used, mb_used);
+                        ++c.prof.n_grown;
                         redo.push_back((uint32_t)j);
                     }
-            if (J.body != UINT64_MAX) chain_round(J, desc, res, cells_used, mb_used, redo);
+            if (J.body != UINT64_MAX) c.prof.n_grown += chain_round(J, desc, res, cells_used, mb_used, redo);
         }
         if (redo.empty()) break;
         if (round > (int)S + 64) {
             set_err(err, RRG_E_DEVICE, "verification chain did not settle");
             return RRG_E_DEVICE;
         }
+        c.prof.n_redecoded += redo.size();
         rc = run_inflate(c, G, desc, res, cells_used, mb_used, redo, err);
         if (rc) return rc;
     }
@@ -843,11 +848,11 @@ void begin_profile(Ctx& c, uint64_t len) {
 void print_profile(const Ctx& c) {
     const rrg_profile& p = c.prof;
     std::fprintf(stderr,
-                 "[rrg-gpu] %llu -> %llu bytes in %.3f ms | find %.3f inflate %.3f (x%llu) tail %.3f (%llu markers, "
-                 "%llu rounds) resolve %.3f crc %.3f | %llu segments\n",
+                 "[rrg-gpu] %llu -> %llu bytes in %.3f ms | find %.3f inflate %.3f (x%llu, %llu re-decoded, %llu "
+                 "grown) tail %.3f (%llu blocks) resolve %.3f crc %.3f | %llu segments\n",
                  (unsigned long long)p.compressed_bytes, (unsigned long long)p.uncompressed_bytes, p.ms_total,
-                 p.ms_find, p.ms_inflate, (unsigned long long)p.n_inflate_launches, p.ms_tail,
-                 (unsigned long long)p.n_markers_tail, (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc,
+                 p.ms_find, p.ms_inflate, (unsigned long long)p.n_inflate_launches, (unsigned long long)p.n_redecoded,
+                 (unsigned long long)p.n_grown, p.ms_tail, (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc,
                  (unsigned long long)p.n_segments);
 }
 

@@ -517,14 +517,33 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     uint64_t pos = 0, hint = 0, bitbuf = 0;
     uint64_t nbits = 0, in_len = 0;  // end of the segment's file in the input buffer
     uint32_t bitcnt = 0;
+    // input words cq, cq+1, cq+2 held in registers; moving to the next word
+    // loads the one after, so it arrives while symbols are being decoded
+    constexpr uint64_t kNoWord = ~0ull >> 1;
+    uint64_t cq = kNoWord, w0 = 0, w1 = 0, w2 = 0;
     uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
     uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
     uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
 
-#define RRG_REFILL()                      \
-    do {                                  \
-        bitbuf = bits64_at(in64, pos);    \
-        bitcnt = 64;                      \
+// 64 bits at pos (the input buffer is padded, so words up to 8 past the end exist)
+#define RRG_REFILL()                                            \
+    do {                                                        \
+        const uint64_t _q = pos >> 6;                           \
+        if (_q != cq) {                                         \
+            if (_q == cq + 1) {                                 \
+                w0 = w1;                                        \
+                w1 = w2;                                        \
+                w2 = in64[_q + 2];                              \
+            } else {                                            \
+                w0 = in64[_q];                                  \
+                w1 = in64[_q + 1];                              \
+                w2 = in64[_q + 2];                              \
+            }                                                   \
+            cq = _q;                                            \
+        }                                                       \
+        const uint32_t _s = (uint32_t)(pos & 63);               \
+        bitbuf = _s ? (w0 >> _s) | (w1 << (64 - _s)) : w0;     \
+        bitcnt = 64;                                            \
     } while (0)
 #define RRG_CONSUME(n)           \
     do {                         \
@@ -560,6 +579,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 const SegDesc d = segs[seg];
                 pos = d.start_bit;
                 hint = d.hint_bit;
+                cq = kNoWord;
                 in_len = d.end_byte;
                 nbits = d.end_byte * 8;
                 out_base = d.cells_base;
@@ -588,7 +608,8 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
         if (state == S_SYM) {
             // ---- Huffman symbols (fast_inflate.rs:2065-2229) ----
             for (int it = 0; it < 64; ++it) {
-                if (bitcnt < 15) RRG_REFILL();
+                // 48 bits cover a whole match: litlen <= 15 + 5 extra, dist <= 15 + 13 extra
+                if (bitcnt < 48) RRG_REFILL();
                 uint32_t e = litlut[bitbuf & LMASK];
                 uint32_t l = e & 15u, sym = e >> 4;
                 if (l == 0) {
@@ -608,7 +629,6 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     break;
                 }
                 if (sym > 285) RRG_FAIL(ST_INVALID, DET_LITLEN_RANGE);
-                if (bitcnt < 48) RRG_REFILL();
                 const uint32_t li = sym - 257;
                 const uint32_t lx = len_extra(li);
                 if (pos + lx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
