@@ -140,7 +140,8 @@ struct Job {
     // targets and segments (index ranges in the batch arrays)
     size_t t_lo = 0, t_hi = 0;
     size_t bg_lo = 0, bg_hi = 0, mp_lo = 0, mp_hi = 0;
-    std::vector<uint8_t> seg_is_chunk;  // per member-path segment
+    std::vector<uint32_t> order;        // member-path segments (desc indices) in stream order
+    std::vector<uint8_t> seg_is_chunk;  // per position in `order`: the segment starts a reference chunk
     std::vector<uint64_t> orig_start;
     std::vector<uint8_t> first_ok;
     size_t n_chain = 0;

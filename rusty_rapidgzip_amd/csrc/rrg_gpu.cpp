@@ -414,20 +414,28 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     }
 }
 
+// A member-path segment that has to be decoded again: the one at chain
+// position `pos` of job `job`, from bit `start` (its predecessor's end).
+struct Redo {
+    size_t job, pos;
+    uint64_t start;
+    uint32_t status;  // ST_OK: false start; ST_OVERFLOW / ST_MB_OVERFLOW: region too small
+};
+
 // Anchored verification chain of one job (pipeline.rs:774-809): accept a
-// segment iff it decoded OK from where its predecessor ended; otherwise queue
-// it for re-decode from there (optimistically continuing with the next
-// segment's own start, verified next round).  Returns the number of output
-// regions grown.
-uint32_t chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegResult>& res, uint64_t& cells_used,
-                     uint64_t& mb_used, std::vector<uint32_t>& redo) {
+// segment iff it decoded OK from where its predecessor ended; otherwise
+// record it for re-decode from there (optimistically continuing with the next
+// segment's own start, verified next round).
+void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res,
+                 std::vector<Redo>& redo) {
     const size_t before = redo.size();
-    uint32_t grown = 0;
     uint64_t expected = J.abs_bits(J.body);
     J.n_chain = 0;
     J.fatal = -1;
     J.saw_final = false;
-    for (size_t j = J.mp_lo; j < J.mp_hi; ++j) {
+    const size_t n = J.order.size();
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t j = J.order[k];
         const SegResult& r = res[j];
         if (desc[j].start_bit == expected && r.status == ST_OK) {
             ++J.n_chain;
@@ -438,23 +446,13 @@ uint32_t chain_round(Job& J, std::vector<SegDesc>& desc, const std::vector<SegRe
             }
             continue;
         }
-        if (desc[j].start_bit == expected && overflowed(r.status)) {
-            // verified start, output region too small: grow and decode again
-            grow_segment(desc[j], r.status, cells_used, mb_used);
-            ++grown;
-            redo.push_back((uint32_t)j);
-            if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
-            continue;
-        }
-        if (desc[j].start_bit == expected) {  // genuine error at a true boundary
-            if (redo.size() == before) J.fatal = (long)(j - J.mp_lo);
+        if (desc[j].start_bit == expected && !overflowed(r.status)) {  // genuine error at a true boundary
+            if (redo.size() == before) J.fatal = (long)k;
             break;
         }
-        desc[j].start_bit = expected;
-        redo.push_back((uint32_t)j);
-        if (j + 1 < J.mp_hi) expected = desc[j + 1].start_bit;
+        redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK});
+        if (k + 1 < n) expected = desc[J.order[k + 1]].start_bit;
     }
-    return grown;
 }
 
 // Errors, CRC/ISIZE checks and chunk emission of the member-path part of a
@@ -464,7 +462,7 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
                         const std::vector<MemberRec>& recs, const std::vector<uint32_t>& flags,
                         const std::vector<uint32_t>& raw, size_t r_lo, uint64_t base, const std::vector<uint64_t>& off) {
     const size_t n_chain = J.n_chain;
-    const size_t S = J.mp_hi - J.mp_lo;
+    const size_t S = J.order.size();
     Decoded& dec = J.dec;
     std::vector<uint32_t> chunk_of(S);
     {
@@ -477,7 +475,7 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     // speculation failure = a chunk whose guessed start was not where the chain
     // arrived, or whose speculative decode failed
     for (size_t j = 1; j < n_chain; ++j)
-        if (J.seg_is_chunk[j] && (J.orig_start[j] != desc[J.mp_lo + j].start_bit || !J.first_ok[j]))
+        if (J.seg_is_chunk[j] && (J.orig_start[j] != desc[J.order[j]].start_bit || !J.first_ok[j]))
             ++dec.spec_failures;
     const uint64_t body_abs = J.in_off + J.body;
     // earliest failure, as the chunk index it stops the stream at
@@ -485,7 +483,7 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     rrg_error ferr{};
     if (J.fatal >= 0) {
         fail_chunk = chunk_of[J.fatal];
-        status_err(&ferr, res[J.mp_lo + J.fatal].status, res[J.mp_lo + J.fatal].detail);
+        status_err(&ferr, res[J.order[J.fatal]].status, res[J.order[J.fatal]].detail);
     }
     for (size_t k = 0; k < n_chain; ++k)
         if (flags[J.c_lo + (J.bg_hi - J.bg_lo) + k]) {
@@ -500,8 +498,8 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     uint64_t mstart = 0;
     bool stop = false;
     for (size_t j = 0; j < n_chain && !stop; ++j) {
-        for (uint32_t k = 0; k < res[J.mp_lo + j].n_mb; ++k, ++mi) {
-            const MemberRec& m = recs[desc[J.mp_lo + j].mb_base + k];
+        for (uint32_t k = 0; k < res[J.order[j]].n_mb; ++k, ++mi) {
+            const MemberRec& m = recs[desc[J.order[j]].mb_base + k];
             const uint64_t e = off[j] + m.out_off;
             const uint64_t len = e - mstart;
             mstart = e;
@@ -557,6 +555,103 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
         dec.failed = true;
         dec.error = ferr;
     }
+}
+
+// Turn the re-decode requests of one settle round into segments to decode:
+// each requested segment restarts at its verified start and, when it spans
+// enough input, is cut at the blocks the finder reports inside it (one finder
+// launch for all requests); the pieces join the job's chain after it.
+int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vector<SegDesc>& desc,
+               uint64_t& cells_used, uint64_t& mb_used, std::vector<uint32_t>& again, rrg_error* err) {
+    if (redo.empty()) return RRG_OK;
+    constexpr uint64_t kPieceBits = 8 * 8192;  // at least 8 KiB of input per piece
+    constexpr uint64_t kMaxPieces = 8;
+    std::vector<uint64_t> targets, limits;
+    std::vector<size_t> t_first(redo.size() + 1);
+    for (size_t i = 0; i < redo.size(); ++i) {
+        t_first[i] = targets.size();
+        const Redo& R = redo[i];
+        const Job& J = jobs[R.job];
+        const SegDesc& d = desc[J.order[R.pos]];
+        const uint64_t end = d.hint_bit == UINT64_MAX ? J.end_bits() : d.hint_bit;
+        const uint64_t span = end > R.start ? end - R.start : 0;
+        const uint64_t pieces = std::min<uint64_t>(kMaxPieces, span / kPieceBits);
+        for (uint64_t p = 1; p < pieces; ++p) {
+            targets.push_back(R.start + span * p / pieces);
+            limits.push_back(J.end_bits());
+        }
+    }
+    t_first[redo.size()] = targets.size();
+    std::vector<uint64_t> found(targets.size(), UINT64_MAX);
+    if (!targets.empty()) {
+        HIPCHK(c.targets.ensure(targets.size() * 8));
+        HIPCHK(c.limits.ensure(limits.size() * 8));
+        HIPCHK(c.found.ensure(targets.size() * 8));
+        HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), targets.size() * 8, hipMemcpyHostToDevice, c.s));
+        HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), limits.size() * 8, hipMemcpyHostToDevice, c.s));
+        StageTimer t(c, &c.prof.ms_find);
+        HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
+                                  c.limits.as<uint64_t>(), c.found.as<uint64_t>(), (uint32_t)targets.size(), c.s));
+        HIPCHK(hipMemcpyAsync(found.data(), c.found.p, found.size() * 8, hipMemcpyDeviceToHost, c.s));
+        t.stop();
+        HIPCHK(hipStreamSynchronize(c.s));
+        for (size_t i = 0; i < found.size(); ++i)
+            if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= limits[i])) {
+                set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu",
+                        (unsigned long long)found[i], (unsigned long long)targets[i]);
+                return RRG_E_DEVICE;
+            }
+    }
+    // later positions first, so earlier insertions do not move them
+    std::vector<size_t> idx(redo.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        return redo[a].job != redo[b].job ? redo[a].job < redo[b].job : redo[a].pos > redo[b].pos;
+    });
+    for (size_t i : idx) {
+        const Redo& R = redo[i];
+        Job& J = jobs[R.job];
+        const uint32_t j = J.order[R.pos];
+        SegDesc& d = desc[j];
+        const uint64_t hint = d.hint_bit;
+        const uint64_t end = hint == UINT64_MAX ? J.end_bits() : hint;
+        if (overflowed(R.status)) {
+            grow_segment(d, R.status, cells_used, mb_used);
+            ++c.prof.n_grown;
+        }
+        d.start_bit = R.start;
+        std::vector<uint64_t> cuts;
+        for (size_t t = t_first[i]; t < t_first[i + 1]; ++t)
+            if (found[t] != UINT64_MAX && found[t] > (cuts.empty() ? R.start : cuts.back()) && found[t] < end)
+                cuts.push_back(found[t]);
+        again.push_back(j);
+        if (cuts.empty()) continue;
+        d.hint_bit = cuts[0];
+        std::vector<uint32_t> ins;
+        for (size_t q = 0; q < cuts.size(); ++q) {
+            SegDesc p{};
+            p.start_bit = cuts[q];
+            p.hint_bit = q + 1 < cuts.size() ? cuts[q + 1] : hint;
+            p.end_byte = d.end_byte;
+            const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : end) - cuts[q]) / 8;
+            p.cells_base = cells_used;
+            p.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
+            cells_used += p.cells_cap;
+            p.flags = SEG_SPEC;
+            p.mb_base = (uint32_t)mb_used;
+            p.mb_cap = 4;
+            mb_used += 4;
+            ins.push_back((uint32_t)desc.size());
+            again.push_back((uint32_t)desc.size());
+            desc.push_back(p);
+        }
+        const size_t at = R.pos + 1;
+        J.order.insert(J.order.begin() + at, ins.begin(), ins.end());
+        J.seg_is_chunk.insert(J.seg_is_chunk.begin() + at, ins.size(), 0);
+        J.orig_start.insert(J.orig_start.begin() + at, cuts.begin(), cuts.end());
+        J.first_ok.insert(J.first_ok.begin() + at, ins.size(), 1);
+    }
+    return RRG_OK;
 }
 
 // Decode every job of the batch (inputs already in c.in) into `out`.
@@ -665,9 +760,10 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         }
         J.mp_hi = desc.size();
         J.orig_start = M;
+        J.order.resize(S);
+        for (size_t j = 0; j < S; ++j) J.order[j] = (uint32_t)(J.mp_lo + j);
     }
     const size_t S = desc.size();
-    c.prof.n_segments += S;
     // 4. one inflate launch over every segment of the batch
     std::vector<SegResult> res;
     std::vector<uint32_t> all(S);
@@ -676,32 +772,39 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     if (rc) return rc;
     for (Job& J : jobs) {
         // a capacity overflow is not a decode failure (the region is grown below)
-        J.first_ok.resize(J.mp_hi - J.mp_lo);
-        for (size_t j = J.mp_lo; j < J.mp_hi; ++j)
-            J.first_ok[j - J.mp_lo] = res[j].status == ST_OK || overflowed(res[j].status);
+        J.first_ok.resize(J.order.size());
+        for (size_t k = 0; k < J.order.size(); ++k)
+            J.first_ok[k] = res[J.order[k]].status == ST_OK || overflowed(res[J.order[k]].status);
     }
-    // 5. settle: chain walks and capacity retries, re-decodes batched per round
+    // 5. settle: chain walks and capacity retries, re-decodes batched per round.
+    // A member-path segment decoded again is split at block boundaries found
+    // inside it, so the extra round costs a fraction of a segment's decode.
     for (int round = 0;; ++round) {
-        std::vector<uint32_t> redo;
-        for (Job& J : jobs) {
+        std::vector<uint32_t> again;
+        std::vector<Redo> redo;
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            Job& J = jobs[ji];
             if (round < 6)
                 for (size_t j = J.bg_lo; j < J.bg_hi; ++j)
                     if (overflowed(res[j].status)) {
                         grow_segment(desc[j], res[j].status, cells_used, mb_used);
                         ++c.prof.n_grown;
-                        redo.push_back((uint32_t)j);
+                        again.push_back((uint32_t)j);
                     }
-            if (J.body != UINT64_MAX) c.prof.n_grown += chain_round(J, desc, res, cells_used, mb_used, redo);
+            if (J.body != UINT64_MAX) chain_round(J, ji, desc, res, redo);
         }
-        if (redo.empty()) break;
-        if (round > (int)S + 64) {
+        if (again.empty() && redo.empty()) break;
+        if (round > 4 * (int)desc.size() + 64) {
             set_err(err, RRG_E_DEVICE, "verification chain did not settle");
             return RRG_E_DEVICE;
         }
-        c.prof.n_redecoded += redo.size();
-        rc = run_inflate(c, G, desc, res, cells_used, mb_used, redo, err);
+        rc = split_redo(c, jobs, redo, desc, cells_used, mb_used, again, err);
+        if (rc) return rc;
+        c.prof.n_redecoded += again.size();
+        rc = run_inflate(c, G, desc, res, cells_used, mb_used, again, err);
         if (rc) return rc;
     }
+    c.prof.n_segments += desc.size();
     // 6. output layout (each file at a 256-byte aligned offset) and the chain
     std::vector<ChainSeg> cs;
     std::vector<uint64_t> cs_out;
@@ -718,7 +821,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             o += res[j].n_cells;
         }
         for (size_t k = 0; k < J.n_chain; ++k) {
-            const size_t j = J.mp_lo + k;
+            const size_t j = J.order[k];
             J.off.push_back(o);
             cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
                                   k == 0 ? 1u : 0u});
@@ -747,7 +850,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         for (size_t k = 0; k < nb; ++k) ranges.push_back({ob + J.off[k], ob + J.off[k + 1]});
         uint64_t mstart = J.off[nb];
         for (size_t k = 0; k < J.n_chain; ++k) {
-            const size_t j = J.mp_lo + k;
+            const size_t j = J.order[k];
             for (uint32_t q = 0; q < res[j].n_mb; ++q) {
                 const uint64_t e = J.off[nb + k] + recs[desc[j].mb_base + q].out_off;
                 ranges.push_back({ob + mstart, ob + e});
