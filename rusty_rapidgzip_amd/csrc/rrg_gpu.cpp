@@ -256,11 +256,9 @@ int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>
         // ~4 blocks per CU keeps phase 1 parallel while phase 2 stays short
         const uint32_t G = std::max<uint32_t>(1, (uint32_t)(n / std::max(1, 4 * c.n_cu)));
         const uint32_t nb = (n + G - 1) / G;
-        HIPCHK(c.tz.ensure((size_t)nb * 32768 * 2 * sizeof(uint16_t)));
-        uint16_t* exitb = c.tz.as<uint16_t>();
-        uint16_t* entry = exitb + (size_t)nb * 32768;
-        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), exitb, entry, c.err.as<uint32_t>(),
-                           c.s));
+        HIPCHK(c.tz.ensure(tail_scratch_bytes(n, G)));
+        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), c.tz.as<uint16_t>(),
+                           c.err.as<uint32_t>(), c.s));
         t.stop();
         c.prof.n_tail_rounds += nb;
     }
@@ -381,6 +379,11 @@ void plan_job(Job& J) {
     J.body = rest + (uint64_t)hl;
 }
 
+// Finder limit flag: any verified block start will do (segment targets).
+// Chunk targets keep the reference's candidate order (pipeline.rs:491-551,
+// block_finder.rs:109-141), since the chunk boundaries must be its own.
+constexpr uint64_t kAnyStart = 1ull << 63;
+
 // Targets of one member-path body: chunk targets (pipeline.rs:497-503) and
 // segment targets between them.
 void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& targets, std::vector<uint64_t>& limits,
@@ -408,7 +411,7 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
             const uint64_t t = k * chunk_bits + j * seg_bits;
             if (t >= total) break;
             targets.push_back(base + t);
-            limits.push_back(J.end_bits());
+            limits.push_back(J.end_bits() | kAnyStart);
             is_chunk.push_back(0);
         }
     }
@@ -578,7 +581,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         const uint64_t pieces = std::min<uint64_t>(kMaxPieces, span / kPieceBits);
         for (uint64_t p = 1; p < pieces; ++p) {
             targets.push_back(R.start + span * p / pieces);
-            limits.push_back(J.end_bits());
+            limits.push_back(J.end_bits() | kAnyStart);
         }
     }
     t_first[redo.size()] = targets.size();
@@ -596,7 +599,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         t.stop();
         HIPCHK(hipStreamSynchronize(c.s));
         for (size_t i = 0; i < found.size(); ++i)
-            if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= limits[i])) {
+            if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= (limits[i] & ~kAnyStart))) {
                 set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu",
                         (unsigned long long)found[i], (unsigned long long)targets[i]);
                 return RRG_E_DEVICE;
@@ -683,7 +686,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     }
     // a find is a bit offset inside the target's file; anything else is a device fault
     for (size_t i = 0; i < found.size(); ++i)
-        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= limits[i])) {
+        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= (limits[i] & ~kAnyStart))) {
             set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu", (unsigned long long)found[i],
                     (unsigned long long)targets[i]);
             return RRG_E_DEVICE;

@@ -303,8 +303,13 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     if (t >= n_targets) return;
     FindWave& fw = s_wave[threadIdx.x >> 6];
     uint64_t cur = uni64(targets[t]);
-    // search limit: the end of the target's file (several files may share the buffer)
-    const uint64_t limit = uni64(limits[t]);
+    // search limit: the end of the target's file (several files may share the
+    // buffer).  Bit 63 set: any verified block start will do -- the candidates
+    // are tried in bit order without the reference's NEXT_LUT skip chain,
+    // which is only needed where the exact candidate it picks matters.
+    const uint64_t lim_word = uni64(limits[t]);
+    const bool any_start = (lim_word >> 63) != 0;
+    const uint64_t limit = lim_word & ~(1ull << 63);
     const uint64_t nbits = limit;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
@@ -333,6 +338,21 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
             if (step == 0) cand = precode_ok(lo, hi, p, nbits);
         }
         const uint64_t cm = __ballot(cand);
+        if (any_start) {
+            // candidates of this window in order; cand already implies p + 13 <= limit
+            uint64_t m = cm;
+            while (m) {
+                const uint32_t rel = (uint32_t)__builtin_ctzll(m);
+                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
+                    result = base + rel;
+                    end = true;
+                    break;
+                }
+                m &= m - 1;
+            }
+            cur = base + 64;
+            continue;
+        }
         const int nx = (step == 0) ? 1 : (int)step;
         // chain walk in window-relative 32-bit offsets: scalar compares only
         const uint64_t room = limit - 13 - base;  // base + 13 <= limit here
@@ -896,8 +916,11 @@ hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResu
 //      bytes before the block).  The window starts as the identity (cell i =
 //      marker for entry byte i); each segment's tail cells are rewritten in
 //      place to symbolic form and the final window is the block's exit tail.
-//   2. k_tail_entry (one workgroup): entry window of block B+1 = exit tail of
-//      B resolved against entry window of B -- one 32 Ki step per block.
+//   2. entry windows: entry of block B+1 = exit tail of B resolved against
+//      the entry of B.  The composition is associative, so it runs as a
+//      three-level scan: k_tail_compose folds groups of ~sqrt(nb) blocks in
+//      parallel, one k_tail_entry workgroup walks the group windows, and
+//      k_tail_entry expands every group in parallel.
 //   3. k_tail_patch (parallel): every symbolic marker in a tail zone becomes
 //      the literal byte of its block's entry window.
 //
@@ -969,13 +992,48 @@ __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict_
     for (uint32_t i = tid; i < WIN; i += 1024) out[i] = win[i];
 }
 
+// Composition of symbolic windows: out[i] = X[i] when it is a literal (or
+// 0x100), else Y's cell for the byte it names.  Both windows describe the 32 Ki
+// bytes before some point; Y is relative to the point where X's markers refer.
+// k_tail_compose: each workgroup folds the exit windows of K consecutive
+// blocks into one window relative to the entry of the first of them.
+__global__ __launch_bounds__(1024) void k_tail_compose(const uint16_t* __restrict__ exit_tail, uint32_t n_blocks,
+                                                       uint32_t K, uint16_t* __restrict__ group_exit) {
+    __shared__ uint16_t acc[WIN];
+    const uint32_t tid = threadIdx.x, g = blockIdx.x;
+    const uint32_t first = g * K, end = min(n_blocks, first + K);
+    for (uint32_t i = tid; i < WIN; i += 1024) acc[i] = (uint16_t)(0x8000u | (WIN - 1 - i));  // identity
+    __syncthreads();
+    for (uint32_t B = first; B < end; ++B) {
+        uint16_t v[WIN / 1024];
+        const uint16_t* ex = exit_tail + (size_t)B * WIN;
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) {
+            uint16_t c = ex[tid + 1024u * r];
+            if (c & 0x8000u) c = acc[WIN - 1 - (c & 0x7fffu)];
+            v[r] = c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) acc[tid + 1024u * r] = v[r];
+        __syncthreads();
+    }
+    uint16_t* out = group_exit + (size_t)g * WIN;
+    for (uint32_t i = tid; i < WIN; i += 1024) out[i] = acc[i];
+}
+
+// k_tail_entry: workgroup g walks blocks [gK, gK + K) in order, starting from
+// init[g] (or "before the output start" when init is null): the entry window
+// of each block, then the next entry = that block's exit resolved against it.
 __global__ __launch_bounds__(1024) void k_tail_entry(const uint16_t* __restrict__ exit_tail, uint32_t n_blocks,
+                                                     uint32_t K, const uint16_t* __restrict__ init,
                                                      uint16_t* __restrict__ entry) {
     __shared__ uint16_t ent[WIN];
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < WIN; i += 1024) ent[i] = 0x100;  // before the output start
+    const uint32_t tid = threadIdx.x, g = blockIdx.x;
+    const uint32_t first = g * K, end = min(n_blocks, first + K);
+    for (uint32_t i = tid; i < WIN; i += 1024) ent[i] = init ? init[(size_t)g * WIN + i] : (uint16_t)0x100;
     __syncthreads();
-    for (uint32_t B = 0; B < n_blocks; ++B) {
+    for (uint32_t B = first; B < end; ++B) {
         uint16_t v[WIN / 1024];
         const uint16_t* ex = exit_tail + (size_t)B * WIN;
         uint16_t* en = entry + (size_t)B * WIN;
@@ -1014,12 +1072,34 @@ __global__ __launch_bounds__(256) void k_tail_patch(const ChainSeg* __restrict__
     }
 }
 
-hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_exit,
-                       uint16_t* d_entry, uint32_t* d_err, hipStream_t s) {
+// Blocks per group of the entry scan (about the square root of the block count).
+uint32_t tail_group(uint32_t nb) {
+    uint32_t K = 1;
+    while (K * K < nb) ++K;
+    return K;
+}
+
+size_t tail_scratch_bytes(uint32_t n_seg, uint32_t G) {
+    const uint32_t nb = (n_seg + G - 1) / G;
+    const uint32_t K = tail_group(nb), ng = (nb + K - 1) / K;
+    return ((size_t)2 * nb + 2 * (size_t)ng) * WIN * sizeof(uint16_t);
+}
+
+hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_scratch,
+                       uint32_t* d_err, hipStream_t s) {
     if (n_seg == 0) return hipSuccess;
     const uint32_t nb = (n_seg + G - 1) / G;
+    const uint32_t K = tail_group(nb), ng = (nb + K - 1) / K;
+    uint16_t* d_exit = d_scratch;
+    uint16_t* d_entry = d_exit + (size_t)nb * WIN;
+    uint16_t* d_gexit = d_entry + (size_t)nb * WIN;
+    uint16_t* d_gentry = d_gexit + (size_t)ng * WIN;
     hipLaunchKernelGGL(k_tail_block, dim3(nb), dim3(1024), 0, s, d_cs, n_seg, G, d_cells, d_exit);
-    hipLaunchKernelGGL(k_tail_entry, dim3(1), dim3(1024), 0, s, d_exit, nb, d_entry);
+    // entry windows: groups of K blocks folded in parallel, a short serial pass
+    // over the ng group windows, then every group expanded in parallel
+    hipLaunchKernelGGL(k_tail_compose, dim3(ng), dim3(1024), 0, s, d_exit, nb, K, d_gexit);
+    hipLaunchKernelGGL(k_tail_entry, dim3(1), dim3(1024), 0, s, d_gexit, ng, ng, (const uint16_t*)nullptr, d_gentry);
+    hipLaunchKernelGGL(k_tail_entry, dim3(ng), dim3(1024), 0, s, d_exit, nb, K, (const uint16_t*)d_gentry, d_entry);
     hipLaunchKernelGGL(k_tail_patch, dim3(n_seg), dim3(256), 0, s, d_cs, n_seg, G, d_entry, d_cells, d_err);
     return hipGetLastError();
 }

@@ -8,8 +8,11 @@
 namespace rrg {
 
 // Block finder: found[i] = first verified dynamic-block offset in
-// [targets[i], limits[i]) (UINT64_MAX when none).  One wavefront per target.
-// total_bits: size of the whole (padded) input buffer.
+// [targets[i], limit) (UINT64_MAX when none), limit = limits[i] without bit 63.
+// Without bit 63 the candidates are visited in the reference's NEXT_LUT skip
+// order (its exact choice); with bit 63 every candidate is tried in bit order
+// (some verified block start).  One wavefront per target.  total_bits: size
+// of the whole (padded) input buffer.
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
                               const uint64_t* d_limits, uint64_t* d_found, uint32_t n_targets, hipStream_t s);
 
@@ -25,12 +28,13 @@ hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResu
                           const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
                           uint16_t* d_sorted, const InflateGeom& g, hipStream_t s);
 
-// Tail chain: blocked three-phase scan over blocks of G chain segments.
-// d_exit / d_entry hold ceil(n_seg/G) * 32768 u16 each; d_err[s] is set to 1
-// for a chain segment s whose marker reaches before the start of its file's
-// output (d_err holds n_seg words, cleared by the caller).
-hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_exit,
-                       uint16_t* d_entry, uint32_t* d_err, hipStream_t s);
+// Tail chain: blocked scan over blocks of G chain segments.  d_scratch holds
+// tail_scratch_bytes(n_seg, G); d_err[s] is set to 1 for a chain segment s
+// whose marker reaches before the start of its file's output (d_err holds
+// n_seg words, cleared by the caller).
+size_t tail_scratch_bytes(uint32_t n_seg, uint32_t G);
+hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_t* d_cells, uint16_t* d_scratch,
+                       uint32_t* d_err, hipStream_t s);
 
 // Resolve markers and narrow u16 cells to u8 output (d_err as for launch_tail).
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs,
