@@ -22,12 +22,18 @@ namespace rrg {
 // in bounds.  Bits at or beyond nbits read as zero; every consumer checks
 // availability against nbits before acting on what it read.
 // ---------------------------------------------------------------------------
+// 64 bits of the 128-bit value hi:lo starting at bit s (0..63), branch-free:
+// both words are always used, so neither load is left pending on any path
+__device__ __forceinline__ uint64_t funnel_lo(uint64_t lo, uint64_t hi, uint32_t s) {
+    return (lo >> s) | ((hi << 1) << (63 - s));
+}
+
 __device__ __forceinline__ uint64_t bits64_at(const uint64_t* __restrict__ in64, uint64_t pos) {
     const uint64_t q = pos >> 6;
     const uint32_t s = (uint32_t)(pos & 63);
     const uint64_t lo = in64[q];
     const uint64_t hi = in64[q + 1];
-    return s ? (lo >> s) | (hi << (64 - s)) : lo;
+    return funnel_lo(lo, hi, s);
 }
 
 __device__ __forceinline__ uint32_t byte_at(const uint64_t* __restrict__ in64, uint64_t i) {
@@ -466,7 +472,11 @@ __device__ bool decode_long(uint64_t bits, const uint16_t* __restrict__ cnt, con
         code |= (int)((bits >> (l - 1)) & 1u);
         const int count = cnt[l];
         if (code - count < first) {
-            *sym = sorted[index + (code - first)];
+            uint32_t v = sorted[index + (code - first)];
+            // use the loaded value here: the wait for this load then stays on the
+            // slow path instead of becoming a full vmcnt wait on every symbol
+            asm volatile("" : "+v"(v));
+            *sym = v;
             *len = (uint32_t)l;
             return true;
         }
@@ -537,33 +547,18 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     uint64_t pos = 0, hint = 0, bitbuf = 0;
     uint64_t nbits = 0, in_len = 0;  // end of the segment's file in the input buffer
     uint32_t bitcnt = 0;
-    // input words cq, cq+1, cq+2 held in registers; moving to the next word
-    // loads the one after, so it arrives while symbols are being decoded
-    constexpr uint64_t kNoWord = ~0ull >> 1;
-    uint64_t cq = kNoWord, w0 = 0, w1 = 0, w2 = 0;
     uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
     uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
     uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
 
-// 64 bits at pos (the input buffer is padded, so words up to 8 past the end exist)
-#define RRG_REFILL()                                            \
-    do {                                                        \
-        const uint64_t _q = pos >> 6;                           \
-        if (_q != cq) {                                         \
-            if (_q == cq + 1) {                                 \
-                w0 = w1;                                        \
-                w1 = w2;                                        \
-                w2 = in64[_q + 2];                              \
-            } else {                                            \
-                w0 = in64[_q];                                  \
-                w1 = in64[_q + 1];                              \
-                w2 = in64[_q + 2];                              \
-            }                                                   \
-            cq = _q;                                            \
-        }                                                       \
-        const uint32_t _s = (uint32_t)(pos & 63);               \
-        bitbuf = _s ? (w0 >> _s) | (w1 << (64 - _s)) : w0;     \
-        bitcnt = 64;                                            \
+// 64 bits at pos (the input buffer is padded, so words up to 8 past the end
+// exist).  The asm use completes the loads inside the refill, so that code
+// merging with a refill-free path does not inherit a pending-load wait.
+#define RRG_REFILL()                      \
+    do {                                  \
+        bitbuf = bits64_at(in64, pos);    \
+        asm volatile("" : "+v"(bitbuf));  \
+        bitcnt = 64;                      \
     } while (0)
 #define RRG_CONSUME(n)           \
     do {                         \
@@ -599,7 +594,6 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 const SegDesc d = segs[seg];
                 pos = d.start_bit;
                 hint = d.hint_bit;
-                cq = kNoWord;
                 in_len = d.end_byte;
                 nbits = d.end_byte * 8;
                 out_base = d.cells_base;
@@ -627,6 +621,12 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
 
         if (state == S_SYM) {
             // ---- Huffman symbols (fast_inflate.rs:2065-2229) ----
+            // Global stores and loads share one in-order counter (vmcnt), so a
+            // wait for any load also waits for every earlier store.  The loop
+            // is arranged so that a match waits once: the bit-buffer refill
+            // loads are issued together with the copy's source loads, and both
+            // complete before the copy's stores are issued.
+            uint16_t* const o = cells + out_base;
             for (int it = 0; it < 64; ++it) {
                 // 48 bits cover a whole match: litlen <= 15 + 5 extra, dist <= 15 + 13 extra
                 if (bitcnt < 48) RRG_REFILL();
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_CONSUME(l);
                 if (sym < 256) {
                     if (out_n >= out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
-                    cells[out_base + out_n] = (uint16_t)sym;
+                    o[out_n] = (uint16_t)sym;
                     ++out_n;
                     continue;
                 }
@@ -667,8 +667,9 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 if (pos + dx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                 const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));
                 RRG_CONSUME(dx);
-                if (out_n + length > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
-                uint16_t* o = cells + out_base;
+                // copies store whole groups of 8 cells (up to 7 past the match end,
+                // overwritten by what follows): keep them inside the region
+                if (out_n + length + 8 > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
                 const uint64_t cur = out_n - member_base;
                 if ((uint64_t)distance > cur) {
                     if (!spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
@@ -678,28 +679,37 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     out_n += pre;
                     length -= pre;
                 }
+                // refill loads now, consumed after the first group's source loads
+                const uint64_t rq = pos >> 6;
+                const uint64_t r0 = in64[rq], r1 = in64[rq + 1];
                 if (length) {
+                    // o[out_n + t] = o[src + (t mod distance)]: every load reads a
+                    // cell written before its group (all of them for distance < 8;
+                    // earlier groups for distance >= 8)
                     const uint64_t src = out_n - distance;
-                    if (distance == 1) {
-                        const uint16_t v = o[src];
-                        for (uint32_t k = 0; k < length; ++k) o[out_n + k] = v;
-                    } else {
-                        // o[dst+k] = o[src + (k mod distance)]: every load reads a
-                        // cell written before this copy started.
-                        uint32_t jm = 0;
-                        for (uint32_t k = 0; k < length; k += 8) {
-                            uint16_t v[8];
+                    uint32_t jm = 0;
+                    for (uint32_t k = 0; k < length; k += 8) {
+                        uint16_t v[8];
 #pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                v[i] = (k + i < length) ? o[src + jm] : (uint16_t)0;
-                                jm = (jm + 1 == distance) ? 0u : jm + 1;
-                            }
-#pragma unroll
-                            for (int i = 0; i < 8; ++i)
-                                if (k + i < length) o[out_n + k + i] = v[i];
+                        for (int i = 0; i < 8; ++i) {
+                            v[i] = o[src + jm];
+                            jm = (jm + 1 == distance) ? 0u : jm + 1;
                         }
+                        if (k == 0) {
+                            bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
+                            bitcnt = 64;
+                            // complete the refill here, before the stores (and so
+                            // not as a full wait at the top of the next symbol)
+                            asm volatile("" : "+v"(bitbuf));
+                        }
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) o[out_n + k + i] = v[i];
                     }
                     out_n += length;
+                } else {
+                    bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
+                    bitcnt = 64;
+                    asm volatile("" : "+v"(bitbuf));
                 }
             }
         } else if (state == S_BLOCK_START) {
@@ -1105,62 +1115,81 @@ hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_
 }
 
 // ===========================================================================
-// Resolve + narrow.  Each thread owns one aligned 8-byte output word.  The 8
-// cells behind it start at a segment-constant cell phase, fetched with two
-// aligned 16-byte loads; marker cells read the (already patched, literal) cell
-// they copy from the preceding tail zone.  Words that straddle a tile edge are
-// written byte by byte.
+// Resolve + narrow.  Each thread owns 16-byte output chunks (16-byte aligned
+// in output coordinates).  The 16 cells behind a chunk start at a phase that
+// is constant over the tile (and so over the wave): three aligned 16-byte
+// cell loads and a wave-uniform funnel give them.  Without markers the low
+// bytes are packed with v_perm; marker cells read the (already patched,
+// literal) cell they copy from the preceding tail zone.  Chunks that straddle
+// a tile edge are written byte by byte.
 // ===========================================================================
-__device__ __forceinline__ uint64_t funnel64(uint64_t lo, uint64_t hi, uint32_t sh) {
-    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+__device__ __forceinline__ uint32_t resolve_cell(const ChainSeg* __restrict__ cs, const uint16_t* __restrict__ cells,
+                                                 uint32_t seg, uint32_t cell, uint32_t* __restrict__ err) {
+    if (cell & 0x8000u) {
+        uint32_t s;
+        uint64_t local;
+        if (marker_target(cs, seg, cell & 0x7fffu, &s, &local)) return cells[cs[s].cells_base + local] & 0xffu;
+        err[seg] = 1u;
+        return 0u;
+    }
+    return cell & 0xffu;
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__ tiles,
                                                  const ChainSeg* __restrict__ cs, const uint16_t* __restrict__ cells,
                                                  uint8_t* __restrict__ out, uint32_t* __restrict__ err) {
     const ResolveTile t = tiles[blockIdx.x];
-    const ChainSeg me = cs[t.seg];
     const uint64_t ob = t.out_off, oe = t.out_off + t.n;
-    // cell index of output byte p is t.cell_addr + (p - ob)
-    const uint64_t w0 = ob >> 3, w1 = (oe + 7) >> 3;
-    const uint64_t* c64 = reinterpret_cast<const uint64_t*>(cells);
-    for (uint64_t w = w0 + threadIdx.x; w < w1; w += 256) {
-        const uint64_t p0 = w << 3;
-        // cell index for byte p0 (may be before the tile; only in-tile bytes are kept)
-        const int64_t cidx = (int64_t)t.cell_addr + ((int64_t)p0 - (int64_t)ob);
-        const int64_t a = cidx & ~3ll;  // 4-cell (8-byte) aligned
-        const uint32_t sh = (uint32_t)(cidx - a) * 16u;
-        const int64_t q = a >> 2;
-        const uint64_t q0 = c64[q < 0 ? 0 : q], q1 = c64[q + 1], q2 = c64[q + 2];
-        const uint64_t lo = funnel64(q0, q1, sh), hi = funnel64(q1, q2, sh);
-        uint64_t word = 0;
-        uint32_t keep = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const uint64_t p = p0 + (uint64_t)i;
-            if (p < ob || p >= oe) continue;
-            keep |= 1u << i;
-            uint32_t cell = (uint32_t)(((i < 4 ? lo : hi) >> (16 * (i & 3))) & 0xffffu);
-            if (cell & 0x8000u) {
-                uint32_t s;
-                uint64_t local;
-                if (marker_target(cs, t.seg, cell & 0x7fffu, &s, &local)) {
-                    cell = cells[cs[s].cells_base + local];
-                } else {
-                    err[t.seg] = 1u;
-                    cell = 0;
-                }
-            }
-            word |= (uint64_t)(cell & 0xffu) << (8 * i);
+    // cell index of output byte p is delta + p
+    const int64_t delta = (int64_t)t.cell_addr - (int64_t)ob;
+    const uint32_t phase = (uint32_t)((uint64_t)delta & 7u);  // cells, within a 16-byte cell group
+    const uint32_t dsh = phase >> 1, odd = phase & 1u;
+    const uint4* c128 = reinterpret_cast<const uint4*>(cells);
+    for (uint64_t p0 = (ob & ~15ull) + 16ull * threadIdx.x; p0 < oe; p0 += 16ull * 256) {
+        const int64_t q = (delta + (int64_t)p0 - (int64_t)phase) >> 3;  // first 16-byte cell group
+        // a group before the buffer start only holds cells of bytes before the tile
+        const uint4 g0 = c128[q < 0 ? 0 : q], g1 = c128[q + 1 < 0 ? 0 : q + 1], g2 = c128[q + 2 < 0 ? 0 : q + 2];
+        const uint32_t w[12] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w, g2.x, g2.y, g2.z, g2.w};
+        uint32_t o[8];
+        // wave-uniform selection of the funnel (phase is constant over the tile)
+#define RRG_FUNNEL(D)                                                                            \
+    _Pragma("unroll") for (int j = 0; j < 8; ++j) o[j] = odd ? ((w[j + (D)] >> 16) | (w[j + (D) + 1] << 16)) \
+                                                             : w[j + (D)];
+        switch (dsh) {
+            case 0: RRG_FUNNEL(0) break;
+            case 1: RRG_FUNNEL(1) break;
+            case 2: RRG_FUNNEL(2) break;
+            default: RRG_FUNNEL(3) break;
         }
-        if (keep == 0xffu) {
-            reinterpret_cast<uint64_t*>(out)[w] = word;
+#undef RRG_FUNNEL
+        uint32_t b[4];
+        const uint32_t any = (o[0] | o[1] | o[2] | o[3] | o[4] | o[5] | o[6] | o[7]) & 0x80008000u;
+        if (!any) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) b[m] = __builtin_amdgcn_perm(o[2 * m + 1], o[2 * m], 0x06040200u);
         } else {
-            for (int i = 0; i < 8; ++i)
-                if (keep & (1u << i)) out[p0 + i] = (uint8_t)(word >> (8 * i));
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const uint32_t cell = (o[2 * m + (h >> 1)] >> (16 * (h & 1))) & 0xffffu;
+                    const uint64_t p = p0 + (uint64_t)(4 * m + h);
+                    const uint32_t byte = (p >= ob && p < oe) ? resolve_cell(cs, cells, t.seg, cell, err) : 0u;
+                    v |= byte << (8 * h);
+                }
+                b[m] = v;
+            }
+        }
+        if (p0 >= ob && p0 + 16 <= oe) {
+            *reinterpret_cast<uint4*>(out + p0) = make_uint4(b[0], b[1], b[2], b[3]);
+        } else {
+            for (int i = 0; i < 16; ++i) {
+                const uint64_t p = p0 + (uint64_t)i;
+                if (p >= ob && p < oe) out[p] = (uint8_t)(b[i >> 2] >> (8 * (i & 3)));
+            }
         }
     }
-    (void)me;
 }
 
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs, const uint16_t* d_cells,
