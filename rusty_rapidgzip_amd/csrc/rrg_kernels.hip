@@ -1201,14 +1201,17 @@ hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const Ch
 
 // ===========================================================================
 // CRC-32 of pieces (raw: register starts at 0, no final inversion; the host
-// folds pieces into member CRCs with GF(2) shifts).  256 threads per piece;
-// thread t owns the 1 KiB sub-block that ends 1 KiB*(255-t) before the piece
-// end, so every sub-block is "full" (leading bytes before the piece start
-// are skipped, which is the same as zeros for a register that starts at 0).
-// Slice-by-8 tables in LDS; the 256 partial CRCs are folded in a tree with
-// multipliers x^(8*1024*2^k) mod P.
+// folds pieces into member CRCs with GF(2) shifts).  256 threads per piece.
+// The piece is read as aligned 8-byte words (bytes before the piece masked to
+// zero, which a zero register ignores) and the words are dealt round-robin:
+// thread t takes every 256th word, so a wave reads 512 contiguous bytes per
+// step.  Thread t's words are 2 KiB apart: acc <- acc * x^(8*2048) + crc(word),
+// the multiplication by four 256-entry tables; crc(word) by slice-by-8 tables.
+// The 256 partial values are folded in a tree with x^(64*2^k); the last
+// (unaligned) bytes of the piece follow byte by byte.
 // ===========================================================================
-__constant__ uint32_t c_crc_shift[8];  // x^(8*1024*2^k) mod P, reflected
+__constant__ uint32_t c_crc_shift[8];  // x^(64*2^k) mod P, reflected (2^k words of 8 bytes)
+__constant__ uint32_t c_crc_step;      // x^(8*2048) mod P (256 words)
 
 __device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
     // a*b mod P over GF(2), reflected bit order (bit 31 = x^0)
@@ -1224,13 +1227,17 @@ __device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
 __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ out,
                                                     const uint64_t* __restrict__ piece_off,
                                                     const uint32_t* __restrict__ piece_len, uint32_t* __restrict__ crc) {
-    __shared__ uint32_t T[8][256];
+    __shared__ uint32_t T[8][256];  // slice-by-8
+    __shared__ uint32_t M[4][256];  // v -> v * x^(8*2048), by bytes of v
     __shared__ uint32_t part[256];
     const uint32_t tid = threadIdx.x;
     {
         uint32_t c = tid;
         for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
         T[0][tid] = c;
+        const uint32_t X = c_crc_step;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) M[j][tid] = gf2_mul(X, tid << (8 * j));
     }
     __syncthreads();
     for (int s = 1; s < 8; ++s) {
@@ -1238,36 +1245,38 @@ __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ 
         T[s][tid] = (prev >> 8) ^ T[0][prev & 0xffu];
         __syncthreads();
     }
-    const uint64_t off = piece_off[blockIdx.x];
-    const uint64_t end = off + piece_len[blockIdx.x];
-    // sub-block [sb, sb+1024): sb = end - 1024*(256 - tid), clipped at off
-    // (signed: for a short piece most sub-blocks lie wholly before `off`)
-    const int64_t sb = (int64_t)end - 1024ll * (int64_t)(256 - tid);
-    const int64_t es = sb + 1024;
-    uint64_t b = sb < (int64_t)off ? off : (uint64_t)sb;
-    const uint64_t e = es < (int64_t)b ? b : (uint64_t)es;
-    uint32_t c = 0;
-    // head bytes up to 8-byte alignment
-    while (b < e && (b & 7)) {
-        c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
-        ++b;
+    const uint64_t start = piece_off[blockIdx.x];
+    const uint64_t end = start + piece_len[blockIdx.x];
+    const uint64_t a0 = start & ~7ull, E = end & ~7ull;
+    const uint64_t N = E > start ? (E - a0) >> 3 : 0;  // aligned words
+    const uint64_t tail = E > start ? E : start;       // bytes [tail, end) follow one by one
+    const uint64_t* w64 = reinterpret_cast<const uint64_t*>(out) + (a0 >> 3);
+    uint32_t acc = 0;
+    const uint64_t r0 = 255 - tid;  // this thread's words: j = N - 1 - (r0 + 256 m)
+    if (N > r0) {
+        for (int64_t m = (int64_t)((N - 1 - r0) >> 8); m >= 0; --m) {
+            const uint64_t j = N - 1 - (r0 + 256ull * (uint64_t)m);
+            uint64_t v = w64[j];
+            if (j == 0) v &= ~0ull << (8 * (start & 7));
+            const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+            const uint32_t cw = T[7][lo & 0xffu] ^ T[6][(lo >> 8) & 0xffu] ^ T[5][(lo >> 16) & 0xffu] ^
+                                T[4][lo >> 24] ^ T[3][hi & 0xffu] ^ T[2][(hi >> 8) & 0xffu] ^
+                                T[1][(hi >> 16) & 0xffu] ^ T[0][hi >> 24];
+            acc = M[0][acc & 0xffu] ^ M[1][(acc >> 8) & 0xffu] ^ M[2][(acc >> 16) & 0xffu] ^ M[3][acc >> 24] ^ cw;
+        }
     }
-    const uint64_t* w64 = reinterpret_cast<const uint64_t*>(out);
-    for (; b + 8 <= e; b += 8) {
-        const uint64_t v = w64[b >> 3];
-        const uint32_t lo = (uint32_t)v ^ c, hi = (uint32_t)(v >> 32);
-        c = T[7][lo & 0xffu] ^ T[6][(lo >> 8) & 0xffu] ^ T[5][(lo >> 16) & 0xffu] ^ T[4][lo >> 24] ^
-            T[3][hi & 0xffu] ^ T[2][(hi >> 8) & 0xffu] ^ T[1][(hi >> 16) & 0xffu] ^ T[0][hi >> 24];
-    }
-    for (; b < e; ++b) c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
-    part[tid] = c;
+    part[tid] = acc;
     __syncthreads();
     for (int k = 0; k < 8; ++k) {
         const uint32_t span = 1u << k;
         if ((tid & ((span << 1) - 1)) == 0) part[tid] = gf2_mul(c_crc_shift[k], part[tid]) ^ part[tid + span];
         __syncthreads();
     }
-    if (tid == 0) crc[blockIdx.x] = part[0];
+    if (tid == 0) {
+        uint32_t c = part[0];
+        for (uint64_t b = tail; b < end; ++b) c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
+        crc[blockIdx.x] = c;
+    }
 }
 
 hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
@@ -1275,8 +1284,6 @@ hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, 
     if (n_pieces == 0) return hipSuccess;
     static bool init = false;
     if (!init) {
-        // x^(8*1024*2^k) mod P: square x^8 ten times to reach x^(8*1024), then keep squaring
-        uint32_t x = 0x40000000u;  // x^1
         auto mul = [](uint32_t a, uint32_t b) {
             uint32_t prod = 0;
             for (int i = 0; i < 32; ++i) {
@@ -1285,13 +1292,16 @@ hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, 
             }
             return prod;
         };
-        for (int i = 0; i < 13; ++i) x = mul(x, x);  // x^(2^13) = x^(8*1024)
+        uint32_t x = 0x40000000u;                   // x^1
+        for (int i = 0; i < 6; ++i) x = mul(x, x);  // x^64: one 8-byte word
         uint32_t k8[8];
         for (int k = 0; k < 8; ++k) {
             k8[k] = x;
             x = mul(x, x);
         }
+        const uint32_t step = x;  // x^(64*256) = x^(8*2048)
         hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_shift), k8, sizeof k8);
+        if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_step), &step, sizeof step);
         if (e != hipSuccess) return e;
         init = true;
     }
