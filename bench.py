@@ -1,21 +1,30 @@
 """Benchmark: decompressed GB/s of the MI355X gzip decoder on one GPU.
 
-Workload (the headline configuration): 4 GiB of synthetic FASTQ-like text
-(4-line records '@r<id>', 150 bp ACGT with ~0.1% N, '+', 150-char Phred walk),
-gzip level 6, one member, read_gz chunk size 4 MiB.  One step = one complete
-device-resident decode of the file: block finder, speculative inflate, chain
-verification, tail chain, marker resolution, CRC32 + ISIZE checks, output left
-in HBM with the reference's chunk boundaries.  The compressed input is already
-resident in HBM when the timed region starts.
+Default workload (the headline configuration): 4 GiB of synthetic FASTQ-like
+text (4-line records '@r<id>', 150 bp ACGT with ~0.1% N, '+', 150-char Phred
+walk), gzip level 6, one member, read_gz chunk size 4 MiB.  One step = one
+complete device-resident decode of the file: block finder, speculative
+inflate, chain verification, tail chain, marker resolution, CRC32 + ISIZE
+checks, output left in HBM with the reference's chunk boundaries.  The
+compressed input is already resident in HBM when the timed region starts.
 
 Prints ONE JSON line.  `cpu_baseline` times the CPU restatement of read_gz
 (oracle/, "port") on this host's cores over a bounded sample of the same
-workload.
+workload.  `roofline.traffic` is the HBM traffic per inflate launch measured
+with rocprofv3 counters (profiles/traffic_fastq4g.json), when present.
+
+Other workloads (--workload), each one JSON line of its own:
+  bgzf    the same 4 GiB as BGZF (65280-byte members, level 6), one batch
+  batch   64 FASTQ files of 64 MiB, one batched call vs one call per file
+  serve   request mix through the serving loop (sizes log-uniform, text /
+          FASTQ / mixed at levels 1/6/9), closed-loop capacity, then Poisson
+          arrivals at fractions of it: throughput and p50/p99 latency
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -24,36 +33,64 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fastq4g.json")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
-    ap.add_argument("--chunk-mib", type=int, default=4)
-    ap.add_argument("--segment-kib", type=int, default=0, help="0 = automatic")
-    ap.add_argument("--lut-bits", type=int, default=9)
-    ap.add_argument("--lanes", type=int, default=16)
-    ap.add_argument("--cpu-sample-mib", type=int, default=512)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1"}))
-        log("only --gpus 1 is supported: the decoder runs on a single MI355X")
-        return 2
+def _pct(v, q):
+    import numpy as np
 
+    return round(float(np.percentile(v, q)), 3) if len(v) else None
+
+
+def _base(metric, value, unit, steps, warmup, ms_per_step, dtype="u8"):
+    return {
+        "metric": metric,
+        "value": round(value, 3),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+    }
+
+
+def make_cfg(args):
+    import rusty_rapidgzip_amd as rrg
+
+    return rrg.Config(chunk_size_bytes=args.chunk_mib << 20, segment_bytes=args.segment_kib << 10,
+                      lut_bits=args.lut_bits, lanes_per_block=args.lanes)
+
+
+def timed_steps(step, steps):
+    from rusty_rapidgzip_amd import hipmem
+
+    lat = []
+    hipmem.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        step()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    hipmem.synchronize()
+    return time.perf_counter() - t_start, lat
+
+
+# ---------------------------------------------------------------------------
+def run_fastq(args, threads):
     import numpy as np
 
     import rusty_rapidgzip_amd as rrg
     from rusty_rapidgzip_amd import hipmem, synth
 
-    threads = min(16, os.cpu_count() or 1)
     n = int(args.gib * (1 << 30))
     t0 = time.time()
     raw = synth.fastq(n, threads=threads)
@@ -64,92 +101,77 @@ def main() -> int:
     log(f"gzip -6: {C} bytes ({n / C:.2f}:1) in {time.time() - t0:.1f}s")
     d_in = hipmem.DeviceBuffer.from_host(gz)
     hipmem.synchronize()
-    cfg = rrg.Config(chunk_size_bytes=args.chunk_mib << 20, segment_bytes=args.segment_kib << 10,
-                     lut_bits=args.lut_bits, lanes_per_block=args.lanes)
+    cfg = make_cfg(args)
     dec = rrg.Decoder(0)
-    host = gz  # numpy array, passed by pointer
 
     def step():
-        return dec.decode_device(d_in.ptr, C, host, cfg)
+        return dec.decode_device(d_in.ptr, C, gz, cfg)
 
-    # warmup (first call also checks the output bytes against the generator)
     for w in range(max(1, args.warmup)):
         ptr, out_len, ends, st = step()
         if w == 0:
             assert out_len == n, (out_len, n)
-            check = hipmem.to_host(ptr, n)
-            if not np.array_equal(check, raw):
+            if not np.array_equal(hipmem.to_host(ptr, n), raw):
                 raise RuntimeError("GPU output differs from the generated bytes")
-            del check
             log(f"warmup decode verified: {out_len} bytes, {len(ends)} chunks, "
                 f"{st.speculation_failures} speculation failures")
-        p = dec.profile()
-        log("warmup", w, {k: round(v, 3) if isinstance(v, float) else v for k, v in p.as_dict().items()})
-
-    lat = []
+        log("warmup", w, {k: round(v, 3) if isinstance(v, float) else v for k, v in dec.profile().as_dict().items()})
     prof = []
-    hipmem.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
+
+    def step_p():
         step()
-        lat.append((time.perf_counter() - t1) * 1e3)
         prof.append(dec.profile().as_dict())
-    hipmem.synchronize()
-    elapsed = time.perf_counter() - t_start
+
+    elapsed, lat = timed_steps(step_p, args.steps)
     value = n * args.steps / elapsed / 1e9
-    ms_per_step = elapsed * 1e3 / args.steps
 
     def avg(key):
         return sum(p[key] for p in prof) / len(prof)
 
     ms_inflate = avg("ms_inflate")
-    launches = avg("n_inflate_launches")
-    segs = avg("n_segments")
+    launches = max(1.0, avg("n_inflate_launches"))
     # dominant kernel: speculative inflate.  Algorithmic bytes per launch: the
     # compressed stream read once plus the u16 cells written (C + 2U).
-    infl_bytes = C + 2 * n
-    infl_ms_launch = ms_inflate / max(1.0, launches)
-    achieved = infl_bytes / max(1, launches) / (infl_ms_launch / 1e3) / 1e9
-
-    result = {
-        "metric": "decompressed GB/s on 1 MI355X (device-resident read_gz, CRC-checked)",
-        "value": round(value, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
-        "config": {
-            "workload": f"fastq-like {args.gib:g} GiB, gzip -6 single member, 4-line records, 150 bp",
-            "chunk_size_bytes": args.chunk_mib << 20,
-            "compressed_bytes": C,
-            "uncompressed_bytes": n,
-            "segments": int(segs),
-            "lut_bits": args.lut_bits,
-            "lanes_per_block": args.lanes,
-            "input": "resident in HBM",
-        },
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
-        "stages_ms": {k: round(avg(k), 3) for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_inflate",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
-            "end_to_end_frac": round((C + n) * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 5),
-        },
-        "cpu_baseline": None,
+    infl_bytes = (C + 2 * n) / launches
+    achieved = infl_bytes / (ms_inflate / launches / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))
+        if t.get("compressed_bytes") == C and t.get("uncompressed_bytes") == n:
+            traffic = t.get("k_inflate_bytes_per_launch")
+    result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, CRC-checked)", value, "GB/s",
+                   args.steps, args.warmup, elapsed * 1e3 / args.steps)
+    result["config"] = {
+        "workload": f"fastq-like {args.gib:g} GiB, gzip -6 single member, 4-line records, 150 bp",
+        "chunk_size_bytes": args.chunk_mib << 20,
+        "compressed_bytes": C,
+        "uncompressed_bytes": n,
+        "segments": int(avg("n_segments")),
+        "lut_bits": args.lut_bits,
+        "lanes_per_block": args.lanes,
+        "input": "resident in HBM",
     }
+    result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
+    result["stages_ms"] = {k: round(avg(k), 3) for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+    result["roofline"] = {
+        "bound": "hbm",
+        "kernel": "k_inflate",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": int(infl_bytes),
+        "end_to_end_frac": round((C + n) * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 5),
+        "other_kernels": {
+            "k_resolve": round(3 * n / (avg("ms_resolve") / 1e3) / 1e9, 1),
+            "k_crc_pieces": round(n / (avg("ms_crc") / 1e3) / 1e9, 1),
+            "unit": "GB/s algorithmic (resolve 3U, CRC U)",
+        },
+    }
+    result["cpu_baseline"] = None
     d_in.free()
+    del raw
 
     if not args.no_cpu:
         from oracle import oracle
@@ -157,11 +179,10 @@ def main() -> int:
         sn = min(n, args.cpu_sample_mib << 20)
         sraw = synth.fastq(sn, threads=threads)
         sgz = synth.gzip(sraw, 6, threads=threads, piece=64 << 20)
-        cores = threads
         times = []
         for _ in range(3):
             t1 = time.perf_counter()
-            rc, _, st, _ = oracle.read_gz(sgz, chunk_size=args.chunk_mib << 20, threads=cores, keep=False)
+            rc, _, st, _ = oracle.read_gz(sgz, chunk_size=args.chunk_mib << 20, threads=threads, keep=False)
             times.append(time.perf_counter() - t1)
             if rc != 0 or st.uncompressed_bytes != sn:
                 raise RuntimeError("CPU baseline decode failed")
@@ -169,12 +190,208 @@ def main() -> int:
         result["cpu_baseline"] = {
             "value": round(sn / t_med / 1e9, 3),
             "unit": "GB/s",
-            "cores": cores,
+            "cores": threads,
             "kind": "port",
             "sample": f"{sn >> 20} MiB of the same FASTQ-like generator, gzip -6, same chunking; median of 3",
         }
         log(f"cpu baseline {result['cpu_baseline']}")
+    return result
 
+
+# ---------------------------------------------------------------------------
+def run_bgzf(args, threads):
+    import numpy as np
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import hipmem, synth
+
+    n = int(args.gib * (1 << 30))
+    raw = synth.fastq(n, threads=threads)
+    t0 = time.time()
+    bg = synth.bgzf(raw, 6, threads=threads)
+    C = int(bg.size)
+    log(f"BGZF level 6: {C} bytes in {time.time() - t0:.1f}s")
+    d_in = hipmem.DeviceBuffer.from_host(bg)
+    cfg = make_cfg(args)
+    dec = rrg.Decoder(0)
+
+    def step():
+        return dec.decode_device(d_in.ptr, C, bg, cfg)
+
+    for w in range(max(1, args.warmup)):
+        ptr, out_len, ends, st = step()
+        if w == 0:
+            assert out_len == n and np.array_equal(hipmem.to_host(ptr, n), raw)
+            log(f"warmup verified: {len(ends)} batches")
+    prof = []
+
+    def step_p():
+        step()
+        prof.append(dec.profile().as_dict())
+
+    elapsed, lat = timed_steps(step_p, args.steps)
+    result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, BGZF)", n * args.steps / elapsed / 1e9,
+                   "GB/s", args.steps, args.warmup, elapsed * 1e3 / args.steps)
+    result["config"] = {"workload": f"fastq-like {args.gib:g} GiB as BGZF (65280-byte members, level 6)",
+                        "compressed_bytes": C, "uncompressed_bytes": n,
+                        "members": int(prof[-1]["n_segments"]), "input": "resident in HBM"}
+    result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
+    result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3)
+                           for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+    d_in.free()
+    return result
+
+
+# ---------------------------------------------------------------------------
+def run_batch(args, threads):
+    import numpy as np
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import hipmem, synth
+
+    nf, fsz = args.batch_files, args.batch_mib << 20
+    raws, gzs = [], []
+    for i in range(nf):
+        r = synth.fastq(fsz, seed=synth.SEED_FASTQ + 1000 + i, threads=threads)
+        raws.append(r)
+        gzs.append(synth.gzip(r, 6, threads=threads, piece=16 << 20))
+    bufs = [hipmem.DeviceBuffer.from_host(g) for g in gzs]
+    ptrs, lens = [b.ptr for b in bufs], [int(g.size) for g in gzs]
+    cfg = make_cfg(args)
+    dec = rrg.Decoder(0)
+    U = nf * fsz
+
+    def batched():
+        return dec.decode_batch(gzs, cfg, device_ptrs=ptrs, device_lens=lens, copy=False)
+
+    def plain():
+        for i in range(nf):
+            dec.decode_device(ptrs[i], lens[i], gzs[i], cfg)
+
+    for w in range(max(1, args.warmup)):
+        res = batched()
+        if w == 0:
+            for i in (0, nf - 1):
+                assert res[i].error is None and np.array_equal(hipmem.to_host(res[i].d_out, fsz), raws[i])
+        plain()
+    t_b, lat_b = timed_steps(batched, args.steps)
+    t_p, lat_p = timed_steps(plain, args.steps)
+    vb, vp = U * args.steps / t_b / 1e9, U * args.steps / t_p / 1e9
+    result = _base("decompressed GB/s on 1 MI355X (batched decode of many files)", vb, "GB/s", args.steps,
+                   args.warmup, t_b * 1e3 / args.steps)
+    result["config"] = {"workload": f"{nf} FASTQ-like files of {args.batch_mib} MiB, gzip -6, one batched call",
+                        "compressed_bytes": sum(lens), "uncompressed_bytes": U, "input": "resident in HBM"}
+    result["plain_path"] = {"value": round(vp, 3), "unit": "GB/s", "how": "one rrg_gpu_decode_device call per file"}
+    result["batched_over_plain"] = round(vb / vp, 2)
+    result["latency_ms"] = {"batch_p50": _pct(lat_b, 50), "batch_p99": _pct(lat_b, 99)}
+    for b in bufs:
+        b.free()
+    return result
+
+
+# ---------------------------------------------------------------------------
+def run_serve(args, threads):
+    import numpy as np
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import synth
+
+    rng = np.random.default_rng(0x5E4E0005)
+    lo, hi = 64 << 10, args.serve_max_mib << 20
+    pool = []  # (gz bytes, uncompressed size)
+    for g in ("text", "fastq", "mixed"):
+        for lvl in (1, 6, 9):
+            for _ in range(args.serve_pool):
+                size = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
+                seed = 0x5E4E0005 + 97 * len(pool)
+                if g == "text":
+                    raw = synth.text(size, seed=seed, threads=threads)
+                    gz = synth.gzip(raw, lvl, threads=threads, piece=16 << 20)
+                elif g == "fastq":
+                    raw = synth.fastq(size, seed=seed, threads=threads)
+                    gz = synth.gzip(raw, lvl, threads=threads, piece=16 << 20)
+                else:
+                    raw, lv, ends = synth.mixed(size, seed=seed, threads=threads)
+                    gz = synth.gzip_segments(raw, lv, ends, threads=threads, piece=16 << 20)
+                pool.append((gz.tobytes(), int(raw.size)))
+    log(f"request pool: {len(pool)} inputs, {sum(u for _, u in pool) >> 20} MiB uncompressed")
+    reqs = [pool[i] for i in rng.integers(0, len(pool), args.serve_requests)]
+    cfg = rrg.Config(chunk_size_bytes=args.chunk_mib << 20)
+
+    def run(rate):
+        """rate None: everything submitted at once (closed loop); else Poisson arrivals per second."""
+        with rrg.Server(0, cfg, hbm_budget_bytes=args.serve_budget_gib << 30) as srv:
+            done, lat, batches, total = 0, [], set(), 0
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
+            t0 = time.perf_counter()
+            k = 0
+            while done < len(reqs):
+                now = time.perf_counter() - t0
+                while k < len(reqs) and arrivals[k] <= now:
+                    srv.submit(reqs[k][0], tag=k)
+                    k += 1
+                for c in srv.poll(256, timeout_ms=1000 if k >= len(reqs) else 1):
+                    if c.error is not None or c.out_len != reqs[c.tag][1]:
+                        raise RuntimeError(f"request {c.tag} failed: {c.error}")
+                    lat.append(c.ms_total)
+                    batches.add(c.batch)
+                    total += c.out_len
+                    srv.release(c.id)
+                    done += 1
+            el = time.perf_counter() - t0
+        return total / el / 1e9, lat, len(batches), el
+
+    run(None)  # warmup: first allocations
+    cap, lat0, nb0, el0 = run(None)
+    log(f"closed loop: {cap:.2f} GB/s, {nb0} batches, {el0:.2f}s")
+    u_mean = sum(u for _, u in reqs) / len(reqs)
+    sweeps = []
+    for frac in args.serve_loads:
+        rate = frac * cap * 1e9 / u_mean
+        thr, lat, nb, el = run(rate)
+        sweeps.append({"offered_frac": frac, "req_per_s": round(rate, 1), "GB_per_s": round(thr, 3),
+                       "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "batches": nb})
+        log(sweeps[-1])
+    result = _base("decompressed GB/s on 1 MI355X (serving loop, request mix)", cap, "GB/s", 1, 1, el0 * 1e3)
+    result["config"] = {
+        "workload": f"{len(reqs)} requests, U log-uniform on [64 KiB, {args.serve_max_mib} MiB], "
+                    "text/FASTQ/mixed x levels 1/6/9, inputs from host memory, outputs left in HBM",
+        "pool_inputs": len(pool),
+        "hbm_budget_gib": args.serve_budget_gib,
+    }
+    result["latency_ms"] = {"closed_loop_p50": _pct(lat0, 50), "closed_loop_p99": _pct(lat0, 99)}
+    result["poisson"] = sweeps
+    return result
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="fastq4g", choices=("fastq4g", "bgzf", "batch", "serve"))
+    ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
+    ap.add_argument("--chunk-mib", type=int, default=4)
+    ap.add_argument("--segment-kib", type=int, default=0, help="0 = automatic")
+    ap.add_argument("--lut-bits", type=int, default=9)
+    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--cpu-sample-mib", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-files", type=int, default=64)
+    ap.add_argument("--batch-mib", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-max-mib", type=int, default=64)
+    ap.add_argument("--serve-pool", type=int, default=2, help="inputs per generator x level cell")
+    ap.add_argument("--serve-budget-gib", type=int, default=32)
+    ap.add_argument("--serve-loads", type=float, nargs="*", default=[0.5, 0.9])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1"}))
+        log("only --gpus 1 is supported: the decoder runs on a single MI355X")
+        return 2
+    threads = min(16, os.cpu_count() or 1)
+    fn = {"fastq4g": run_fastq, "bgzf": run_bgzf, "batch": run_batch, "serve": run_serve}[args.workload]
+    result = fn(args, threads)
     print(json.dumps(result), flush=True)
     return 0
 

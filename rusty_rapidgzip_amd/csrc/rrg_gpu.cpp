@@ -262,8 +262,9 @@ int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>
         t.stop();
         c.prof.n_tail_rounds += nb;
     }
-    // resolve tiles: 64 KiB of output each, aligned in output coordinates
-    const uint64_t TILE = 65536;
+    // resolve tiles: up to 256 KiB of output each, aligned in output
+    // coordinates (a tile loads its segment's 32 KiB marker window once)
+    const uint64_t TILE = 256 * 1024;
     std::vector<ResolveTile> tiles;
     for (uint32_t j = 0; j < n; ++j) {
         const uint64_t ob = out_off[j], oe = ob + cs[j].len;

@@ -42,6 +42,8 @@ __device__ __forceinline__ uint32_t byte_at(const uint64_t* __restrict__ in64, u
 
 // RFC 1951 code-length order (deflate/tables.rs:4-6)
 __constant__ uint8_t c_cl_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+// position of each precode symbol in the header's length order (inverse of c_cl_order)
+__constant__ uint8_t c_cl_pos[19] = {3, 17, 15, 13, 11, 9, 7, 5, 4, 6, 8, 10, 12, 14, 16, 18, 0, 1, 2};
 
 // Length / distance bases computed instead of looked up (same values as
 // deflate/tables.rs:10-30).
@@ -177,27 +179,29 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     const uint32_t hlit = 257u + (uint32_t)((w0 >> 3) & 31u);
     const uint32_t hdist = 1u + (uint32_t)((w0 >> 8) & 31u);
     const uint32_t hclen = 4u + (uint32_t)((w0 >> 13) & 15u);
-    // precode lengths (in symbol order) and canonical codes; every lane builds
-    // the two LUT entries lane and lane+64
-    uint8_t cl[19];
-    for (int i = 0; i < 19; ++i) cl[i] = 0;
+    // precode lengths and canonical codes, one symbol per lane (lanes 0..18),
+    // counts and ranks by ballot: everything stays in registers.  Every lane
+    // then builds the two LUT entries lane and lane+64.
     const uint64_t w1 = ubits(in64, fw, rbase, p + 17);
-    for (uint32_t i = 0; i < hclen; ++i) cl[c_cl_order[i]] = (uint8_t)((w1 >> (3 * i)) & 7u);
+    const uint32_t k = lane < 19 ? c_cl_pos[lane] : 19u;  // position of symbol `lane` in the header order
+    const uint32_t my_len = (k < hclen) ? (uint32_t)((w1 >> (3 * k)) & 7u) : 0u;
+    uint32_t my_code = 0;
     {
-        uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 19; ++i) cnt[cl[i]]++;
-        cnt[0] = 0;
-        uint32_t next[8];
-        uint32_t code = 0;
-        for (int l = 1; l < 8; ++l) {
-            code = (code + cnt[l - 1]) << 1;
-            next[l] = code;
+        uint32_t code = 0, cnt_prev = 0;
+        for (uint32_t l = 1; l < 8; ++l) {
+            const uint64_t same = __ballot(my_len == l);
+            code = (code + cnt_prev) << 1;  // first code of length l
+            if (my_len == l) my_code = code + (uint32_t)__popcll(same & ((1ull << lane) - 1));
+            cnt_prev = (uint32_t)__popcll(same);
         }
+    }
+    const uint32_t my_rev = my_len ? (__brev(my_code) >> (32 - my_len)) : 0u;
+    {
         uint8_t e0 = 0, e1 = 0;
         for (int sy = 0; sy < 19; ++sy) {
-            const uint32_t l = cl[sy];
+            const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)my_len, sy);
             if (!l) continue;
-            const uint32_t rev = __brev(next[l]++) >> (32 - l);
+            const uint32_t rev = (uint32_t)__builtin_amdgcn_readlane((int)my_rev, sy);
             const uint32_t m = (1u << l) - 1;
             if (((uint32_t)lane & m) == rev) e0 = (uint8_t)((sy << 3) | l);
             if (((uint32_t)(lane + 64) & m) == rev) e1 = (uint8_t)((sy << 3) | l);
@@ -1135,17 +1139,60 @@ __device__ __forceinline__ uint32_t resolve_cell(const ChainSeg* __restrict__ cs
     return cell & 0xffu;
 }
 
-__global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__ tiles,
-                                                 const ChainSeg* __restrict__ cs, const uint16_t* __restrict__ cells,
-                                                 uint8_t* __restrict__ out, uint32_t* __restrict__ err) {
+constexpr int RESOLVE_THREADS = 512;
+constexpr int WIN_PARTS = 64;  // chain segments a window may be gathered from
+
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(const ResolveTile* __restrict__ tiles,
+                                                             const ChainSeg* __restrict__ cs,
+                                                             const uint16_t* __restrict__ cells,
+                                                             uint8_t* __restrict__ out, uint32_t* __restrict__ err) {
+    // byte that marker offset m names: win[WIN - 1 - m], valid for m < s_avail
+    __shared__ uint8_t win[WIN];
+    __shared__ uint64_t part_src[WIN_PARTS];
+    __shared__ uint32_t part_dst[WIN_PARTS], part_n[WIN_PARTS];
+    __shared__ uint32_t s_parts, s_avail;
     const ResolveTile t = tiles[blockIdx.x];
+    const bool spec = cs[t.seg].spec != 0;
+    if (spec) {
+        // the 32 Ki bytes before the segment: tail zones (already literal) of
+        // the chain segments before it, back to the start of its file
+        if (threadIdx.x == 0) {
+            uint32_t rem = WIN, np = 0;
+            for (int64_t s = (int64_t)t.seg - 1; s >= 0 && rem; --s) {
+                const uint64_t len = cs[s].len;
+                const uint32_t take = (uint32_t)(len < rem ? len : rem);
+                if (take) {
+                    if (np == WIN_PARTS) break;  // fragmented window: markers use the chain walk
+                    part_src[np] = cs[s].cells_base + len - take;
+                    part_dst[np] = rem - take;
+                    part_n[np] = take;
+                    ++np;
+                    rem -= take;
+                }
+                if (cs[s].first) break;
+            }
+            s_parts = (rem && np == WIN_PARTS) ? ~0u : np;
+            s_avail = WIN - rem;
+        }
+        __syncthreads();
+        const uint32_t np = s_parts;
+        if (np != ~0u)
+            for (uint32_t q = 0; q < np; ++q) {
+                const uint64_t src = part_src[q];
+                const uint32_t dst = part_dst[q], n = part_n[q];
+                for (uint32_t i = threadIdx.x; i < n; i += RESOLVE_THREADS) win[dst + i] = (uint8_t)cells[src + i];
+            }
+        __syncthreads();
+    }
+    const bool lds_win = spec && s_parts != ~0u;
+    const uint32_t avail = spec ? s_avail : 0u;
     const uint64_t ob = t.out_off, oe = t.out_off + t.n;
     // cell index of output byte p is delta + p
     const int64_t delta = (int64_t)t.cell_addr - (int64_t)ob;
     const uint32_t phase = (uint32_t)((uint64_t)delta & 7u);  // cells, within a 16-byte cell group
     const uint32_t dsh = phase >> 1, odd = phase & 1u;
     const uint4* c128 = reinterpret_cast<const uint4*>(cells);
-    for (uint64_t p0 = (ob & ~15ull) + 16ull * threadIdx.x; p0 < oe; p0 += 16ull * 256) {
+    for (uint64_t p0 = (ob & ~15ull) + 16ull * threadIdx.x; p0 < oe; p0 += 16ull * RESOLVE_THREADS) {
         const int64_t q = (delta + (int64_t)p0 - (int64_t)phase) >> 3;  // first 16-byte cell group
         // a group before the buffer start only holds cells of bytes before the tile
         const uint4 g0 = c128[q < 0 ? 0 : q], g1 = c128[q + 1 < 0 ? 0 : q + 1], g2 = c128[q + 2 < 0 ? 0 : q + 2];
@@ -1175,7 +1222,20 @@ __global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__
                 for (int h = 0; h < 4; ++h) {
                     const uint32_t cell = (o[2 * m + (h >> 1)] >> (16 * (h & 1))) & 0xffffu;
                     const uint64_t p = p0 + (uint64_t)(4 * m + h);
-                    const uint32_t byte = (p >= ob && p < oe) ? resolve_cell(cs, cells, t.seg, cell, err) : 0u;
+                    uint32_t byte = cell & 0xffu;
+                    if ((cell & 0x8000u) && p >= ob && p < oe) {
+                        const uint32_t off = cell & 0x7fffu;
+                        if (lds_win) {
+                            if (off < avail) {
+                                byte = win[WIN - 1 - off];
+                            } else {
+                                err[t.seg] = 1u;
+                                byte = 0;
+                            }
+                        } else {
+                            byte = resolve_cell(cs, cells, t.seg, cell, err);
+                        }
+                    }
                     v |= byte << (8 * h);
                 }
                 b[m] = v;
@@ -1195,7 +1255,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ResolveTile* __restrict__
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs, const uint16_t* d_cells,
                           uint8_t* d_out, uint32_t* d_err, hipStream_t s) {
     if (n_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_resolve, dim3(n_tiles), dim3(256), 0, s, d_tiles, d_cs, d_cells, d_out, d_err);
+    hipLaunchKernelGGL(k_resolve, dim3(n_tiles), dim3(RESOLVE_THREADS), 0, s, d_tiles, d_cs, d_cells, d_out, d_err);
     return hipGetLastError();
 }
 
