@@ -186,14 +186,15 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
 }
 
 // Give a segment a larger output region (cells or member records) after it
-// overflowed; the old region is abandoned.
-void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& mb_used) {
+// overflowed; the old region is abandoned.  A cell overflow reports the cells
+// the whole segment needs (the kernel counts on without storing).
+void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& mb_used, uint64_t needed) {
     if (status == ST_MB_OVERFLOW) {
         d.mb_base = (uint32_t)mb_used;
         d.mb_cap = d.mb_cap * 16 + 16;
         mb_used += d.mb_cap;
     } else {
-        const uint64_t cap = d.cells_cap * 8 + 65536;
+        const uint64_t cap = (std::max<uint64_t>(needed + 64, 2 * d.cells_cap) + 7) & ~7ull;
         d.cells_base = (cells_used + 7) & ~7ull;
         d.cells_cap = cap;
         cells_used = d.cells_base + cap;
@@ -412,7 +413,9 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
             const uint64_t t = k * chunk_bits + j * seg_bits;
             if (t >= total) break;
             targets.push_back(base + t);
-            limits.push_back(J.end_bits() | kAnyStart);
+            // a segment target is only a hint: search two segments' worth of
+            // input (long stored or block-free stretches have no starts at all)
+            limits.push_back(std::min(J.end_bits(), base + t + 2 * seg_bits) | kAnyStart);
             is_chunk.push_back(0);
         }
     }
@@ -424,6 +427,7 @@ struct Redo {
     size_t job, pos;
     uint64_t start;
     uint32_t status;  // ST_OK: false start; ST_OVERFLOW / ST_MB_OVERFLOW: region too small
+    uint64_t needed;  // cells the segment needs (ST_OVERFLOW)
 };
 
 // Anchored verification chain of one job (pipeline.rs:774-809): accept a
@@ -454,7 +458,7 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
             if (redo.size() == before) J.fatal = (long)k;
             break;
         }
-        redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK});
+        redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK, r.n_cells});
         if (k + 1 < n) expected = desc[J.order[k + 1]].start_bit;
     }
 }
@@ -582,7 +586,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         const uint64_t pieces = std::min<uint64_t>(kMaxPieces, span / kPieceBits);
         for (uint64_t p = 1; p < pieces; ++p) {
             targets.push_back(R.start + span * p / pieces);
-            limits.push_back(J.end_bits() | kAnyStart);
+            limits.push_back(end | kAnyStart);  // finds past the segment are not used
         }
     }
     t_first[redo.size()] = targets.size();
@@ -620,7 +624,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         const uint64_t hint = d.hint_bit;
         const uint64_t end = hint == UINT64_MAX ? J.end_bits() : hint;
         if (overflowed(R.status)) {
-            grow_segment(d, R.status, cells_used, mb_used);
+            grow_segment(d, R.status, cells_used, mb_used, R.needed);
             ++c.prof.n_grown;
         }
         d.start_bit = R.start;
@@ -791,7 +795,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             if (round < 6)
                 for (size_t j = J.bg_lo; j < J.bg_hi; ++j)
                     if (overflowed(res[j].status)) {
-                        grow_segment(desc[j], res[j].status, cells_used, mb_used);
+                        grow_segment(desc[j], res[j].status, cells_used, mb_used, res[j].n_cells);
                         ++c.prof.n_grown;
                         again.push_back((uint32_t)j);
                     }

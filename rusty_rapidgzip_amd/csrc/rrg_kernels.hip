@@ -363,26 +363,49 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
             cur = base + 64;
             continue;
         }
-        const int nx = (step == 0) ? 1 : (int)step;
-        // chain walk in window-relative 32-bit offsets: scalar compares only
+        // The reference's skip chain (block_finder.rs:109-141) over this window,
+        // followed with jump tables: J[k][l] = the position 2^k chain steps after
+        // l (64 = left the window).  Whether a candidate lies on the chain, and
+        // where the chain leaves the window, then take six readlanes each
+        // instead of a walk of up to 64 dependent steps.
+        const uint32_t nxu = (step == 0) ? 1u : step;
+        uint32_t J[6];
+        J[0] = min((uint32_t)lane + nxu, 64u);
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+            const uint32_t prev = J[k - 1];
+            const uint32_t nxt = (uint32_t)__shfl((int)prev, (int)min(prev, 63u), 64);
+            J[k] = prev >= 64u ? 64u : nxt;
+        }
         const uint64_t room = limit - 13 - base;  // base + 13 <= limit here
-        const uint32_t rel_max = room < 64 ? (uint32_t)room : 64u;
-        uint32_t rel = 0;
-        while (rel < 64) {
-            if (rel > rel_max) {
+        uint32_t pth = 0;                           // last chain position visited
+        uint64_t m = cm;
+        while (m) {
+            const uint32_t c = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+#pragma unroll
+            for (int k = 5; k >= 0; --k) {
+                const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
+                if (q <= c) pth = q;
+            }
+            if (pth != c) continue;  // the chain jumps over this candidate
+            if (uni32(deep_verify(in64, fw, rbase, nbits, base + c, lane) ? 1u : 0u)) {
+                result = base + c;
                 end = true;
                 break;
             }
-            if ((cm >> rel) & 1ull) {
-                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
-                    result = base + rel;
-                    end = true;
-                    break;
-                }
-            }
-            rel = uni32(rel + (uint32_t)__builtin_amdgcn_readlane(nx, (int)rel));
         }
-        cur = base + rel;
+        if (end) break;
+        if (room < 64) {  // every later chain position is past the limit
+            end = true;
+            break;
+        }
+#pragma unroll
+        for (int k = 5; k >= 0; --k) {
+            const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
+            if (q < 64u) pth = q;
+        }
+        cur = base + pth + uni32((uint32_t)__builtin_amdgcn_readlane((int)nxu, (int)pth));
     }
     if (lane == 0) found[t] = result;
 }
@@ -553,6 +576,9 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     uint32_t bitcnt = 0;
     uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
     uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
+    // counting: the output region is full; decode on without storing so the
+    // segment reports the size it needs (one regrow instead of repeated ones)
+    uint32_t counting = 0;
     uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
 
 // 64 bits at pos (the input buffer is padded, so words up to 8 past the end
@@ -598,6 +624,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 const SegDesc d = segs[seg];
                 pos = d.start_bit;
                 hint = d.hint_bit;
+                counting = 0;
                 in_len = d.end_byte;
                 nbits = d.end_byte * 8;
                 out_base = d.cells_base;
@@ -643,8 +670,8 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 if (pos + l > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                 RRG_CONSUME(l);
                 if (sym < 256) {
-                    if (out_n >= out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
-                    o[out_n] = (uint16_t)sym;
+                    if (out_n >= out_cap) counting = 1;
+                    if (!counting) o[out_n] = (uint16_t)sym;
                     ++out_n;
                     continue;
                 }
@@ -673,10 +700,15 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_CONSUME(dx);
                 // copies store whole groups of 8 cells (up to 7 past the match end,
                 // overwritten by what follows): keep them inside the region
-                if (out_n + length + 8 > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
+                if (out_n + length + 8 > out_cap) counting = 1;
                 const uint64_t cur = out_n - member_base;
+                if ((uint64_t)distance > cur && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
+                if (counting) {
+                    out_n += length;
+                    RRG_REFILL();
+                    continue;
+                }
                 if ((uint64_t)distance > cur) {
-                    if (!spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
                     const uint32_t gap = (uint32_t)((uint64_t)distance - cur);
                     const uint32_t pre = gap < length ? gap : length;
                     for (uint32_t k = 0; k < pre; ++k) o[out_n + k] = (uint16_t)(0x8000u | (gap - k - 1));
@@ -686,7 +718,34 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 // refill loads now, consumed after the first group's source loads
                 const uint64_t rq = pos >> 6;
                 const uint64_t r0 = in64[rq], r1 = in64[rq + 1];
-                if (length) {
+                if (length && (distance == 1 || distance == 2 || distance == 4)) {
+                    // period 1, 2 or 4: load the pattern once, then store it --
+                    // single cells up to an 8-cell boundary, then whole aligned
+                    // 16-byte groups (the last may run up to 7 cells past the end)
+                    const uint64_t src = out_n - distance;
+                    const uint32_t dm = distance - 1;
+                    uint32_t c0 = o[src], c1 = o[src + (1 & dm)], c2 = o[src + (2 & dm)], c3 = o[src + (3 & dm)];
+                    bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
+                    bitcnt = 64;
+                    asm volatile("" : "+v"(bitbuf), "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
+                    uint32_t k = 0;
+                    while (k < length && ((out_n + k) & 7)) {
+                        const uint32_t j = k & 3;
+                        o[out_n + k] = (uint16_t)(j == 0 ? c0 : j == 1 ? c1 : j == 2 ? c2 : c3);
+                        ++k;
+                    }
+                    if (k < length) {
+                        const uint32_t ph = k & 3;  // pattern phase at the first aligned cell
+                        const uint32_t q0 = ph == 0 ? c0 : ph == 1 ? c1 : ph == 2 ? c2 : c3;
+                        const uint32_t q1 = ph == 0 ? c1 : ph == 1 ? c2 : ph == 2 ? c3 : c0;
+                        const uint32_t q2 = ph == 0 ? c2 : ph == 1 ? c3 : ph == 2 ? c0 : c1;
+                        const uint32_t q3 = ph == 0 ? c3 : ph == 1 ? c0 : ph == 2 ? c1 : c2;
+                        const uint4 g = make_uint4(q0 | (q1 << 16), q2 | (q3 << 16), q0 | (q1 << 16), q2 | (q3 << 16));
+                        uint4* gp = reinterpret_cast<uint4*>(o + out_n + k);
+                        for (; k < length; k += 8) *gp++ = g;
+                    }
+                    out_n += length;
+                } else if (length) {
                     // o[out_n + t] = o[src + (t mod distance)]: every load reads a
                     // cell written before its group (all of them for distance < 8;
                     // earlier groups for distance >= 8)
@@ -734,7 +793,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_GETBITS(nln, 16);
                 if ((ln ^ nln) != 0xffffu) RRG_FAIL(ST_INVALID, DET_STORED_NLEN);
                 if (pos + 8ull * ln > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                if (out_n + ln > out_cap) RRG_FAIL(ST_OVERFLOW, DET_NONE);
+                if (out_n + ln > out_cap) counting = 1;
                 stored_left = ln;
                 state = S_STORED;
             } else if (bt == 1) {
@@ -813,12 +872,28 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_FAIL(ST_INVALID, DET_RESERVED_BTYPE);
             }
         } else if (state == S_STORED) {
-            // ---- stored block, 8 bytes per step (fast_inflate.rs:1937-1963) ----
-            for (int it = 0; it < 32 && stored_left; ++it) {
-                const uint64_t w = bits64_at(in64, pos);
-                const uint32_t n = stored_left < 8 ? stored_left : 8u;
-                uint16_t* o = cells + out_base + out_n;
-                for (uint32_t k = 0; k < n; ++k) o[k] = (uint16_t)((w >> (8 * k)) & 0xffu);
+            // ---- stored block, 32 bytes per step (fast_inflate.rs:1937-1963) ----
+            // (five words loaded together, one wait per 32 bytes; the block
+            // lies inside the input, so the words read exist)
+            for (int it = 0; it < 16 && stored_left; ++it) {
+                const uint32_t n = stored_left < 32 ? stored_left : 32u;
+                const uint64_t q = pos >> 6;
+                const uint32_t sh = (uint32_t)(pos & 63);
+                uint64_t w[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) w[i] = in64[q + i];
+                if (!counting) {
+                    uint16_t* o = cells + out_base + out_n;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint64_t v = funnel_lo(w[i], w[i + 1], sh);
+#pragma unroll
+                        for (int b = 0; b < 8; ++b)
+                            if ((uint32_t)(8 * i + b) < n) o[8 * i + b] = (uint16_t)((v >> (8 * b)) & 0xffu);
+                    }
+                } else {
+                    asm volatile("" : "+v"(w[4]));
+                }
                 out_n += n;
                 pos += 8ull * n;
                 stored_left -= n;
@@ -867,6 +942,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
         }
         continue;
     finish : {
+        if (counting && status == ST_OK) status = ST_OVERFLOW;  // n_cells = the cells it needs
         SegResult r;
         r.n_cells = out_n;
         r.end_bit = pos;
