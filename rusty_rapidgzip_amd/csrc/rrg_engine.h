@@ -64,7 +64,7 @@ struct Ctx {
     int device = 0;
     hipStream_t s = nullptr;
     int n_cu = 0;
-    DevBuf in, targets, limits, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
+    DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
         pcrc, err;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
@@ -73,7 +73,7 @@ struct Ctx {
     rrg_profile prof{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Ctx() {
-        DevBuf* all[] = {&in,    &targets, &limits, &found, &segs, &res,  &mrec, &work, &counter, &cells,
+        DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
                          &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err};
         for (DevBuf* b : all) b->release();
         if (pin) (void)hipHostFree(pin);

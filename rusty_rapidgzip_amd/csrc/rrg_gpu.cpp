@@ -389,7 +389,7 @@ constexpr uint64_t kAnyStart = 1ull << 63;
 // Targets of one member-path body: chunk targets (pipeline.rs:497-503) and
 // segment targets between them.
 void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& targets, std::vector<uint64_t>& limits,
-                  std::vector<uint8_t>& is_chunk) {
+                  std::vector<uint64_t>& scan_ends, std::vector<uint8_t>& is_chunk) {
     const uint64_t base = J.abs_bits(J.body);
     const uint64_t total = J.end_bits() - base;
     const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
@@ -407,6 +407,7 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
         if (k) {
             targets.push_back(base + k * chunk_bits);
             limits.push_back(J.end_bits());
+            scan_ends.push_back(J.end_bits());
             is_chunk.push_back(1);
         }
         for (uint64_t j = 1; j < D; ++j) {
@@ -414,8 +415,9 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
             if (t >= total) break;
             targets.push_back(base + t);
             // a segment target is only a hint: search two segments' worth of
-            // input (long stored or block-free stretches have no starts at all)
-            limits.push_back(std::min(J.end_bits(), base + t + 2 * seg_bits) | kAnyStart);
+            // input (long stretches without block starts are not scanned through)
+            limits.push_back(J.end_bits() | kAnyStart);
+            scan_ends.push_back(std::min(J.end_bits(), base + t + 2 * seg_bits));
             is_chunk.push_back(0);
         }
     }
@@ -565,6 +567,36 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     }
 }
 
+// One finder launch; every find must lie in [target, scan end) of its file.
+int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
+               const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err) {
+    found.assign(targets.size(), UINT64_MAX);
+    if (targets.empty()) return RRG_OK;
+    const size_t n = targets.size();
+    HIPCHK(c.targets.ensure(n * 8));
+    HIPCHK(c.limits.ensure(n * 8));
+    HIPCHK(c.scan_ends.ensure(n * 8));
+    HIPCHK(c.found.ensure(n * 8));
+    HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), n * 8, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), n * 8, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.scan_ends.p, scan_ends.data(), n * 8, hipMemcpyHostToDevice, c.s));
+    StageTimer t(c, &c.prof.ms_find);
+    HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
+                              c.limits.as<uint64_t>(), c.scan_ends.as<uint64_t>(), c.found.as<uint64_t>(),
+                              (uint32_t)n, c.s));
+    HIPCHK(hipMemcpyAsync(found.data(), c.found.p, n * 8, hipMemcpyDeviceToHost, c.s));
+    t.stop();
+    HIPCHK(hipStreamSynchronize(c.s));
+    // a find is a bit offset inside the target's scan range; anything else is a device fault
+    for (size_t i = 0; i < n; ++i)
+        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= scan_ends[i])) {
+            set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu", (unsigned long long)found[i],
+                    (unsigned long long)targets[i]);
+            return RRG_E_DEVICE;
+        }
+    return RRG_OK;
+}
+
 // Turn the re-decode requests of one settle round into segments to decode:
 // each requested segment restarts at its verified start and, when it spans
 // enough input, is cut at the blocks the finder reports inside it (one finder
@@ -574,7 +606,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
     if (redo.empty()) return RRG_OK;
     constexpr uint64_t kPieceBits = 8 * 8192;  // at least 8 KiB of input per piece
     constexpr uint64_t kMaxPieces = 8;
-    std::vector<uint64_t> targets, limits;
+    std::vector<uint64_t> targets, limits, scan_ends;
     std::vector<size_t> t_first(redo.size() + 1);
     for (size_t i = 0; i < redo.size(); ++i) {
         t_first[i] = targets.size();
@@ -586,30 +618,14 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         const uint64_t pieces = std::min<uint64_t>(kMaxPieces, span / kPieceBits);
         for (uint64_t p = 1; p < pieces; ++p) {
             targets.push_back(R.start + span * p / pieces);
-            limits.push_back(end | kAnyStart);  // finds past the segment are not used
+            limits.push_back(J.end_bits() | kAnyStart);
+            scan_ends.push_back(end);  // finds past the segment are not used
         }
     }
     t_first[redo.size()] = targets.size();
-    std::vector<uint64_t> found(targets.size(), UINT64_MAX);
-    if (!targets.empty()) {
-        HIPCHK(c.targets.ensure(targets.size() * 8));
-        HIPCHK(c.limits.ensure(limits.size() * 8));
-        HIPCHK(c.found.ensure(targets.size() * 8));
-        HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), targets.size() * 8, hipMemcpyHostToDevice, c.s));
-        HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), limits.size() * 8, hipMemcpyHostToDevice, c.s));
-        StageTimer t(c, &c.prof.ms_find);
-        HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
-                                  c.limits.as<uint64_t>(), c.found.as<uint64_t>(), (uint32_t)targets.size(), c.s));
-        HIPCHK(hipMemcpyAsync(found.data(), c.found.p, found.size() * 8, hipMemcpyDeviceToHost, c.s));
-        t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
-        for (size_t i = 0; i < found.size(); ++i)
-            if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= (limits[i] & ~kAnyStart))) {
-                set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu",
-                        (unsigned long long)found[i], (unsigned long long)targets[i]);
-                return RRG_E_DEVICE;
-            }
-    }
+    std::vector<uint64_t> found;
+    const int frc = run_finder(c, targets, limits, scan_ends, found, err);
+    if (frc) return frc;
     // later positions first, so earlier insertions do not move them
     std::vector<size_t> idx(redo.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
@@ -668,34 +684,16 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // 1. host plan
     for (Job& J : jobs) plan_job(J);
     // 2. block finder over every member-path body
-    std::vector<uint64_t> targets, limits;
+    std::vector<uint64_t> targets, limits, scan_ends;
     std::vector<uint8_t> is_chunk;
     for (Job& J : jobs) {
         J.t_lo = targets.size();
-        if (J.body != UINT64_MAX) make_targets(J, G, targets, limits, is_chunk);
+        if (J.body != UINT64_MAX) make_targets(J, G, targets, limits, scan_ends, is_chunk);
         J.t_hi = targets.size();
     }
-    std::vector<uint64_t> found(targets.size());
-    if (!targets.empty()) {
-        HIPCHK(c.targets.ensure(targets.size() * 8));
-        HIPCHK(c.limits.ensure(limits.size() * 8));
-        HIPCHK(c.found.ensure(targets.size() * 8));
-        HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), targets.size() * 8, hipMemcpyHostToDevice, c.s));
-        HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), limits.size() * 8, hipMemcpyHostToDevice, c.s));
-        StageTimer t(c, &c.prof.ms_find);
-        HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), total_bits, c.targets.as<uint64_t>(), c.limits.as<uint64_t>(),
-                                  c.found.as<uint64_t>(), (uint32_t)targets.size(), c.s));
-        HIPCHK(hipMemcpyAsync(found.data(), c.found.p, found.size() * 8, hipMemcpyDeviceToHost, c.s));
-        t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
-    }
-    // a find is a bit offset inside the target's file; anything else is a device fault
-    for (size_t i = 0; i < found.size(); ++i)
-        if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= (limits[i] & ~kAnyStart))) {
-            set_err(err, RRG_E_DEVICE, "block finder returned %llu for target %llu", (unsigned long long)found[i],
-                    (unsigned long long)targets[i]);
-            return RRG_E_DEVICE;
-        }
+    std::vector<uint64_t> found;
+    int rc = run_finder(c, targets, limits, scan_ends, found, err);
+    if (rc) return rc;
     // 3. segment descriptors
     std::vector<SegDesc> desc;
     uint64_t cells_used = 0, mb_used = 0;
@@ -776,7 +774,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<SegResult> res;
     std::vector<uint32_t> all(S);
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
-    int rc = run_inflate(c, G, desc, res, cells_used, mb_used, all, err);
+    rc = run_inflate(c, G, desc, res, cells_used, mb_used, all, err);
     if (rc) return rc;
     for (Job& J : jobs) {
         // a capacity overflow is not a decode failure (the region is grown below)

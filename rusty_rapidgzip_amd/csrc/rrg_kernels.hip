@@ -296,6 +296,7 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
 __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
                                                                 const uint64_t* __restrict__ targets,
                                                                 const uint64_t* __restrict__ limits,
+                                                                const uint64_t* __restrict__ scan_ends,
                                                                 uint64_t* __restrict__ found, uint32_t n_targets) {
     __shared__ uint8_t s_next[8192];
     __shared__ FindWave s_wave[FIND_WAVES];
@@ -319,8 +320,11 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     // which is only needed where the exact candidate it picks matters.
     const uint64_t lim_word = uni64(limits[t]);
     const bool any_start = (lim_word >> 63) != 0;
-    const uint64_t limit = lim_word & ~(1ull << 63);
-    const uint64_t nbits = limit;
+    // candidates lie below `limit` (where the scan stops); verification may
+    // read up to `nbits`, the end of the file
+    const uint64_t nbits_file = lim_word & ~(1ull << 63);
+    const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits_file) : nbits_file;
+    const uint64_t nbits = nbits_file;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
     uint64_t rbase = ~0ull;
@@ -349,11 +353,27 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         }
         const uint64_t cm = __ballot(cand);
         if (any_start) {
+            // Stored blocks are block starts too (decoding from one is as good as
+            // from a dynamic header), and long stored stretches hold no dynamic
+            // block at all.  A byte-aligned non-final stored header (low 3 bits
+            // 0, LEN == ~NLEN) whose block is followed by another stored header
+            // is accepted without further checks (2^-32 per byte on random data).
+            bool st = false;
+            if ((p & 7) == 0 && p + 40 <= limit && (lo & 7u) == 0 && ((((lo >> 8) ^ (lo >> 24)) & 0xffffu) == 0xffffu)) {
+                const uint64_t nb = (p >> 3) + 5 + ((lo >> 8) & 0xffffu);  // next header byte
+                if ((nb + 5) * 8 <= nbits) {
+                    const uint32_t h = byte_at(in64, nb), l0 = byte_at(in64, nb + 1), l1 = byte_at(in64, nb + 2),
+                                   n0 = byte_at(in64, nb + 3), n1 = byte_at(in64, nb + 4);
+                    st = (h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu;
+                }
+            }
+            const uint64_t sm = __ballot(st);
             // candidates of this window in order; cand already implies p + 13 <= limit
-            uint64_t m = cm;
+            uint64_t m = cm | sm;
             while (m) {
                 const uint32_t rel = (uint32_t)__builtin_ctzll(m);
-                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
+                if (((sm >> rel) & 1ull) ||
+                    uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
                     result = base + rel;
                     end = true;
                     break;
@@ -411,10 +431,11 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
 }
 
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
-                              const uint64_t* d_limits, uint64_t* d_found, uint32_t n_targets, hipStream_t s) {
+                              const uint64_t* d_limits, const uint64_t* d_scan_ends, uint64_t* d_found,
+                              uint32_t n_targets, hipStream_t s) {
     if (n_targets == 0) return hipSuccess;
     hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
-                       d_in64, total_bits, d_targets, d_limits, d_found, n_targets);
+                       d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_found, n_targets);
     return hipGetLastError();
 }
 

@@ -8,13 +8,16 @@
 namespace rrg {
 
 // Block finder: found[i] = first verified dynamic-block offset in
-// [targets[i], limit) (UINT64_MAX when none), limit = limits[i] without bit 63.
-// Without bit 63 the candidates are visited in the reference's NEXT_LUT skip
-// order (its exact choice); with bit 63 every candidate is tried in bit order
-// (some verified block start).  One wavefront per target.  total_bits: size
-// of the whole (padded) input buffer.
+// [targets[i], scan end) (UINT64_MAX when none).  limits[i] without bit 63 is
+// the end of the target's file (verification reads up to it); the scan ends
+// there or at scan_ends[i] when given.  Without bit 63 the candidates are
+// visited in the reference's NEXT_LUT skip order (its exact choice); with bit
+// 63 every candidate is tried in bit order and verified stored-block headers
+// count as starts too (some block start).  One wavefront per target.
+// total_bits: size of the whole (padded) input buffer.
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
-                              const uint64_t* d_limits, uint64_t* d_found, uint32_t n_targets, hipStream_t s);
+                              const uint64_t* d_limits, const uint64_t* d_scan_ends, uint64_t* d_found,
+                              uint32_t n_targets, hipStream_t s);
 
 // Persistent inflate over the segment indices in d_work[0..n_work).
 struct InflateGeom {

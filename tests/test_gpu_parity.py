@@ -170,6 +170,54 @@ def test_errors_not_faults(decoder):
     check_same(decoder, good, 65536, expect=p)
 
 
+def test_high_ratio_overflow(decoder):
+    # runs expand far beyond a segment's first output region: the lane counts the
+    # cells it needs and the segment is regrown once
+    parts = []
+    rng = np.random.default_rng(21)
+    for i in range(24):
+        parts.append(bytes([65 + i % 26]) * int(rng.integers(1 << 19, 3 << 20)))
+        parts.append(lcg_payload(int(rng.integers(2000, 20000)), i))
+    raw = b"".join(parts)
+    g = gz(raw, 6)
+    _say("high ratio", len(raw), len(g))
+    check_same(decoder, g, 1 << 20, seg=32 << 10, expect=raw)
+    prof = decoder.profile()
+    assert prof.n_inflate_launches <= 3, prof.as_dict()
+
+
+def test_periodic_copies(decoder):
+    # back-references with periods 1, 2, 3, 4 and 8 (the store-only fast path
+    # covers 1, 2 and 4), long and short, next to random text
+    rng = np.random.default_rng(5)
+    parts = []
+    for i in range(400):
+        period = int(rng.choice([1, 2, 3, 4, 8]))
+        unit = lcg_payload(period, 1000 + i)
+        parts.append(unit * int(rng.integers(1, 400)))
+        parts.append(lcg_payload(int(rng.integers(1, 300)), 2000 + i))
+    raw = b"".join(parts)
+    for level in (1, 6, 9):
+        check_same(decoder, gz(raw, level), 65536, seg=32 << 10, expect=raw)
+
+
+def test_stored_stretches(decoder):
+    # level 0: the whole member is stored blocks; segment targets start at
+    # verified stored headers, so the stretch is decoded by many lanes
+    raw = np.random.default_rng(9).integers(0, 256, 24 << 20, dtype=np.uint8).tobytes()
+    g = gz(raw, 0)
+    check_same(decoder, g, 4 << 20, expect=raw)
+    assert decoder.profile().n_segments > 100
+    # stored stretches between compressed ones
+    mixed = raw[: 6 << 20] + lcg_payload(2 << 20, 4) + raw[6 << 20: 9 << 20]
+    from rusty_rapidgzip_amd import synth
+
+    levels = np.array([0, 6, 0], dtype=np.int32)
+    ends = np.array([6 << 20, 8 << 20, 11 << 20], dtype=np.uint64)
+    g2 = synth.gzip_segments(np.frombuffer(mixed, dtype=np.uint8), levels, ends, piece=4 << 20).tobytes()
+    check_same(decoder, g2, 1 << 20, expect=mixed)
+
+
 def test_large_offsets(decoder):
     # compressed offsets beyond 2^31 bits (256 MiB): 64-bit positions end to end
     from rusty_rapidgzip_amd import synth
