@@ -203,7 +203,7 @@ class Config:
     verbose: Verbosity = Verbosity.Off
     segment_bytes: int = 0
     lut_bits: int = 9
-    lanes_per_block: int = 16
+    lanes_per_block: int = 8
 
     def _c(self) -> _Cfg:
         c = _Cfg()

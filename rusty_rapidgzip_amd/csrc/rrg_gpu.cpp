@@ -174,7 +174,7 @@ double now_ms() {
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
     G.g.lut_bits = (cfg.lut_bits == 8 || cfg.lut_bits == 10) ? cfg.lut_bits : 9;
-    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 16;
+    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 8;
     if (lanes > 64) lanes = 64;
     while (lanes > 1 && (size_t)lanes * inflate_lds_per_lane(G.g.lut_bits) > 65536) lanes /= 2;
     G.g.lanes_per_block = lanes;
@@ -207,7 +207,8 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
                 uint64_t cells_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
-    HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * 480 * sizeof(uint16_t)));
+    HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * inflate_scratch_u16_per_lane() *
+                           sizeof(uint16_t)));
     HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), c.cells.cap, c.s));
     HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
     HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));
@@ -995,7 +996,7 @@ void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);
     cfg->chunk_size_bytes = 4u << 20;
     cfg->lut_bits = 9;
-    cfg->lanes_per_block = 16;
+    cfg->lanes_per_block = 8;
 }
 
 void* rrg_gpu_create(int device) {

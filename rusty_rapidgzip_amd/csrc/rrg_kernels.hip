@@ -457,13 +457,20 @@ hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const
 // ===========================================================================
 constexpr int DIST_LB = 7;
 
+// Per-lane LDS: literal/length LUT (2^LB u16 entries (sym << 4) | len),
+// distance LUT (128 u8 entries (sym << 3) | len, len <= 7; the precode LUT
+// while a header is read), and four 16-entry u16 arrays (code-length counts of
+// both tables, canonical next code and sorted offset while a table is built).
 template <int LB>
 struct LaneTables {
     static constexpr int kLit = 1 << LB;
-    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) * 2 + 5 * 32;
+    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) + 4 * 32;
 };
-// per-lane global scratch, in u16: 320 length-sorted symbols + 160 (= 320 B of code lengths)
-constexpr int LANE_SCRATCH_U16 = 480;
+// per-lane global scratch, in u16: 320 length-sorted symbols, 320 B of code
+// lengths, 32 B of precode lengths
+constexpr int LANE_SCRATCH_U16 = 496;
+
+int inflate_scratch_u16_per_lane() { return LANE_SCRATCH_U16; }
 
 int inflate_lds_per_lane(int lut_bits) {
     switch (lut_bits) {
@@ -475,9 +482,12 @@ int inflate_lds_per_lane(int lut_bits) {
 
 // Canonical table build (huffman.rs:215-325 semantics, incomplete accepted).
 // Returns false when oversubscribed with >= 2 symbols.
-__device__ bool build_table(const uint8_t* __restrict__ lens, int n, uint16_t* __restrict__ lut, int lbits,
+template <typename E>
+__device__ bool build_table(const uint8_t* __restrict__ lens, int n, E* __restrict__ lut, int lbits,
                             uint16_t* __restrict__ cnt, uint16_t* __restrict__ tnext, uint16_t* __restrict__ toffs,
                             uint16_t* __restrict__ sorted) {
+    // entry: (symbol << 4) | length in u16 tables, (symbol << 3) | length in u8 ones
+    constexpr int kSh = sizeof(E) == 1 ? 3 : 4;
     for (int l = 0; l < 16; ++l) cnt[l] = 0;
     for (int s = 0; s < n; ++s) cnt[lens[s]]++;
     cnt[0] = 0;
@@ -495,7 +505,7 @@ __device__ bool build_table(const uint8_t* __restrict__ lens, int n, uint16_t* _
         off += cnt[l];
     }
     uint32_t* lut32 = reinterpret_cast<uint32_t*>(lut);
-    for (int i = 0; i < (1 << lbits) / 2; ++i) lut32[i] = 0;
+    for (int i = 0; i < (int)(((1u << lbits) * sizeof(E)) / 4); ++i) lut32[i] = 0;
     for (int s = 0; s < n; ++s) {
         const int l = lens[s];
         if (!l) continue;
@@ -505,7 +515,7 @@ __device__ bool build_table(const uint8_t* __restrict__ lens, int n, uint16_t* _
         toffs[l] = (uint16_t)(toffs[l] + 1);
         if (l <= lbits) {
             const uint32_t rev = __brev(c) >> (32 - l);
-            const uint16_t e = (uint16_t)((s << 4) | l);
+            const E e = (E)((s << kSh) | l);
             for (uint32_t idx = rev; idx < (1u << lbits); idx += 1u << l) lut[idx] = e;
         }
     }
@@ -578,17 +588,17 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     const int tid = (int)threadIdx.x;
     uint8_t* my = lds + tid * LaneTables<LB>::kBytes;
     uint16_t* litlut = reinterpret_cast<uint16_t*>(my);
-    uint16_t* distlut = litlut + (1 << LB);
-    uint16_t* cntL = distlut + (1 << DIST_LB);
+    uint8_t* distlut = my + 2 * (1 << LB);
+    uint16_t* cntL = reinterpret_cast<uint16_t*>(distlut + (1 << DIST_LB));
     uint16_t* cntD = cntL + 16;
     uint16_t* tnext = cntD + 16;
     uint16_t* toffs = tnext + 16;
-    uint16_t* cltmp = toffs + 16;  // 16 u16 = 32 B: precode lengths (19 B) while a header is read
     // code lengths and length-sorted symbols live in per-lane global scratch: they
     // are touched once per block header, not per symbol
     uint16_t* sortedL = sorted_scratch + ((size_t)blockIdx.x * blockDim.x + tid) * LANE_SCRATCH_U16;
     uint16_t* sortedD = sortedL + 288;
     uint8_t* lens = reinterpret_cast<uint8_t*>(sortedL + 320);
+    uint8_t* cltmp = reinterpret_cast<uint8_t*>(sortedL + 480);  // precode lengths while a header is read
 
     int state = S_FETCH;
     uint32_t seg = 0;
@@ -707,7 +717,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));
                 RRG_CONSUME(lx);
                 uint32_t de = distlut[bitbuf & ((1u << DIST_LB) - 1)];
-                uint32_t dl = de & 15u, dsym = de >> 4;
+                uint32_t dl = de & 7u, dsym = de >> 3;
                 if (dl == 0) {
                     if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                     if (!decode_long(bitbuf, cntD, sortedD, &dsym, &dl)) RRG_FAIL(ST_INVALID, DET_NO_CODE);
@@ -830,7 +840,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_GETBITS(c2, 4);
                 const int hlit = (int)a + 257, hdist = (int)b2 + 1, hclen = (int)c2 + 4;
                 if (hlit > 286 || hdist > 30) RRG_FAIL(ST_INVALID, DET_HLIT_HDIST);
-                uint8_t* cl = reinterpret_cast<uint8_t*>(cltmp);
+                uint8_t* cl = cltmp;
                 for (int i = 0; i < 19; ++i) cl[i] = 0;
                 for (int i = 0; i < hclen; ++i) {
                     uint32_t x;
@@ -847,13 +857,13 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     if (pos + 10 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                     if (bitcnt < 32) RRG_REFILL();
                     const uint32_t e = distlut[bitbuf & 127u];
-                    const uint32_t l = e & 15u;
+                    const uint32_t l = e & 7u;
                     if (l == 0) {
                         if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                         RRG_FAIL(ST_INVALID, DET_NO_CODE);
                     }
                     RRG_CONSUME(l);
-                    const uint32_t sym = e >> 4;
+                    const uint32_t sym = e >> 3;
                     if (sym <= 15) {
                         last = (uint8_t)sym;
                         lens[i++] = last;

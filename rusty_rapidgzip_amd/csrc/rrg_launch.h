@@ -26,6 +26,7 @@ struct InflateGeom {
     int lut_bits;         // literal/length LUT bits (8..10)
 };
 int inflate_lds_per_lane(int lut_bits);
+int inflate_scratch_u16_per_lane();  // per-lane global scratch (d_sorted), in u16
 int inflate_blocks_per_cu(int lut_bits, int lanes);
 hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
                           const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
