@@ -396,10 +396,11 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
     uint64_t seg_bytes = J.segment_bytes;
     if (!seg_bytes) {
-        // about one segment per lane in flight (a lane decodes one segment in
+        // about 0.8 segments per lane in flight (a lane decodes one segment in
         // roughly the time the launch takes), between 32 and 64 KiB: shorter
-        // segments cost finder work and speculation, longer ones idle lanes
-        seg_bytes = (total / 8) * 10 / std::max<uint64_t>(1, 11 * G.lanes_in_flight);
+        // segments cost finder work and more false starts (each a re-decode
+        // round), longer ones idle lanes
+        seg_bytes = (total / 8) * 5 / std::max<uint64_t>(1, 4 * G.lanes_in_flight);
         seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 32 * 1024), 64 * 1024);
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));

@@ -579,7 +579,8 @@ __device__ int parse_gz_header(const uint64_t* __restrict__ in64, uint64_t len, 
 enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_STORED, S_BLOCK_END, S_EXIT };
 
 template <int LB>
-__global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
+// 4 waves per SIMD (<= 128 VGPRs): the few spills fall in block-header code
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_inflate(const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
                                                 MemberRec* __restrict__ mrec, const uint32_t* __restrict__ work,
                                                 uint32_t n_work, uint32_t* __restrict__ counter,
                                                 uint16_t* __restrict__ cells, uint16_t* __restrict__ sorted_scratch) {
@@ -603,9 +604,11 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
     int state = S_FETCH;
     uint32_t seg = 0;
     uint64_t pos = 0, hint = 0, bitbuf = 0;
-    uint64_t nbits = 0, in_len = 0;  // end of the segment's file in the input buffer
+    uint64_t nbits = 0;  // end of the segment's file in the input buffer (bits)
     uint32_t bitcnt = 0;
-    uint64_t out_base = 0, out_cap = 0, out_n = 0, member_base = 0;
+    // output region of the segment (a region holds < 2^32 cells)
+    uint16_t* o = cells;
+    uint32_t out_cap = 0, out_n = 0, member_base = 0;
     uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
     // counting: the output region is full; decode on without storing so the
     // segment reports the size it needs (one regrow instead of repeated ones)
@@ -656,10 +659,9 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 pos = d.start_bit;
                 hint = d.hint_bit;
                 counting = 0;
-                in_len = d.end_byte;
                 nbits = d.end_byte * 8;
-                out_base = d.cells_base;
-                out_cap = d.cells_cap;
+                o = cells + d.cells_base;
+                out_cap = (uint32_t)d.cells_cap;
                 out_n = 0;
                 member_base = 0;
                 spec = d.flags & SEG_SPEC;
@@ -688,7 +690,6 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
             // is arranged so that a match waits once: the bit-buffer refill
             // loads are issued together with the copy's source loads, and both
             // complete before the copy's stores are issued.
-            uint16_t* const o = cells + out_base;
             for (int it = 0; it < 64; ++it) {
                 // 48 bits cover a whole match: litlen <= 15 + 5 extra, dist <= 15 + 13 extra
                 if (bitcnt < 48) RRG_REFILL();
@@ -731,8 +732,8 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_CONSUME(dx);
                 // copies store whole groups of 8 cells (up to 7 past the match end,
                 // overwritten by what follows): keep them inside the region
-                if (out_n + length + 8 > out_cap) counting = 1;
-                const uint64_t cur = out_n - member_base;
+                if ((uint64_t)out_n + length + 8 > out_cap) counting = 1;
+                const uint32_t cur = out_n - member_base;
                 if ((uint64_t)distance > cur && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
                 if (counting) {
                     out_n += length;
@@ -753,7 +754,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     // period 1, 2 or 4: load the pattern once, then store it --
                     // single cells up to an 8-cell boundary, then whole aligned
                     // 16-byte groups (the last may run up to 7 cells past the end)
-                    const uint64_t src = out_n - distance;
+                    const uint32_t src = out_n - distance;
                     const uint32_t dm = distance - 1;
                     uint32_t c0 = o[src], c1 = o[src + (1 & dm)], c2 = o[src + (2 & dm)], c3 = o[src + (3 & dm)];
                     bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     // o[out_n + t] = o[src + (t mod distance)]: every load reads a
                     // cell written before its group (all of them for distance < 8;
                     // earlier groups for distance >= 8)
-                    const uint64_t src = out_n - distance;
+                    const uint32_t src = out_n - distance;
                     uint32_t jm = 0;
                     for (uint32_t k = 0; k < length; k += 8) {
                         uint16_t v[8];
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 RRG_GETBITS(nln, 16);
                 if ((ln ^ nln) != 0xffffu) RRG_FAIL(ST_INVALID, DET_STORED_NLEN);
                 if (pos + 8ull * ln > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                if (out_n + ln > out_cap) counting = 1;
+                if ((uint64_t)out_n + ln > out_cap) counting = 1;
                 stored_left = ln;
                 state = S_STORED;
             } else if (bt == 1) {
@@ -914,13 +915,13 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
 #pragma unroll
                 for (int i = 0; i < 5; ++i) w[i] = in64[q + i];
                 if (!counting) {
-                    uint16_t* o = cells + out_base + out_n;
+                    uint16_t* ob = o + out_n;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const uint64_t v = funnel_lo(w[i], w[i + 1], sh);
 #pragma unroll
                         for (int b = 0; b < 8; ++b)
-                            if ((uint32_t)(8 * i + b) < n) o[8 * i + b] = (uint16_t)((v >> (8 * b)) & 0xffu);
+                            if ((uint32_t)(8 * i + b) < n) ob[8 * i + b] = (uint16_t)((v >> (8 * b)) & 0xffu);
                     }
                 } else {
                     asm volatile("" : "+v"(w[4]));
@@ -940,7 +941,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 // member end: trailer, then the next member's header (pipeline.rs:1125-1170)
                 pos = (pos + 7) & ~7ull;
                 const uint64_t tb = pos >> 3;
-                if (tb + 8 > in_len) RRG_FAIL(ST_TRUNCATED, DET_NONE);
+                if (tb + 8 > (nbits >> 3)) RRG_FAIL(ST_TRUNCATED, DET_NONE);
                 const uint64_t w = bits64_at(in64, pos);
                 if (n_mb >= mb_cap) RRG_FAIL(ST_MB_OVERFLOW, DET_NONE);
                 MemberRec m;
@@ -952,7 +953,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                 ++n_mb;
                 const uint64_t after = tb + 8;
                 pos = after * 8;
-                if (after >= in_len) {
+                if (after >= (nbits >> 3)) {
                     final_block = 1;
                     state = S_FETCH;
                     goto finish;
@@ -962,7 +963,7 @@ __global__ __launch_bounds__(64) void k_inflate(const uint64_t* __restrict__ in6
                     goto finish;
                 }
                 uint64_t hl = 0;
-                const int hs = parse_gz_header(in64, in_len, after, &hl);
+                const int hs = parse_gz_header(in64, nbits >> 3, after, &hl);
                 if (hs != ST_OK) RRG_FAIL((uint32_t)hs, DET_NONE);
                 pos = (after + hl) * 8;
                 RRG_REFILL();
