@@ -9,6 +9,8 @@
  * Reference interfaces replaced (paths relative to crates/rusty-rapidgzip/src/):
  *   rrg_gpu_read_gz        read_gz(path, Sender<Arc<Vec<u8>>>, Config)   lib.rs:366-496
  *   rrg_gpu_decode         read_gz over an in-memory (mmapped) file       lib.rs:435-484
+ *   rrg_gpu_decode_fd      read_gz over a non-seekable source (pipe,      lib.rs:378-431,
+ *                          stdin): the streaming-parallel chunking        streaming.rs:278-346,389-600
  *   rrg_cfg                Config{num_threads, chunk_size_bytes, verbose} lib.rs:49-100
  *   rrg_stats              DecodeStats                                    lib.rs:102-108
  *   rrg_error              Error / GzipError / DeflateError               lib.rs:110-118, gzip.rs:25-61,
@@ -118,6 +120,16 @@ int rrg_gpu_read_gz(void* ctx, const char* path, const rrg_cfg* cfg, rrg_chunk_c
 /* Same over a file already in host memory. */
 int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user,
                    rrg_stats* stats, rrg_error* err);
+
+/* read_gz over a non-seekable source (pipe, stdin, socket) read to its end:
+ * chunk k starts at the first verified block at or after the start of chunk
+ * k-1 plus chunk_size (the reference's streaming dispatcher), BGZF is not
+ * special-cased and speculation_failures is 0, as on the reference's
+ * streaming path.  rrg_gpu_read_gz takes this path for non-regular and empty
+ * files.  The GPU still verifies every chunk start against its predecessor's
+ * end. */
+int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user, rrg_stats* stats,
+                      rrg_error* err);
 
 /* Device-resident form.  d_file: the gzip file in device memory.  h_file:
  * optional host copy (used only to walk BGZF member headers; NULL = copy the

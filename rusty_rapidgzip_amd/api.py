@@ -136,6 +136,7 @@ EXPORTED_SYMBOLS = (
     "rrg_gpu_read_gz",
     "rrg_gpu_decode",
     "rrg_gpu_decode_device",
+    "rrg_gpu_decode_fd",
     "rrg_gpu_last_profile",
     "rrg_gpu_decode_batch",
     "rrg_server_create",
@@ -166,6 +167,8 @@ def _lib() -> ctypes.CDLL:
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
             ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
         lib.rrg_gpu_last_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(Profile)]
+        lib.rrg_gpu_decode_fd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_Cfg), _CB, ctypes.c_void_p,
+                                          ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
         lib.rrg_gpu_decode_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(_BatchItem), ctypes.c_size_t,
                                              ctypes.POINTER(_Cfg), ctypes.POINTER(ctypes.c_void_p),
                                              ctypes.POINTER(_Err)]
@@ -412,6 +415,11 @@ class Decoder:
         return self._run(
             lambda c, cb, st, er: self._lib.rrg_gpu_decode(self._ctx, ctypes.addressof(buf), len(data), c, cb, None,
                                                            st, er), config, sink)
+
+    def decode_fd(self, fd: int, sink, config: Config | None = None) -> DecodeStats:
+        """read_gz over a non-seekable source (pipe, stdin): the streaming path's chunking."""
+        return self._run(lambda c, cb, st, er: self._lib.rrg_gpu_decode_fd(self._ctx, int(fd), c, cb, None, st, er),
+                         config, sink)
 
     def decode_device(self, d_ptr: int, length: int, host: bytes | None = None, config: Config | None = None,
                       max_chunks: int = 1 << 20):

@@ -127,6 +127,10 @@ struct Job {
     const uint8_t* h = nullptr;  // host copy of the file: headers and BGZF framing are read here
     size_t chunk_size = 4u << 20;
     size_t segment_bytes = 0;
+    // non-seekable input (streaming.rs): chunk boundaries are a chain of finds,
+    // each one chunk past the previous boundary; no BGZF special case
+    bool stream = false;
+    std::vector<uint64_t> stream_B;  // chunk starts (absolute bits)
     // host plan
     struct Mb {
         uint64_t off, size;  // file offsets

@@ -15,6 +15,7 @@
 //   6. in-order output chunks with the reference's chunk boundaries.
 #include "rrg_gpu.h"
 
+#include <errno.h>
 #include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <sys/mman.h>
@@ -355,7 +356,7 @@ void plan_job(Job& J) {
     const uint8_t* h = J.h;
     const uint64_t len = J.len;
     uint64_t rest = 0;
-    if (bgzf_block_size(h, len)) {
+    if (!J.stream && bgzf_block_size(h, len)) {
         J.bgzf = true;
         uint64_t pos = 0;
         while (pos < len) {
@@ -405,6 +406,20 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
     const uint64_t seg_bits = chunk_bits / D;
+    if (J.stream) {
+        // chunk starts already fixed: segment targets inside each chunk
+        for (size_t k = 0; k < J.stream_B.size(); ++k) {
+            const uint64_t lo = J.stream_B[k];
+            const uint64_t hi = k + 1 < J.stream_B.size() ? J.stream_B[k + 1] : J.end_bits();
+            for (uint64_t t = lo + seg_bits; t < hi; t += seg_bits) {
+                targets.push_back(t);
+                limits.push_back(J.end_bits() | kAnyStart);
+                scan_ends.push_back(std::min(hi, t + 2 * seg_bits));
+                is_chunk.push_back(0);
+            }
+        }
+        return;
+    }
     for (uint64_t k = 0; k * chunk_bits < total; ++k) {
         if (k) {
             targets.push_back(base + k * chunk_bits);
@@ -686,6 +701,20 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // 1. host plan
     for (Job& J : jobs) plan_job(J);
     // 2. block finder over every member-path body
+    // streaming chunking (streaming.rs:278-346): target = previous boundary +
+    // chunk_bits, first verified candidate at or after it, until none is left
+    for (Job& J : jobs) {
+        if (!J.stream || J.body == UINT64_MAX) continue;
+        const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
+        J.stream_B.assign(1, J.abs_bits(J.body));
+        for (uint64_t t = J.stream_B.back() + chunk_bits; t < J.end_bits(); t = J.stream_B.back() + chunk_bits) {
+            std::vector<uint64_t> f;
+            const int frc = run_finder(c, {t}, {J.end_bits()}, {J.end_bits()}, f, err);
+            if (frc) return frc;
+            if (f[0] == UINT64_MAX) break;
+            J.stream_B.push_back(f[0]);
+        }
+    }
     std::vector<uint64_t> targets, limits, scan_ends;
     std::vector<uint8_t> is_chunk;
     for (Job& J : jobs) {
@@ -737,6 +766,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (J.body == UINT64_MAX) continue;
         const uint64_t base = J.abs_bits(J.body);
         std::vector<uint64_t> B{base};
+        if (J.stream) B = J.stream_B;
         for (size_t i = J.t_lo; i < J.t_hi; ++i)
             if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
         std::vector<uint64_t> M(B);
@@ -924,17 +954,20 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         std::vector<uint64_t> off_mp(J.off.begin() + nb, J.off.end());
         for (uint64_t& v : off_mp) v -= J.off[nb];
         finish_member_path(J, desc, res, recs, flags, raw, J.r_lo + nb, J.off[nb], off_mp);
+        if (J.stream) dec.spec_failures = 0;  // the streaming path counts none (streaming.rs:389-395)
     }
     return RRG_OK;
 }
 
 // Decode the single file already in c.in (and at host pointer h).
-int decode_loaded(Ctx& c, const uint8_t* h, uint64_t len, const rrg_cfg& cfg, Decoded& dec, rrg_error* err) {
+int decode_loaded(Ctx& c, const uint8_t* h, uint64_t len, const rrg_cfg& cfg, Decoded& dec, rrg_error* err,
+                  bool stream = false) {
     const Geometry G = pick_geometry(c, cfg);
     std::vector<Job> jobs(1);
     jobs[0].in_off = 0;
     jobs[0].len = len;
     jobs[0].h = h;
+    jobs[0].stream = stream;
     jobs[0].chunk_size = cfg.chunk_size_bytes;
     jobs[0].segment_bytes = cfg.segment_bytes;
     const int rc = decode_batch(c, G, jobs, c.out, err);
@@ -1078,8 +1111,8 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
     return RRG_OK;
 }
 
-int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg_in, rrg_chunk_cb cb, void* user,
-                   rrg_stats* stats, rrg_error* err) {
+static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg_in, rrg_chunk_cb cb, void* user,
+                       rrg_stats* stats, rrg_error* err, bool stream) {
     if (err) std::memset(err, 0, sizeof *err);
     if (stats) std::memset(stats, 0, sizeof *stats);
     if (!ctx) {
@@ -1094,13 +1127,15 @@ int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cf
     const double t0 = now_ms();
     begin_profile(c, len);
     if (len == 0) {
-        set_err(err, RRG_E_IO, "empty input");
-        return RRG_E_IO;
+        // the streaming path reads no gzip header (streaming.rs:174-196)
+        if (stream) set_err(err, RRG_E_TRUNCATED, "truncated gzip stream");
+        else set_err(err, RRG_E_IO, "empty input");
+        return err ? err->kind : (stream ? RRG_E_TRUNCATED : RRG_E_IO);
     }
     int rc = load_host(c, file, len, err);
     if (rc) return rc;
     Decoded dec;
-    rc = decode_loaded(c, file, len, cfg, dec, err);
+    rc = decode_loaded(c, file, len, cfg, dec, err, stream);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c.s));
     // in-order emission (pipeline.rs:921-937)
@@ -1142,7 +1177,7 @@ int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cf
     }
     if (closed) {
         // speculative path: Ok with partial stats; BGZF: BrokenPipe (pipeline.rs:1028-1030, 1575-1578)
-        if (bgzf_block_size(file, len)) {
+        if (!stream && bgzf_block_size(file, len)) {
             set_err(err, RRG_E_IO, "sink closed");
             return RRG_E_IO;
         }
@@ -1155,6 +1190,41 @@ int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cf
     return RRG_OK;
 }
 
+
+int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg_in, rrg_chunk_cb cb, void* user,
+                   rrg_stats* stats, rrg_error* err) {
+    return decode_host(ctx, file, len, cfg_in, cb, user, stats, err, false);
+}
+
+// Everything readable from fd, for non-seekable sources.
+static int read_all(int fd, std::vector<uint8_t>& buf, rrg_error* err) {
+    buf.clear();
+    size_t n = 0;
+    for (;;) {
+        if (buf.size() < n + (1u << 20)) buf.resize(std::max<size_t>(2 * buf.size(), n + (1u << 20)));
+        const ssize_t got = read(fd, buf.data() + n, buf.size() - n);
+        if (got < 0) {
+            if (errno == EINTR) continue;
+            set_err(err, RRG_E_IO, "read failed: %s", std::strerror(errno));
+            return RRG_E_IO;
+        }
+        if (got == 0) break;
+        n += (size_t)got;
+    }
+    buf.resize(n);
+    return RRG_OK;
+}
+
+int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user, rrg_stats* stats,
+                      rrg_error* err) {
+    if (err) std::memset(err, 0, sizeof *err);
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    std::vector<uint8_t> buf;
+    const int rc = read_all(fd, buf, err);
+    if (rc) return rc;
+    return decode_host(ctx, buf.data(), buf.size(), cfg, cb, user, stats, err, true);
+}
+
 int rrg_gpu_read_gz(void* ctx, const char* path, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user, rrg_stats* stats,
                     rrg_error* err) {
     if (err) std::memset(err, 0, sizeof *err);
@@ -1165,10 +1235,16 @@ int rrg_gpu_read_gz(void* ctx, const char* path, const rrg_cfg* cfg, rrg_chunk_c
         return RRG_E_IO;
     }
     struct stat st;
-    if (fstat(fd, &st) != 0 || !S_ISREG(st.st_mode) || st.st_size == 0) {
+    if (fstat(fd, &st) != 0) {
         close(fd);
-        set_err(err, RRG_E_IO, "%s: not a non-empty regular file (pipes and empty files are not supported)", path);
+        set_err(err, RRG_E_IO, "cannot stat %s", path);
         return RRG_E_IO;
+    }
+    if (!S_ISREG(st.st_mode) || st.st_size == 0) {
+        // non-seekable or empty: the streaming path (lib.rs:378-431)
+        const int rc = rrg_gpu_decode_fd(ctx, fd, cfg, cb, user, stats, err);
+        close(fd);
+        return rc;
     }
     const size_t len = (size_t)st.st_size;
     void* m = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);

@@ -273,3 +273,78 @@ def test_read_gz_file(decoder, tmp_path):
     assert b"".join(out) == p
     assert st.compressed_bytes == f.stat().st_size
     assert zlib.crc32(b"".join(out)) == zlib.crc32(p)
+
+
+def _stream_expected(data, chunk):
+    """Chunk lengths of the reference's streaming path (streaming.rs:278-346):
+    each chunk starts at the first verified block at or after the previous
+    start + chunk_size, and ends where the decode from its start reaches the
+    next chunk's start.  Every start must be a true one (checked here)."""
+    hl = oracle.parse_header(data)
+    body = data[hl:]
+    nbits = len(body) * 8
+    cb = max(chunk, 65536) * 8
+    B = [0]
+    while B[-1] + cb < nbits:
+        f = oracle.find_next_dynamic_block(body, B[-1] + cb, nbits)
+        if f is None:
+            break
+        B.append(f)
+    lens = []
+    for k in range(len(B)):
+        end = B[k + 1] if k + 1 < len(B) else (1 << 64) - 1
+        r = oracle.decode_chunk(body, B[k], end)
+        assert r["err"] == 0
+        if k + 1 < len(B):
+            assert r["actual_end_bit"] == B[k + 1]  # a true start: the expected lengths are exact
+        lens.append(len(r["cells"]))
+    return lens
+
+
+def test_streaming_pipe(decoder, tmp_path):
+    import threading
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(9 << 20, seed=77)
+    g = synth.gzip(raw, 6, piece=4 << 20).tobytes()
+    chunk = 1 << 20
+    want = _stream_expected(g, chunk)
+    cfg = rrg.Config(chunk_size_bytes=chunk)
+
+    def feed(fd, data):
+        with os.fdopen(fd, "wb") as f:
+            f.write(data)
+
+    rfd, wfd = os.pipe()
+    t = threading.Thread(target=feed, args=(wfd, g))
+    t.start()
+    out = []
+    st = decoder.decode_fd(rfd, out, cfg)
+    t.join()
+    os.close(rfd)
+    assert b"".join(out) == raw.tobytes()
+    assert [len(c) for c in out] == want
+    assert st.chunks_decoded == len(want) and st.speculation_failures == 0
+    assert st.compressed_bytes == len(g)
+    # read_gz on a named pipe takes the same path
+    fifo = tmp_path / "in.fifo"
+    os.mkfifo(fifo)
+
+    def feed_fifo():
+        with open(fifo, "wb") as f:
+            f.write(g)
+
+    t = threading.Thread(target=feed_fifo)
+    t.start()
+    out2 = []
+    rrg.read_gz(fifo, out2, cfg)
+    t.join()
+    assert [len(c) for c in out2] == want
+    # an empty file reads no header: Truncated, as the reference's streaming path
+    empty = tmp_path / "empty.gz"
+    empty.write_bytes(b"")
+    with pytest.raises(rrg.GzipError) as ei:
+        rrg.read_gz(empty, [], cfg)
+    assert ei.value.kind == "Truncated"
