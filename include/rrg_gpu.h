@@ -77,7 +77,10 @@ typedef struct {
 
 typedef struct {
     size_t chunk_size_bytes;  /* Config.chunk_size_bytes, default 4 MiB */
-    int num_threads;          /* Config.num_threads (host threads for framing; 0 = auto) */
+    int num_threads;          /* Config.num_threads.  1 = the reference's single-worker chunking
+                                 (pipeline.rs:481-488, lib.rs:391-408): chunks end at block ends,
+                                 no speculation is counted; any other value (0 = auto) = the
+                                 speculative path's chunking.  The GPU decode is the same. */
     int verbose;              /* Config.verbose: 0 = Off, 1 = On (stage timings on stderr) */
     /* GPU decomposition (no reference counterpart; 0 = automatic) */
     size_t segment_bytes;     /* compressed bytes per lane-decoded segment */

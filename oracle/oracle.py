@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
         L.orc_chunk_free.argtypes = [ctypes.POINTER(Chunk)]
         L.orc_resolve_cells.argtypes = [vp, sz, vp, sz, vp]
         L.orc_read_gz.argtypes = [vp, sz, sz, ctypes.c_int, SINK, vp, ctypes.POINTER(Stats), ctypes.POINTER(Error)]
+        L.orc_read_gz_stream_serial.argtypes = [vp, sz, SINK, vp, ctypes.POINTER(Stats), ctypes.POINTER(Error)]
         L.orc_chunk_boundaries.argtypes = [vp, sz, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), sz]
         L.orc_chunk_boundaries.restype = sz
         _lib = L
@@ -181,4 +182,21 @@ def read_gz(data, chunk_size: int = 4 << 20, threads: int = 4, keep: bool = True
 
     st, er = Stats(), Error()
     rc = lib().orc_read_gz(p, a.size, chunk_size, threads, SINK(cb), None, ctypes.byref(st), ctypes.byref(er))
+    return rc, chunks, st, er
+
+
+def read_gz_stream_serial(data, sink_limit: int | None = None):
+    """The single-worker streaming path for non-seekable inputs (lib.rs:244-361):
+    (rc, chunks, Stats, Error).  sink_limit closes the sink after that many chunks."""
+    a, p = _buf(data)
+    chunks: list[bytes] = []
+
+    def cb(_u, ptr, n):
+        if sink_limit is not None and len(chunks) >= sink_limit:
+            return 1
+        chunks.append(ctypes.string_at(ptr, n))
+        return 0
+
+    st, er = Stats(), Error()
+    rc = lib().orc_read_gz_stream_serial(p, a.size, SINK(cb), None, ctypes.byref(st), ctypes.byref(er))
     return rc, chunks, st, er

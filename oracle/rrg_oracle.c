@@ -28,6 +28,7 @@
 #include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 /* ------------------------------------------------------------------------ */
 /* RFC 1951 tables (deflate/tables.rs:1-49)                                   */
@@ -750,7 +751,7 @@ static int decode_huff_block(BR* b, orc_chunk* c, const Huff* lit, const Huff* d
         uint16_t* o = c->cells;
         if (distance > cur) {
             if (!spec) {
-                *msg = "distance out of bounds";
+                *msg = "back-reference distance out of bounds"; /* fast_inflate.rs:1711-1715 */
                 return ORC_E_INVALID;
             }
             /* fast_inflate.rs:2209-2222 */
@@ -1156,6 +1157,139 @@ static int decode_member_path(const uint8_t* body, size_t len, size_t chunk_size
     return rc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Single-worker paths.                                                        */
+/* ------------------------------------------------------------------------ */
+/* serial_decode_member (pipeline.rs:1211-1345) when streaming == 0: blocks
+ * decode one after another with a real 32 KiB window; after a non-final block,
+ * once chunk_size + 32 KiB bytes are pending, all but the last 32 KiB are sent;
+ * a member's final block sends the rest, before its trailer is checked.
+ * file[body_off..] is the first member's deflate body; trailer offsets in
+ * errors are file offsets (pipeline.rs:1318,1326).
+ *
+ * read_gz_streaming (lib.rs:244-361) when streaming != 0: file[0..] starts at
+ * the first member's header; after every non-final block all but the last
+ * 32 KiB are sent (lib.rs:286-303), an input that ends inside a block or
+ * trailer is Truncated (lib.rs:200-205, 188-190), trailer offsets in errors
+ * are 0 (lib.rs:323-340), an empty input is Truncated (lib.rs:355-357), and a
+ * closed sink does not stop the decode (lib.rs:296-298). */
+static int serial_member_path(const uint8_t* file, size_t len, size_t body_off, size_t chunk_size, int streaming,
+                              orc_sink_cb cb, void* user, orc_stats* st, orc_error* err) {
+    const size_t W = 32768;
+    const size_t T = chunk_size > 65536 ? chunk_size : 65536;
+    orc_chunk c;
+    memset(&c, 0, sizeof c);
+    Huff* tabs = (Huff*)malloc(3 * sizeof(Huff));
+    uint8_t* bytes = NULL;
+    size_t bytes_cap = 0;
+    int rc = ORC_OK, closed = 0;
+    uint32_t member = 0;
+    size_t pos = body_off; /* streaming: next header; else: next body */
+    int need_header = streaming;
+    if (streaming && len == 0) {
+        set_err(err, ORC_E_TRUNCATED, "truncated gzip stream");
+        rc = ORC_E_TRUNCATED;
+        goto out;
+    }
+    for (;;) {
+        if (need_header) {
+            long hl = orc_parse_header(file + pos, len - pos);
+            if (hl < 0) {
+                set_err(err, (int)-hl, "bad gzip header");
+                rc = (int)-hl;
+                goto out;
+            }
+            pos += (size_t)hl;
+        }
+        need_header = 1;
+        BR b = {file, len, (uint64_t)len * 8, (uint64_t)pos * 8};
+        uint32_t crc = 0;
+        uint64_t muc = 0;
+        c.n = 0;
+        for (;;) {
+            int bfinal = 0;
+            const char* msg = NULL;
+            int r = decode_block(&b, &c, tabs, 0, 0, &bfinal, &msg);
+            if (r) {
+                if (streaming && r == ORC_E_EOF) {
+                    set_err(err, ORC_E_TRUNCATED, "truncated gzip stream");
+                    rc = ORC_E_TRUNCATED;
+                } else {
+                    set_err(err, r, msg ? msg : "deflate error");
+                    rc = r;
+                }
+                goto out;
+            }
+            size_t emit = 0; /* cells sent now, from the front */
+            if (bfinal) emit = c.n;
+            else if (streaming ? c.n > W : c.n >= T + W) emit = c.n - W;
+            if (emit) {
+                if (emit > bytes_cap) {
+                    free(bytes);
+                    bytes_cap = emit;
+                    bytes = (uint8_t*)malloc(bytes_cap);
+                }
+                for (size_t k = 0; k < emit; k++) bytes[k] = (uint8_t)c.cells[k];
+                crc = orc_crc32(crc, bytes, emit);
+                muc += emit;
+                if (!closed && cb && cb(user, bytes, emit)) {
+                    if (!streaming) goto out; /* Ok with what was sent (pipeline.rs:1264-1266) */
+                    closed = 1;
+                }
+                st->uncompressed_bytes += emit;
+                st->chunks_decoded++;
+                /* keep the window: the remaining cells move to the front; a
+                 * back-reference can never reach past them (distance <= 32 KiB) */
+                memmove(c.cells, c.cells + emit, (c.n - emit) * sizeof(uint16_t));
+                c.n -= emit;
+            }
+            if (bfinal) break;
+        }
+        b.pos = (b.pos + 7) & ~7ull;
+        size_t tb = (size_t)(b.pos >> 3);
+        if (tb + 8 > len) {
+            set_err(err, ORC_E_TRUNCATED, "truncated gzip stream");
+            rc = ORC_E_TRUNCATED;
+            goto out;
+        }
+        const uint32_t want_crc = (uint32_t)file[tb] | ((uint32_t)file[tb + 1] << 8) |
+                                  ((uint32_t)file[tb + 2] << 16) | ((uint32_t)file[tb + 3] << 24);
+        const uint32_t want_isize = (uint32_t)file[tb + 4] | ((uint32_t)file[tb + 5] << 8) |
+                                    ((uint32_t)file[tb + 6] << 16) | ((uint32_t)file[tb + 7] << 24);
+        if (crc != want_crc) {
+            set_err(err, ORC_E_CRC, "crc mismatch");
+            if (err) {
+                err->expected = want_crc;
+                err->got32 = crc;
+                err->member = member;
+                err->got64 = muc;
+                err->trailer_byte = streaming ? 0 : tb;
+            }
+            rc = ORC_E_CRC;
+            goto out;
+        }
+        if ((uint32_t)muc != want_isize) {
+            set_err(err, ORC_E_ISIZE, "isize mismatch");
+            if (err) {
+                err->expected = want_isize;
+                err->got64 = muc;
+                err->member = member;
+                err->trailer_byte = streaming ? 0 : tb;
+            }
+            rc = ORC_E_ISIZE;
+            goto out;
+        }
+        member++;
+        pos = tb + 8;
+        if (pos >= len) break;
+    }
+out:
+    free(bytes);
+    free(tabs);
+    orc_chunk_free(&c);
+    return rc;
+}
+
 typedef struct {
     const uint8_t* file;
     const size_t* ms;
@@ -1304,9 +1438,19 @@ static int decode_bgzf_path(const uint8_t* file, size_t len, size_t chunk_size, 
             set_err(err, (int)-hl, "bad gzip header");
             return (int)-hl;
         }
-        rc = decode_member_path(file + pos + hl, len - pos - (size_t)hl, chunk_size, nthreads, cb, user, st, err);
+        if (nthreads == 1)
+            rc = serial_member_path(file, len, pos + (size_t)hl, chunk_size, 0, cb, user, st, err);
+        else
+            rc = decode_member_path(file + pos + hl, len - pos - (size_t)hl, chunk_size, nthreads, cb, user, st, err);
     }
     return rc;
+}
+
+/* effective_threads (pipeline.rs:1187-1196): 0 = the machine's parallelism */
+static int effective_threads(int nthreads) {
+    if (nthreads > 0) return nthreads;
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 1 ? (int)n : 1;
 }
 
 /* lib.rs:366-496 */
@@ -1319,12 +1463,24 @@ int orc_read_gz(const uint8_t* file, size_t len, size_t chunk_size, int nthreads
         return ORC_E_IO;
     }
     st->compressed_bytes = len;
-    if (nthreads < 1) nthreads = 1;
+    nthreads = effective_threads(nthreads);
     if (orc_parse_bgzf_block_size(file, len)) return decode_bgzf_path(file, len, chunk_size, nthreads, cb, user, st, err);
     long hl = orc_parse_header(file, len);
     if (hl < 0) {
         set_err(err, (int)-hl, "bad gzip header");
         return (int)-hl;
     }
+    /* a single worker decodes serially (pipeline.rs:481-488) */
+    if (nthreads == 1) return serial_member_path(file, len, (size_t)hl, chunk_size, 0, cb, user, st, err);
     return decode_member_path(file + hl, len - (size_t)hl, chunk_size, nthreads, cb, user, st, err);
+}
+
+/* lib.rs:378-408: a non-seekable (or empty) input with one worker takes the
+ * streaming serial path, which reads every member header itself and knows no
+ * BGZF; compressed_bytes stays 0 (lib.rs:402-407). */
+int orc_read_gz_stream_serial(const uint8_t* file, size_t len, orc_sink_cb cb, void* user, orc_stats* st,
+                              orc_error* err) {
+    memset(st, 0, sizeof *st);
+    if (err) memset(err, 0, sizeof *err);
+    return serial_member_path(file, len, 0, 0, 1, cb, user, st, err);
 }

@@ -98,11 +98,18 @@ int orc_resolve_cells(const uint16_t* cells, size_t n, const uint8_t* tail, size
 
 /* read_gz (lib.rs:366-496 + pipeline.rs:468-1041, 1357-1636) over an in-memory
  * file.  Each in-order output chunk is handed to cb (return nonzero = sink
- * closed).  nthreads >= 1 workers are used for the finder, decode, resolve and
+ * closed).  nthreads workers are used for the finder, decode, resolve and
  * CRC stages. */
 typedef int (*orc_sink_cb)(void* user, const uint8_t* p, size_t n);
 int orc_read_gz(const uint8_t* file, size_t len, size_t chunk_size, int nthreads,
                 orc_sink_cb cb, void* user, orc_stats* st, orc_error* err);
+/* nthreads == 1 (0 = the machine's parallelism, pipeline.rs:1187-1196) takes
+ * the serial path of pipeline.rs:1211-1345 for non-BGZF members. */
+
+/* The streaming serial path for non-seekable inputs with one worker
+ * (lib.rs:244-361, dispatched at lib.rs:383-408). */
+int orc_read_gz_stream_serial(const uint8_t* file, size_t len, orc_sink_cb cb, void* user,
+                              orc_stats* st, orc_error* err);
 
 /* All boundaries the block finder yields for a body (pipeline.rs:491-551),
  * written to out (cap entries, first is 0).  Returns the count. */

@@ -199,7 +199,9 @@ class Verbosity(enum.Enum):
 
 @dataclass
 class Config:
-    """lib.rs:49-100.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition."""
+    """lib.rs:49-100.  ``num_threads == 1`` gives the reference's single-worker chunking (chunks
+    end at block ends, pipeline.rs:1211-1345 / lib.rs:244-361); the GPU decode is the same for
+    any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition."""
 
     num_threads: int = 0
     chunk_size_bytes: int = 4 * 1024 * 1024

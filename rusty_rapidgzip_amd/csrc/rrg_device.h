@@ -16,7 +16,7 @@ struct SegDesc {
     uint64_t cells_base;  // first u16 cell of this segment's output region
     uint64_t cells_cap;   // capacity in cells
     uint64_t end_byte;    // end of the segment's file in the input buffer (bits past it are not read)
-    uint32_t flags;       // SEG_SPEC | SEG_SINGLE_MEMBER
+    uint32_t flags;       // SEG_SPEC | SEG_SINGLE_MEMBER | SEG_BLOCK_ENDS
     uint32_t mb_base;     // first MemberRec slot
     uint32_t mb_cap;      // MemberRec slots
     uint32_t pad;
@@ -25,7 +25,11 @@ struct SegDesc {
 enum : uint32_t {
     SEG_SPEC = 1u,           // first member may reference bytes before the segment (markers)
     SEG_SINGLE_MEMBER = 2u,  // stop after one member's trailer (BGZF members)
+    SEG_BLOCK_ENDS = 4u,     // also record every non-final block end (single-worker chunking)
 };
+
+// MemberRec.trailer_byte of a record that marks a non-final block end
+constexpr uint64_t MB_BLOCK_END = ~0ull;
 
 struct SegResult {
     uint64_t n_cells;   // cells produced
@@ -36,7 +40,8 @@ struct SegResult {
     uint32_t n_mb;      // member boundaries recorded
 };
 
-// Member boundary inside a segment (pipeline.rs:417-427 MemberBoundary).
+// Member boundary inside a segment (pipeline.rs:417-427 MemberBoundary), or,
+// with SEG_BLOCK_ENDS, a non-final block end (trailer_byte == MB_BLOCK_END).
 struct MemberRec {
     uint64_t out_off;       // cell offset in the segment where the member ended
     uint64_t trailer_byte;  // byte offset of the trailer in the body
@@ -82,6 +87,14 @@ struct ResolveTile {
     uint64_t out_off;    // global output offset of the tile's first byte
     uint32_t seg;        // chain-order segment index
     uint32_t n;          // cells in the tile
+};
+
+// A segment whose markers may name bytes of the previous member: `reach`
+// bytes of its own member precede it (single-worker paths).
+struct ReachScan {
+    uint64_t cells_base;  // absolute cell index of the segment's first cell (a multiple of 8)
+    uint32_t len;         // cells
+    uint32_t reach;       // markers with prefix offset >= reach name bytes before the member
 };
 
 // Chain-order segment geometry used by the tail / resolve kernels.

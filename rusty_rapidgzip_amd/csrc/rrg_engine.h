@@ -65,16 +65,17 @@ struct Ctx {
     hipStream_t s = nullptr;
     int n_cu = 0;
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
-        pcrc, err;
+        pcrc, err, reach, first;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
     uint8_t* pin = nullptr;
     size_t pin_cap = 0;
     rrg_profile prof{};
+    bool verbose = false;  // Config.verbose of the current call: per-round lines on stderr
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Ctx() {
         DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
-                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err};
+                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first};
         for (DevBuf* b : all) b->release();
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -131,6 +132,10 @@ struct Job {
     // each one chunk past the previous boundary; no BGZF special case
     bool stream = false;
     std::vector<uint64_t> stream_B;  // chunk starts (absolute bits)
+    // single worker (Config.num_threads == 1): 1 = the serial path of
+    // pipeline.rs:1211-1345, 2 = the streaming serial path of lib.rs:244-361
+    // (non-seekable input).  Chunks end at block ends; no speculation counted.
+    int serial = 0;
     // host plan
     struct Mb {
         uint64_t off, size;  // file offsets
@@ -149,6 +154,7 @@ struct Job {
     std::vector<uint64_t> orig_start;
     std::vector<uint8_t> first_ok;
     size_t n_chain = 0;
+    size_t n_out = 0;  // chain segments laid out: the chain, and on the serial paths its failing segment
     long fatal = -1;
     bool saw_final = false;
     // output

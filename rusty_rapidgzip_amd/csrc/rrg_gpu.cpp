@@ -60,7 +60,7 @@ const char* detail_msg(uint32_t d) {
         case DET_NO_CODE: return "no matching huffman code";
         case DET_LITLEN_RANGE: return "literal/length symbol out of range";
         case DET_DIST_RANGE: return "distance symbol out of range";
-        case DET_DIST_FAR: return "back-reference before the start of the member";
+        case DET_DIST_FAR: return "back-reference distance out of bounds";
         case DET_STORED_NLEN: return "stored block: LEN/NLEN mismatch";
         case DET_RESERVED_BTYPE: return "reserved block type";
         case DET_MARKER_RANGE: return "marker references bytes outside the available prefix window";
@@ -186,6 +186,13 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     return G;
 }
 
+// Member-record slots of a segment spanning `span` input bytes: a few member
+// ends, or, when block ends are recorded, about one block per 4 KiB of input
+// (more are found by regrowing).
+uint32_t mb_slots(uint64_t span, bool block_ends) {
+    return block_ends ? (uint32_t)std::min<uint64_t>(8 + span / 4096, 1u << 20) : 4u;
+}
+
 // Give a segment a larger output region (cells or member records) after it
 // overflowed; the old region is abandoned.  A cell overflow reports the cells
 // the whole segment needs (the kernel counts on without storing).
@@ -219,6 +226,7 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     HIPCHK(hipMemcpyAsync(c.segs.p, desc.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
+    const double before = c.prof.ms_inflate;
     {
         StageTimer t(c, &c.prof.ms_inflate);
         HIPCHK(launch_inflate(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(),
@@ -227,6 +235,9 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
         t.stop();
     }
     c.prof.n_inflate_launches++;
+    if (c.verbose)
+        std::fprintf(stderr, "[rrg-gpu] inflate launch %llu: %zu segments in %.3f ms\n",
+                     (unsigned long long)c.prof.n_inflate_launches, work.size(), c.prof.ms_inflate - before);
     res.resize(S);
     HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
@@ -405,8 +416,8 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
         seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 32 * 1024), 64 * 1024);
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
-    const uint64_t seg_bits = chunk_bits / D;
-    if (J.stream) {
+    const uint64_t seg_bits = J.serial ? seg_bytes * 8 : chunk_bits / D;
+    if (J.stream || J.serial) {
         // chunk starts already fixed: segment targets inside each chunk
         for (size_t k = 0; k < J.stream_B.size(); ++k) {
             const uint64_t lo = J.stream_B[k];
@@ -447,6 +458,7 @@ struct Redo {
     uint64_t start;
     uint32_t status;  // ST_OK: false start; ST_OVERFLOW / ST_MB_OVERFLOW: region too small
     uint64_t needed;  // cells the segment needs (ST_OVERFLOW)
+    uint64_t end_bit = UINT64_MAX;  // where its decode ends, when known (ST_OVERFLOW at a true start)
 };
 
 // Anchored verification chain of one job (pipeline.rs:774-809): accept a
@@ -478,7 +490,15 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
             break;
         }
         redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK, r.n_cells});
-        if (k + 1 < n) expected = desc[J.order[k + 1]].start_bit;
+        if (desc[j].start_bit == expected && r.status == ST_OVERFLOW) {
+            // out of cells, the lane decoded on without storing: where it ends is
+            // known, so the next segment is checked in this same round
+            redo.back().end_bit = r.end_bit;
+            if (r.final_block) break;
+            expected = r.end_bit;
+        } else if (k + 1 < n) {
+            expected = desc[J.order[k + 1]].start_bit;
+        }
     }
 }
 
@@ -487,7 +507,8 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
 // offset of the part; `off[k]` the part-relative offset of chain segment k.
 void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res,
                         const std::vector<MemberRec>& recs, const std::vector<uint32_t>& flags,
-                        const std::vector<uint32_t>& raw, size_t r_lo, uint64_t base, const std::vector<uint64_t>& off) {
+                        const std::vector<uint32_t>& raw, size_t r_lo, uint64_t base, const std::vector<uint64_t>& off,
+                        const std::vector<uint64_t>& bad) {
     const size_t n_chain = J.n_chain;
     const size_t S = J.order.size();
     Decoded& dec = J.dec;
@@ -512,6 +533,16 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
         fail_chunk = chunk_of[J.fatal];
         status_err(&ferr, res[J.order[J.fatal]].status, res[J.order[J.fatal]].detail);
     }
+    // a back-reference before its member's start fails its chunk's decode,
+    // ahead of any later error in that chunk
+    for (size_t k = 0; k < n_chain; ++k)
+        if (bad[k] != UINT64_MAX) {
+            if (chunk_of[k] <= fail_chunk) {
+                fail_chunk = chunk_of[k];
+                set_err(&ferr, RRG_E_INVALID, "invalid deflate stream: %s", detail_msg(DET_DIST_FAR));
+            }
+            break;
+        }
     for (size_t k = 0; k < n_chain; ++k)
         if (flags[J.c_lo + (J.bg_hi - J.bg_lo) + k]) {
             if (chunk_of[k] < fail_chunk) {
@@ -584,6 +615,154 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     }
 }
 
+// Errors, CRC/ISIZE checks and chunk emission of a single-worker job: the
+// serial path (pipeline.rs:1211-1345) or, on non-seekable input, the streaming
+// serial path (lib.rs:244-361).  Chunks end at block ends: after a non-final
+// block all but the member's last 32 KiB are sent once more than that is
+// pending (the serial path waits for chunk size + 32 KiB), and a member's
+// final block sends the rest before its trailer is checked.  An error stops
+// the stream after the chunks sent before the failing block.  `bad[k]` is the
+// first cell of chain segment k produced by a back-reference before its
+// member's start (UINT64_MAX: none).
+void finish_serial(Job& J, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res,
+                   const std::vector<MemberRec>& recs, const std::vector<uint32_t>& raw, size_t r_lo, uint64_t base,
+                   const std::vector<uint64_t>& off, const std::vector<uint64_t>& bad) {
+    constexpr uint64_t W = 32768;
+    const bool streaming = J.serial == 2;
+    const uint64_t T = std::max<uint64_t>(J.chunk_size, 65536);
+    Decoded& dec = J.dec;
+    dec.spec_failures = 0;
+    uint64_t sent = 0, mstart = 0;  // part-relative: end of the output sent, start of the current member
+    size_t mi = 0, ri = r_lo;
+    rrg_error ferr{};
+    bool failed = false;
+    auto emit = [&](uint64_t e) {
+        dec.chunk_ends.push_back(base + e);
+        sent = e;
+    };
+    for (size_t k = 0; k < J.n_out && !failed; ++k) {
+        const uint32_t j = J.order[k];
+        const SegResult& r = res[j];
+        const uint64_t bad_at = bad[k] == UINT64_MAX ? UINT64_MAX : off[k] + bad[k];
+        for (uint32_t q = 0; q < r.n_mb && !failed; ++q) {
+            const MemberRec& m = recs[desc[j].mb_base + q];
+            const uint64_t e = off[k] + m.out_off;
+            if (e > bad_at) break;  // this block holds the failing back-reference
+            if (m.trailer_byte == MB_BLOCK_END) {
+                if (streaming ? e > sent + W : e >= sent + T + W) emit(e - W);
+                continue;
+            }
+            if (e > sent) emit(e);
+            const uint64_t len = e - mstart;
+            const uint32_t got = std_crc(raw[ri++], len);
+            // file offset of the trailer (pipeline.rs:1318); the streaming path reports 0 (lib.rs:329,338)
+            const uint64_t tb = streaming ? 0 : m.trailer_byte - J.in_off;
+            if (got != m.crc) {
+                set_err(&ferr, RRG_E_CRC,
+                        "CRC32 mismatch in member #%zu (uncompressed bytes: %llu, trailer at byte %llu): expected "
+                        "%#010x, got %#010x",
+                        mi, (unsigned long long)len, (unsigned long long)tb, m.crc, got);
+                ferr.expected = m.crc;
+                ferr.got = got;
+                ferr.got64 = len;
+                ferr.member = (uint32_t)mi;
+                ferr.trailer_byte = tb;
+                failed = true;
+            } else if ((uint32_t)len != m.isize) {
+                set_err(&ferr, RRG_E_ISIZE, "ISIZE mismatch in member #%zu (trailer at byte %llu): expected %u, got %llu",
+                        mi, (unsigned long long)tb, m.isize, (unsigned long long)len);
+                ferr.expected = m.isize;
+                ferr.got64 = len;
+                ferr.member = (uint32_t)mi;
+                ferr.trailer_byte = tb;
+                failed = true;
+            }
+            mstart = e;
+            ++mi;
+        }
+        if (failed) break;
+        if (bad_at != UINT64_MAX) {
+            set_err(&ferr, RRG_E_INVALID, "invalid deflate stream: %s", detail_msg(DET_DIST_FAR));
+            failed = true;
+        } else if ((long)k == J.fatal) {
+            const uint32_t st = r.status;
+            if (st == ST_TRUNCATED) {
+                // the member's final block was sent before its trailer was found cut off
+                const uint64_t e = off[k] + r.n_cells;
+                if (e > sent) emit(e);
+            }
+            // the streaming path reads a block again after more input, and at
+            // the end of the input reports Truncated (lib.rs:200-205)
+            if (streaming && (st == ST_EOF || st == ST_TRUNCATED)) set_err(&ferr, RRG_E_TRUNCATED, "truncated gzip stream");
+            else status_err(&ferr, st, r.detail);
+            failed = true;
+        }
+    }
+    if (!failed && !J.saw_final) {
+        set_err(&ferr, RRG_E_IO, "decode produced no final block");
+        failed = true;
+    }
+    if (failed) {
+        dec.failed = true;
+        dec.error = ferr;
+    }
+}
+
+// The first cell of each laid-out chain segment that a back-reference before
+// the start of its member produced, where that is an error: on the serial
+// paths always (fast_inflate.rs:1704-1715, inflate.rs:339-346); on the
+// speculative path when the member began inside the current chunk, since the
+// reference decodes such a member on its own (pipeline.rs:153-254) and only a
+// chunk's first member may reach before the chunk.  A segment that starts
+// inside a member decodes such a back-reference into markers; only segments
+// that start less than 32 KiB after their member's start can hold one.
+int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc>& desc,
+                 const std::vector<SegResult>& res, const std::vector<MemberRec>& recs,
+                 std::vector<std::vector<uint64_t>>& bad, rrg_error* err) {
+    constexpr uint64_t W = 32768;
+    std::vector<ReachScan> items;
+    std::vector<std::pair<size_t, size_t>> where;  // (job, chain position)
+    for (size_t ji = 0; ji < jobs.size(); ++ji) {
+        const Job& J = jobs[ji];
+        if (J.body == UINT64_MAX) continue;
+        bad[ji].assign(J.n_out, UINT64_MAX);
+        const size_t nb = J.bg_hi - J.bg_lo;
+        uint64_t mstart = J.off[nb];  // job-relative start of the current member
+        uint64_t cstart = J.off[nb];  // job-relative start of the current chunk
+        for (size_t k = 0; k < J.n_out; ++k) {
+            const uint32_t j = J.order[k];
+            const uint64_t o = J.off[nb + k];
+            bool check = J.serial != 0;
+            if (!J.serial) {
+                if (J.seg_is_chunk[k]) cstart = o;
+                check = !J.seg_is_chunk[k] && mstart > cstart;  // not the chunk's first member
+            }
+            if (check && (desc[j].flags & SEG_SPEC) && o - mstart < W && res[j].n_cells) {
+                items.push_back(ReachScan{desc[j].cells_base, (uint32_t)res[j].n_cells, (uint32_t)(o - mstart)});
+                where.push_back({ji, k});
+            }
+            for (uint32_t q = 0; q < res[j].n_mb; ++q) {
+                const MemberRec& m = recs[desc[j].mb_base + q];
+                if (m.trailer_byte != MB_BLOCK_END) mstart = o + m.out_off;
+            }
+        }
+    }
+    if (items.empty()) return RRG_OK;
+    const size_t n = items.size();
+    HIPCHK(c.reach.ensure(n * sizeof(ReachScan)));
+    HIPCHK(c.first.ensure(n * sizeof(uint32_t)));
+    std::vector<uint32_t> first(n, ~0u);
+    HIPCHK(hipMemcpyAsync(c.reach.p, items.data(), n * sizeof(ReachScan), hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.first.p, first.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c.s));
+    HIPCHK(launch_marker_reach(c.reach.as<ReachScan>(), (uint32_t)n, c.cells.as<uint16_t>(), c.first.as<uint32_t>(),
+                               c.s));
+    HIPCHK(hipMemcpyAsync(first.data(), c.first.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    for (size_t i = 0; i < n; ++i)
+        if (first[i] != ~0u) bad[where[i].first][where[i].second] = first[i];
+    return RRG_OK;
+}
+
 // One finder launch; every find must lie in [target, scan end) of its file.
 int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
                const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err) {
@@ -621,8 +800,8 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
 int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vector<SegDesc>& desc,
                uint64_t& cells_used, uint64_t& mb_used, std::vector<uint32_t>& again, rrg_error* err) {
     if (redo.empty()) return RRG_OK;
-    constexpr uint64_t kPieceBits = 8 * 8192;  // at least 8 KiB of input per piece
-    constexpr uint64_t kMaxPieces = 8;
+    constexpr uint64_t kPieceBits = 8 * 2048;  // at least 2 KiB of input per piece
+    constexpr uint64_t kMaxPieces = 32;
     std::vector<uint64_t> targets, limits, scan_ends;
     std::vector<size_t> t_first(redo.size() + 1);
     for (size_t i = 0; i < redo.size(); ++i) {
@@ -656,6 +835,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         SegDesc& d = desc[j];
         const uint64_t hint = d.hint_bit;
         const uint64_t end = hint == UINT64_MAX ? J.end_bits() : hint;
+        const uint64_t true_end = R.end_bit != UINT64_MAX ? std::max(R.end_bit, end) : end;
         if (overflowed(R.status)) {
             grow_segment(d, R.status, cells_used, mb_used, R.needed);
             ++c.prof.n_grown;
@@ -674,14 +854,15 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
             p.start_bit = cuts[q];
             p.hint_bit = q + 1 < cuts.size() ? cuts[q + 1] : hint;
             p.end_byte = d.end_byte;
-            const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : end) - cuts[q]) / 8;
+            // the last piece decodes on to where the whole segment ended
+            const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : true_end) - cuts[q]) / 8;
             p.cells_base = cells_used;
             p.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
             cells_used += p.cells_cap;
-            p.flags = SEG_SPEC;
+            p.flags = SEG_SPEC | (d.flags & SEG_BLOCK_ENDS);
             p.mb_base = (uint32_t)mb_used;
-            p.mb_cap = 4;
-            mb_used += 4;
+            p.mb_cap = mb_slots(span, p.flags & SEG_BLOCK_ENDS);
+            mb_used += p.mb_cap;
             ins.push_back((uint32_t)desc.size());
             again.push_back((uint32_t)desc.size());
             desc.push_back(p);
@@ -704,7 +885,13 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // streaming chunking (streaming.rs:278-346): target = previous boundary +
     // chunk_bits, first verified candidate at or after it, until none is left
     for (Job& J : jobs) {
-        if (!J.stream || J.body == UINT64_MAX) continue;
+        if (J.body == UINT64_MAX) continue;
+        if (J.serial) {
+            // one chunk: the single-worker paths cut chunks at block ends instead
+            J.stream_B.assign(1, J.abs_bits(J.body));
+            continue;
+        }
+        if (!J.stream) continue;
         const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
         J.stream_B.assign(1, J.abs_bits(J.body));
         for (uint64_t t = J.stream_B.back() + chunk_bits; t < J.end_bits(); t = J.stream_B.back() + chunk_bits) {
@@ -766,7 +953,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (J.body == UINT64_MAX) continue;
         const uint64_t base = J.abs_bits(J.body);
         std::vector<uint64_t> B{base};
-        if (J.stream) B = J.stream_B;
+        if (J.stream || J.serial) B = J.stream_B;
         for (size_t i = J.t_lo; i < J.t_hi; ++i)
             if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
         std::vector<uint64_t> M(B);
@@ -790,10 +977,10 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             d.cells_base = cells_used;
             d.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
             cells_used += d.cells_cap;
-            d.flags = j ? SEG_SPEC : 0u;
+            d.flags = (j ? SEG_SPEC : 0u) | (J.serial ? SEG_BLOCK_ENDS : 0u);
             d.mb_base = (uint32_t)mb_used;
-            d.mb_cap = 4;
-            mb_used += 4;
+            d.mb_cap = mb_slots(span, J.serial);
+            mb_used += d.mb_cap;
             desc.push_back(d);
         }
         J.mp_hi = desc.size();
@@ -836,6 +1023,12 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             set_err(err, RRG_E_DEVICE, "verification chain did not settle");
             return RRG_E_DEVICE;
         }
+        if (c.verbose) {
+            size_t n_false = 0, n_over = 0;
+            for (const Redo& R : redo) (R.status == ST_OK ? n_false : n_over) += 1;
+            std::fprintf(stderr, "[rrg-gpu] settle round %d: %zu false starts, %zu out of room, %zu regrown alone\n",
+                         round, n_false, n_over, again.size());
+        }
         rc = split_redo(c, jobs, redo, desc, cells_used, mb_used, again, err);
         if (rc) return rc;
         c.prof.n_redecoded += again.size();
@@ -843,6 +1036,9 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (rc) return rc;
     }
     c.prof.n_segments += desc.size();
+    // the serial paths also send output of the failing segment (the blocks
+    // before the failure, or a member whose trailer is cut off)
+    for (Job& J : jobs) J.n_out = J.n_chain + (J.serial && J.fatal >= 0 ? 1 : 0);
     // 6. output layout (each file at a 256-byte aligned offset) and the chain
     std::vector<ChainSeg> cs;
     std::vector<uint64_t> cs_out;
@@ -858,7 +1054,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             cs_out.push_back(out_total + o);
             o += res[j].n_cells;
         }
-        for (size_t k = 0; k < J.n_chain; ++k) {
+        for (size_t k = 0; k < J.n_out; ++k) {
             const size_t j = J.order[k];
             J.off.push_back(o);
             cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
@@ -871,15 +1067,20 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         out_total = (out_total + o + 255) & ~255ull;
     }
     HIPCHK(out.ensure(out_total + 64));
-    std::vector<uint32_t> flags;
-    rc = resolve_chain(c, cs, cs_out, out, flags, err);
-    if (rc) return rc;
-    // 7. CRC of every member of every file (BGZF members, then member-path members)
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
         HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
         HIPCHK(hipStreamSynchronize(c.s));
     }
+    // back-references before their member's start (read before the tail pass
+    // rewrites markers)
+    std::vector<std::vector<uint64_t>> bad(jobs.size());
+    rc = member_reach(c, jobs, desc, res, recs, bad, err);
+    if (rc) return rc;
+    std::vector<uint32_t> flags;
+    rc = resolve_chain(c, cs, cs_out, out, flags, err);
+    if (rc) return rc;
+    // 7. CRC of every member of every file (BGZF members, then member-path members)
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     for (Job& J : jobs) {
         J.r_lo = ranges.size();
@@ -887,9 +1088,10 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         const size_t nb = J.bg_hi - J.bg_lo;
         for (size_t k = 0; k < nb; ++k) ranges.push_back({ob + J.off[k], ob + J.off[k + 1]});
         uint64_t mstart = J.off[nb];
-        for (size_t k = 0; k < J.n_chain; ++k) {
+        for (size_t k = 0; k < J.n_out; ++k) {
             const size_t j = J.order[k];
             for (uint32_t q = 0; q < res[j].n_mb; ++q) {
+                if (recs[desc[j].mb_base + q].trailer_byte == MB_BLOCK_END) continue;
                 const uint64_t e = J.off[nb + k] + recs[desc[j].mb_base + q].out_off;
                 ranges.push_back({ob + mstart, ob + e});
                 mstart = e;
@@ -953,7 +1155,11 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (J.body == UINT64_MAX) continue;
         std::vector<uint64_t> off_mp(J.off.begin() + nb, J.off.end());
         for (uint64_t& v : off_mp) v -= J.off[nb];
-        finish_member_path(J, desc, res, recs, flags, raw, J.r_lo + nb, J.off[nb], off_mp);
+        if (J.serial) {
+            finish_serial(J, desc, res, recs, raw, J.r_lo + nb, J.off[nb], off_mp, bad[&J - jobs.data()]);
+            continue;
+        }
+        finish_member_path(J, desc, res, recs, flags, raw, J.r_lo + nb, J.off[nb], off_mp, bad[&J - jobs.data()]);
         if (J.stream) dec.spec_failures = 0;  // the streaming path counts none (streaming.rs:389-395)
     }
     return RRG_OK;
@@ -968,6 +1174,7 @@ int decode_loaded(Ctx& c, const uint8_t* h, uint64_t len, const rrg_cfg& cfg, De
     jobs[0].len = len;
     jobs[0].h = h;
     jobs[0].stream = stream;
+    jobs[0].serial = cfg.num_threads == 1 ? (stream ? 2 : 1) : 0;
     jobs[0].chunk_size = cfg.chunk_size_bytes;
     jobs[0].segment_bytes = cfg.segment_bytes;
     const int rc = decode_batch(c, G, jobs, c.out, err);
@@ -986,6 +1193,7 @@ int load_host(Ctx& c, const uint8_t* file, size_t len, rrg_error* err) {
 
 void begin_profile(Ctx& c, uint64_t len) {
     std::memset(&c.prof, 0, sizeof c.prof);
+    c.verbose = false;
     c.prof.compressed_bytes = len;
 }
 
@@ -1069,6 +1277,7 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
     else rrg_cfg_default(&cfg);
     const double t0 = now_ms();
     begin_profile(c, len);
+    c.verbose = cfg.verbose != 0;
     if (len == 0) {
         set_err(err, RRG_E_IO, "empty input");
         return RRG_E_IO;
@@ -1126,6 +1335,7 @@ static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg
     else rrg_cfg_default(&cfg);
     const double t0 = now_ms();
     begin_profile(c, len);
+    c.verbose = cfg.verbose != 0;
     if (len == 0) {
         // the streaming path reads no gzip header (streaming.rs:174-196)
         if (stream) set_err(err, RRG_E_TRUNCATED, "truncated gzip stream");
@@ -1152,30 +1362,41 @@ static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg
         HIPCHK(hipHostMalloc((void**)&c.pin, biggest + 64, hipHostMallocDefault));
         c.pin_cap = biggest + 64;
     }
+    // a closed sink: the speculative path returns Ok counting the refused
+    // chunk's bytes (pipeline.rs:1028-1030), the serial path without them
+    // (pipeline.rs:1264-1266); the streaming serial path ignores it and
+    // decodes on, counting every chunk (lib.rs:296-298)
+    const bool serial = cfg.num_threads == 1;
+    const bool stream_serial = stream && serial;
     prev = 0;
     bool closed = false;
     for (uint64_t e : dec.chunk_ends) {
         const uint64_t n = e - prev;
-        if (n) HIPCHK(hipMemcpyAsync(c.pin, c.out.as<uint8_t>() + prev, n, hipMemcpyDeviceToHost, c.s));
-        HIPCHK(hipStreamSynchronize(c.s));
-        total += n;
-        prev = e;
-        if (cb && cb(user, c.pin, n)) {
-            closed = true;
-            break;
+        if (!closed) {
+            if (n) HIPCHK(hipMemcpyAsync(c.pin, c.out.as<uint8_t>() + prev, n, hipMemcpyDeviceToHost, c.s));
+            HIPCHK(hipStreamSynchronize(c.s));
         }
+        prev = e;
+        if (!closed && cb && cb(user, c.pin, n)) {
+            closed = true;
+            if (!stream_serial) {
+                if (!serial) total += n;
+                break;
+            }
+        }
+        total += n;
         ++sent;
     }
     c.prof.uncompressed_bytes = total;
     c.prof.ms_total = now_ms() - t0;
     if (cfg.verbose) print_profile(c);
     if (stats) {
-        stats->compressed_bytes = len;
+        stats->compressed_bytes = stream_serial ? 0 : len;  // lib.rs:402-407
         stats->uncompressed_bytes = total;
         stats->chunks_decoded = sent;
         stats->speculation_failures = dec.spec_failures;
     }
-    if (closed) {
+    if (closed && !stream_serial) {
         // speculative path: Ok with partial stats; BGZF: BrokenPipe (pipeline.rs:1028-1030, 1575-1578)
         if (!stream && bgzf_block_size(file, len)) {
             set_err(err, RRG_E_IO, "sink closed");

@@ -609,7 +609,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // output region of the segment (a region holds < 2^32 cells)
     uint16_t* o = cells;
     uint32_t out_cap = 0, out_n = 0, member_base = 0;
-    uint32_t spec = 0, single = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
+    uint32_t spec = 0, mflags = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
     // counting: the output region is full; decode on without storing so the
     // segment reports the size it needs (one regrow instead of repeated ones)
     uint32_t counting = 0;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 out_n = 0;
                 member_base = 0;
                 spec = d.flags & SEG_SPEC;
-                single = d.flags & SEG_SINGLE_MEMBER;
+                mflags = d.flags & (SEG_SINGLE_MEMBER | SEG_BLOCK_ENDS);
                 mb_base = d.mb_base;
                 mb_cap = d.mb_cap;
                 n_mb = 0;
@@ -936,6 +936,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         } else if (state == S_BLOCK_END) {
             if (!bfinal) {
+                if (mflags & SEG_BLOCK_ENDS) {
+                    // the single-worker paths cut chunks at block ends (pipeline.rs:1254-1272)
+                    if (n_mb >= mb_cap) RRG_FAIL(ST_MB_OVERFLOW, DET_NONE);
+                    MemberRec m;
+                    m.out_off = out_n;
+                    m.trailer_byte = MB_BLOCK_END;
+                    m.crc = 0;
+                    m.isize = 0;
+                    mrec[mb_base + n_mb] = m;
+                    ++n_mb;
+                }
                 state = S_BLOCK_START;
             } else {
                 // member end: trailer, then the next member's header (pipeline.rs:1125-1170)
@@ -958,7 +969,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     state = S_FETCH;
                     goto finish;
                 }
-                if (single) {
+                if (mflags & SEG_SINGLE_MEMBER) {
                     state = S_FETCH;
                     goto finish;
                 }
@@ -974,7 +985,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         continue;
     finish : {
-        if (counting && status == ST_OK) status = ST_OVERFLOW;  // n_cells = the cells it needs
+        // n_cells = the cells it needs (to the failure, when block ends are
+        // recorded: the cells before a failure are part of that output)
+        if (counting && (status == ST_OK || (mflags & SEG_BLOCK_ENDS))) status = ST_OVERFLOW;
         SegResult r;
         r.n_cells = out_n;
         r.end_bit = pos;
@@ -1445,6 +1458,43 @@ __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ 
         for (uint64_t b = tail; b < end; ++b) c = (c >> 8) ^ T[0][(c ^ out[b]) & 0xffu];
         crc[blockIdx.x] = c;
     }
+}
+
+// ===========================================================================
+// Member reach check of the single-worker paths.  There a back-reference
+// before the start of its member is an error (fast_inflate.rs:1704-1715,
+// inflate.rs:339-346), but a segment that starts inside a member turns it
+// into a marker like any other.  The first cell of a listed segment whose
+// marker offset is at or beyond the segment's reach is that back-reference's
+// first output byte.  One workgroup per segment, 8 cells per thread step.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_marker_reach(const ReachScan* __restrict__ items, uint32_t n,
+                                                      const uint16_t* __restrict__ cells, uint32_t* __restrict__ first) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const ReachScan it = items[i];
+        const uint4* c128 = reinterpret_cast<const uint4*>(cells + it.cells_base);
+        const uint32_t groups = (it.len + 7u) >> 3;
+        uint32_t best = ~0u;
+        for (uint32_t g = threadIdx.x; g < groups; g += 256u) {
+            const uint4 v = c128[g];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t c = (w[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                const uint32_t idx = 8u * g + (uint32_t)k;
+                if (idx < it.len && (c & 0x8000u) && (c & 0x7fffu) >= it.reach && idx < best) best = idx;
+            }
+            if (best != ~0u) break;  // later groups of this thread only hold later cells
+        }
+        if (best != ~0u) atomicMin(first + i, best);
+    }
+}
+
+hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint16_t* d_cells, uint32_t* d_first,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_marker_reach, dim3(n < 4096u ? n : 4096u), dim3(256), 0, s, d_items, n, d_cells, d_first);
+    return hipGetLastError();
 }
 
 hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,

@@ -44,6 +44,11 @@ hipError_t launch_tail(const ChainSeg* d_cs, uint32_t n_seg, uint32_t G, uint16_
 hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const ChainSeg* d_cs,
                           const uint16_t* d_cells, uint8_t* d_out, uint32_t* d_err, hipStream_t s);
 
+// first[i] = the first cell of items[i] holding a marker at or beyond its
+// reach (unchanged when there is none; the caller fills first[] with ~0u).
+hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint16_t* d_cells, uint32_t* d_first,
+                               hipStream_t s);
+
 // Raw (init 0, no final xor) CRC-32 of each piece [off, off+len), len <= 256 KiB.
 hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
                              uint32_t n_pieces, uint32_t* d_crc, hipStream_t s);

@@ -109,6 +109,7 @@ int run_batch(Ctx& c, const rrg_cfg& cfg, const std::vector<Src>& src, const std
         J.h = src[k].h ? src[k].h : hcopy[k].data();
         J.chunk_size = cfg.chunk_size_bytes;
         J.segment_bytes = cfg.segment_bytes;
+        J.serial = cfg.num_threads == 1 ? 1 : 0;
         job_of[k] = jobs.size();
         jobs.push_back(std::move(J));
     }

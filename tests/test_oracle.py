@@ -239,6 +239,113 @@ def test_bgzf_path_and_ramp():
     assert zlib.decompress(bg[:oracle.bgzf_block_size(bg)], 31) == raw.tobytes()[:65280]
 
 
+@pytest.mark.parametrize("chunk", [65536, 100_000, 1 << 20])
+def test_serial_paths_cut_at_known_block_ends(chunk):
+    """Single-worker chunking (pipeline.rs:1254-1287, lib.rs:286-303) on members
+    whose block ends are known from how zlib wrote them."""
+    import crafted
+
+    data, payload, members = crafted.synced_file(5)
+    rc, chunks, st, er = oracle.read_gz(data, chunk_size=chunk, threads=1)
+    assert rc == 0, er.msg
+    assert b"".join(chunks) == payload
+    assert [len(c) for c in chunks] == crafted.serial_chunks(members, chunk, streaming=False)
+    assert (st.chunks_decoded, st.uncompressed_bytes, st.speculation_failures) == (len(chunks), len(payload), 0)
+    assert st.compressed_bytes == len(data)
+    rc, chunks, st, er = oracle.read_gz_stream_serial(data)
+    assert rc == 0, er.msg
+    assert b"".join(chunks) == payload
+    assert [len(c) for c in chunks] == crafted.serial_chunks(members, 0, streaming=True)
+    assert (st.chunks_decoded, st.uncompressed_bytes, st.compressed_bytes) == (len(chunks), len(payload), 0)
+
+
+def test_serial_paths_errors_and_sink():
+    import crafted
+
+    data, payload, members = crafted.synced_file(6, n_members=1)
+    n = len(data)
+    # CRC: every byte is sent before the trailer is checked; the serial path
+    # names the trailer's file offset, the streaming path 0 (pipeline.rs:1318, lib.rs:329)
+    bad = bytearray(data)
+    bad[-6] ^= 1
+    rc, chunks, _, er = oracle.read_gz(bytes(bad), chunk_size=65536, threads=1)
+    assert rc == oracle.ORC_E_CRC and b"".join(chunks) == payload
+    assert (er.trailer_byte, er.member, er.got64) == (n - 8, 0, len(payload))
+    rc, chunks, _, er = oracle.read_gz_stream_serial(bytes(bad))
+    assert rc == oracle.ORC_E_CRC and b"".join(chunks) == payload and er.trailer_byte == 0
+    bad = bytearray(data)
+    bad[-1] ^= 1
+    assert oracle.read_gz(bytes(bad), chunk_size=65536, threads=1)[0] == oracle.ORC_E_ISIZE
+    # cut inside the deflate body: UnexpectedEof; the streaming path says Truncated
+    rc, chunks, _, _ = oracle.read_gz(data[: n // 2], chunk_size=65536, threads=1)
+    assert rc == oracle.ORC_E_EOF and payload.startswith(b"".join(chunks))
+    rc, chunks_s, _, _ = oracle.read_gz_stream_serial(data[: n // 2])
+    assert rc == oracle.ORC_E_TRUNCATED and payload.startswith(b"".join(chunks_s))
+    assert len(b"".join(chunks_s)) >= len(b"".join(chunks))
+    # trailer cut off: the member's bytes are all sent first
+    rc, chunks, _, _ = oracle.read_gz(data[:-3], chunk_size=65536, threads=1)
+    assert rc == oracle.ORC_E_TRUNCATED and b"".join(chunks) == payload
+    assert oracle.read_gz_stream_serial(b"")[0] == oracle.ORC_E_TRUNCATED
+    assert oracle.read_gz(data + b"junk", 65536, 1)[0] == oracle.read_gz(data + b"junk", 65536, 4)[0] != 0
+    # a closed sink: the serial path returns Ok with what was sent; the
+    # streaming serial path decodes on and counts every chunk (lib.rs:296-298)
+    rc, chunks, st, _ = oracle.read_gz(data, chunk_size=65536, threads=1, sink_limit=1)
+    assert rc == 0 and len(chunks) == 1 and st.chunks_decoded == 1
+    full = oracle.read_gz_stream_serial(data)[1]
+    rc, chunks, st, _ = oracle.read_gz_stream_serial(data, sink_limit=2)
+    assert rc == 0 and len(chunks) == 2 and st.chunks_decoded == len(full)
+    assert st.uncompressed_bytes == len(payload)
+
+
+def test_serial_back_reference_before_member_start():
+    """Member 2's block Y reaches back into member 1.  The serial path's
+    member-local window rejects it (fast_inflate.rs:1704-1715).  On the
+    speculative path the same stream decodes when a chunk starts at Y's block
+    (its markers resolve from the previous chunk's output), and fails when Y
+    is decoded as a later member of a chunk (pipeline.rs:153-254)."""
+    import crafted
+
+    f, pay, y_at, (la, lz, ly) = crafted.reach_before_member(80000, 15000, 12000, 3000)
+    hl = oracle.parse_header(f)
+    assert oracle.chunk_boundaries(f[hl:], 65536) == [0, (y_at - hl) * 8]
+    rc, chunks, _, er = oracle.read_gz(f, chunk_size=65536, threads=4)
+    assert rc == 0 and b"".join(chunks) == pay
+    rc, chunks, _, er = oracle.read_gz(f, chunk_size=65536, threads=1)
+    assert rc == oracle.ORC_E_INVALID and er.msg == b"back-reference distance out of bounds"
+    assert [len(c) for c in chunks] == [la]
+    rc, chunks, _, er = oracle.read_gz_stream_serial(f)
+    assert rc == oracle.ORC_E_INVALID and b"".join(chunks) == pay[:la]
+    f, pay, y_at, (la, lz, ly) = crafted.reach_before_member(76000, 15000, 12000, 3000)
+    hl = oracle.parse_header(f)
+    assert oracle.chunk_boundaries(f[hl:], 65536) == [0]
+    rc, chunks, _, er = oracle.read_gz(f, chunk_size=65536, threads=4)
+    assert rc == oracle.ORC_E_INVALID and chunks == []
+
+
+CONCAT_CASES = {
+    # tests/concat.rs:84-265, all run at threads 1, 2 and 4 there
+    "hello_world": [(b"Hello", 6), (b"World", 6)],
+    "empty_interspersed": [(b"", 6), (b"alpha", 6), (b"", 9), (b"", 1), (b"beta", 6), (b"", 6)],
+    "mixed_levels": [(lcg_payload(1, 1), 6), (lcg_payload(100, 2), 1), (lcg_payload(65_537, 3), 9),
+                     (lcg_payload(1, 4), 6), (lcg_payload(300_000, 5), 6), (lcg_payload(7, 6), 1),
+                     (lcg_payload(1_000_000, 7), 9)],
+    "many_small": [(f"member-{i}-payload\n".encode(), 6) for i in range(500)],
+}
+
+
+@pytest.mark.parametrize("case", sorted(CONCAT_CASES))
+def test_concat_cases_all_thread_counts(case):
+    parts = CONCAT_CASES[case]
+    g = b"".join(gz(p, lv) for p, lv in parts)
+    want = b"".join(p for p, _ in parts)
+    for threads in (1, 2, 4):
+        rc, chunks, st, er = oracle.read_gz(g, chunk_size=65536, threads=threads)
+        assert rc == 0, (threads, er.msg)
+        assert b"".join(chunks) == want, threads
+    rc, chunks, st, er = oracle.read_gz_stream_serial(g)
+    assert rc == 0 and b"".join(chunks) == want
+
+
 def test_generators_deterministic_and_valid():
     from rusty_rapidgzip_amd import synth
 
