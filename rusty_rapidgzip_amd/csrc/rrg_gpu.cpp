@@ -186,6 +186,16 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     return G;
 }
 
+// Output cells of a speculative segment spanning `span` input bytes: room
+// for an expansion of 4, plus the rest of a block of up to 32 KiB of input.
+// A segment decodes on to the first block boundary at or after its hint, so
+// when the next segment's start is a false find (no block starts there) its
+// decode runs into the next segment's range; that room keeps it from
+// overflowing, and the false start then costs one re-decode of the next
+// segment only.  Beyond it the kernel counts the cells needed and the segment
+// is regrown.
+uint64_t seg_cells(uint64_t span) { return ((span + 32768) * 4 + 16384 + 7) & ~7ull; }
+
 // Member-record slots of a segment spanning `span` input bytes: a few member
 // ends, or, when block ends are recorded, about one block per 4 KiB of input
 // (more are found by regrowing).
@@ -835,7 +845,10 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         SegDesc& d = desc[j];
         const uint64_t hint = d.hint_bit;
         const uint64_t end = hint == UINT64_MAX ? J.end_bits() : hint;
-        const uint64_t true_end = R.end_bit != UINT64_MAX ? std::max(R.end_bit, end) : end;
+        // a decode from a true start that ran past its hint passed no block
+        // start in [hint, end_bit): cuts stay below the hint, and the last
+        // piece's region covers the decode up to end_bit
+        const uint64_t last_end = R.end_bit != UINT64_MAX && R.end_bit > end ? R.end_bit : end;
         if (overflowed(R.status)) {
             grow_segment(d, R.status, cells_used, mb_used, R.needed);
             ++c.prof.n_grown;
@@ -854,10 +867,9 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
             p.start_bit = cuts[q];
             p.hint_bit = q + 1 < cuts.size() ? cuts[q + 1] : hint;
             p.end_byte = d.end_byte;
-            // the last piece decodes on to where the whole segment ended
-            const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : true_end) - cuts[q]) / 8;
+            const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : last_end) - cuts[q]) / 8;
             p.cells_base = cells_used;
-            p.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
+            p.cells_cap = seg_cells(span);
             cells_used += p.cells_cap;
             p.flags = SEG_SPEC | (d.flags & SEG_BLOCK_ENDS);
             p.mb_base = (uint32_t)mb_used;
@@ -975,7 +987,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             d.end_byte = J.in_off + J.len;
             const uint64_t span = ((j + 1 < S ? M[j + 1] : J.end_bits()) - M[j]) / 8;
             d.cells_base = cells_used;
-            d.cells_cap = ((span * 4 + 16384) + 7) & ~7ull;
+            d.cells_cap = seg_cells(span);
             cells_used += d.cells_cap;
             d.flags = (j ? SEG_SPEC : 0u) | (J.serial ? SEG_BLOCK_ENDS : 0u);
             d.mb_base = (uint32_t)mb_used;

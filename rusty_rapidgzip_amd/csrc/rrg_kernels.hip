@@ -171,14 +171,20 @@ __device__ __forceinline__ bool precode_ok(uint64_t lo, uint64_t hi, uint64_t p,
     return nsym == 1 || (nsym >= 2 && sum == 128u);
 }
 
-// Full strict verification (block_finder.rs:175-301) of `p` as wave-uniform
-// code; the lanes cooperate only to build the precode LUT.
-__device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
-                            uint64_t p, int lane) {
+// Strict dynamic header (block_finder.rs:196-262) at `p`, the block's first
+// bit (BFINAL/BTYPE are the caller's), as wave-uniform code; the lanes
+// cooperate to build the precode LUT and to fill runs of code lengths.  On
+// success the canonical tables of both codes are in fw and *after is the
+// first bit of the block's data.
+__device__ bool strict_header(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
+                              uint64_t p, int lane, uint64_t* after, uint32_t* hdist_out) {
     const uint64_t w0 = ubits(in64, fw, rbase, p);
     const uint32_t hlit = 257u + (uint32_t)((w0 >> 3) & 31u);
     const uint32_t hdist = 1u + (uint32_t)((w0 >> 8) & 31u);
     const uint32_t hclen = 4u + (uint32_t)((w0 >> 13) & 15u);
+    // field ranges and the precode (the scan's prefilter has checked them for
+    // its candidates; a header behind a block end has not been)
+    if (hlit > 286u || hdist > 30u || !precode_ok(w0, ubits(in64, fw, rbase, p + 64), p, nbits)) return false;
     // precode lengths and canonical codes, one symbol per lane (lanes 0..18),
     // counts and ranks by ballot: everything stays in registers.  Every lane
     // then builds the two LUT entries lane and lane+64.
@@ -215,21 +221,29 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     const uint32_t total = hlit + hdist;
     uint32_t i = 0, last = 0;
     uint32_t sumL = 0, nL = 0, sumD = 0, nD = 0;
+    // a run of n lengths L: the lanes fill lens[] together, the Kraft sums
+    // and counts follow by arithmetic
     auto put = [&](uint32_t L, uint32_t n) -> bool {
-        for (uint32_t k = 0; k < n; ++k, ++i) {
-            if (lane == 0) fw.lens[i] = (uint8_t)L;
-            if (L) {
-                if (i < hlit) {
-                    sumL += 1u << (15 - L);
-                    ++nL;
-                } else {
-                    sumD += 1u << (15 - L);
-                    ++nD;
-                }
-            }
+        for (uint32_t b = 0; b < n; b += 64)
+            if ((uint32_t)lane < n - b) fw.lens[i + b + (uint32_t)lane] = (uint8_t)L;
+        if (L) {
+            const uint32_t nl = i < hlit ? min(n, hlit - i) : 0u, nd = n - nl;
+            sumL += nl << (15 - L);
+            nL += nl;
+            sumD += nd << (15 - L);
+            nD += nd;
         }
+        const bool crossed = i < hlit && i + n >= hlit;
+        i += n;
         // oversubscription only grows: reject as soon as it appears
-        return !((nL >= 2 && sumL > 32768u) || (nD >= 2 && sumD > 32768u));
+        if ((nL >= 2 && sumL > 32768u) || (nD >= 2 && sumD > 32768u)) return false;
+        // the literal/length code is settled once its lengths are all read
+        if (crossed) {
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (uni32(fw.lens[256]) == 0 || (nL >= 2 && sumL != 32768u)) return false;
+        }
+        return true;
     };
     while (i < total) {
         if (pos + 10 > nbits) return false;
@@ -252,15 +266,13 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
             if (pos + 3 > nbits) return false;
             const uint32_t n = 3u + (uint32_t)(x & 7u);
             pos += 3;
-            if (i + n > total) return false;
-            put(0, n);
+            if (i + n > total || !put(0, n)) return false;
             last = 0;
         } else {
             if (pos + 7 > nbits) return false;
             const uint32_t n = 11u + (uint32_t)(x & 127u);
             pos += 7;
-            if (i + n > total) return false;
-            put(0, n);
+            if (i + n > total || !put(0, n)) return false;
             last = 0;
         }
     }
@@ -268,30 +280,56 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (uni32(fw.lens[256]) == 0) return false;
     if ((nL >= 2 && sumL != 32768u) || (nD >= 2 && sumD != 32768u)) return false;
-    // trial decode of 64 symbols (block_finder.rs:268-301)
     if (lane == 0) {
         sort_lengths(fw.lens, (int)hlit, fw.cnt, fw.sorted);
         sort_lengths(fw.lens + hlit, (int)hdist, fw.cnt + 16, fw.sorted + 288);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int k = 0; k < 64; ++k) {
-        const int sym = ucanon(in64, fw, rbase, nbits, pos, fw.cnt, fw.sorted);
-        if (sym < 0) return false;
-        if (sym <= 255) continue;
-        if (sym == 256) return true;
-        if (sym > 285) return false;
-        const uint32_t lx = len_extra((uint32_t)sym - 257);
-        if (pos + lx > nbits) return false;
-        pos += lx;
-        const int ds = ucanon(in64, fw, rbase, nbits, pos, fw.cnt + 16, fw.sorted + 288);
-        if (ds < 0 || ds >= 30) return false;
-        const uint32_t dx = dist_extra((uint32_t)ds);
-        if (pos + dx > nbits) return false;
-        pos += dx;
-    }
+    *after = pos;
+    *hdist_out = hdist;
     return true;
 }
+
+// Decode up to max_syms symbols of the block whose tables are in fw
+// (block_finder.rs:268-301): 0 = an invalid symbol or the end of the input,
+// 1 = end of block (pos just after it), 2 = max_syms decoded.
+__device__ int trial_syms(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
+                          uint64_t& pos, int max_syms) {
+    for (int k = 0; k < max_syms; ++k) {
+        const int sym = ucanon(in64, fw, rbase, nbits, pos, fw.cnt, fw.sorted);
+        if (sym < 0) return 0;
+        if (sym <= 255) continue;
+        if (sym == 256) return 1;
+        if (sym > 285) return 0;
+        const uint32_t lx = len_extra((uint32_t)sym - 257);
+        if (pos + lx > nbits) return 0;
+        pos += lx;
+        const int ds = ucanon(in64, fw, rbase, nbits, pos, fw.cnt + 16, fw.sorted + 288);
+        if (ds < 0 || ds >= 30) return 0;
+        const uint32_t dx = dist_extra((uint32_t)ds);
+        if (pos + dx > nbits) return 0;
+        pos += dx;
+    }
+    return 2;
+}
+
+// The reference's candidate verification (block_finder.rs:175-189): strict
+// header and a 64-symbol trial; *hdist_out = the header's HDIST.
+__device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
+                            uint64_t p, int lane, uint32_t* hdist_out) {
+    uint64_t pos;
+    if (!strict_header(in64, fw, rbase, nbits, p, lane, &pos, hdist_out)) return false;
+    return trial_syms(in64, fw, rbase, nbits, pos, 64) != 0;
+}
+
+// Segment targets take any true block start, so they may pass over true ones.
+// A header with at most this many distance codes constrains random bits far
+// less than the rest (a code of one to three symbols is complete almost by
+// itself): every false block start seen on 1.3 GB of gzip -6 FASTQ had one.
+// The scan for a segment target takes the first verified candidate with more
+// distance codes, and the first weaker one only if its range holds none.
+constexpr uint32_t kWeakHdist = 4;
 
 __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
                                                                 const uint64_t* __restrict__ targets,
@@ -327,6 +365,7 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     const uint64_t nbits = nbits_file;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
+    uint64_t weak = ~0ull;  // any_start: the first verified candidate with a weak header
     uint64_t rbase = ~0ull;
     bool end = (cur + 13 > limit);
     while (!end) {
@@ -372,11 +411,19 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
             uint64_t m = cm | sm;
             while (m) {
                 const uint32_t rel = (uint32_t)__builtin_ctzll(m);
-                if (((sm >> rel) & 1ull) ||
-                    uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane) ? 1u : 0u)) {
+                if ((sm >> rel) & 1ull) {
                     result = base + rel;
                     end = true;
                     break;
+                }
+                uint32_t hd = 0;
+                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane, &hd) ? 1u : 0u)) {
+                    if (uni32(hd) > kWeakHdist) {
+                        result = base + rel;
+                        end = true;
+                        break;
+                    }
+                    if (weak == ~0ull) weak = base + rel;
                 }
                 m &= m - 1;
             }
@@ -409,7 +456,8 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
                 if (q <= c) pth = q;
             }
             if (pth != c) continue;  // the chain jumps over this candidate
-            if (uni32(deep_verify(in64, fw, rbase, nbits, base + c, lane) ? 1u : 0u)) {
+            uint32_t hd = 0;
+            if (uni32(deep_verify(in64, fw, rbase, nbits, base + c, lane, &hd) ? 1u : 0u)) {
                 result = base + c;
                 end = true;
                 break;
@@ -427,6 +475,7 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         }
         cur = base + pth + uni32((uint32_t)__builtin_amdgcn_readlane((int)nxu, (int)pth));
     }
+    if (result == ~0ull) result = weak;
     if (lane == 0) found[t] = result;
 }
 
