@@ -15,6 +15,8 @@ with rocprofv3 counters (profiles/traffic_fastq4g.json), when present.
 
 Other workloads (--workload), each one JSON line of its own:
   bgzf    the same 4 GiB as BGZF (65280-byte members, level 6), one batch
+  mixed   2 GiB mixed-entropy stress in one member (stored random segments,
+          runs, repeats, text at levels 1/9)
   batch   64 FASTQ files of 64 MiB, one batched call vs one call per file
   serve   request mix through the serving loop (sizes log-uniform, text /
           FASTQ / mixed at levels 1/6/9), closed-loop capacity, then Poisson
@@ -243,6 +245,56 @@ def run_bgzf(args, threads):
 
 
 # ---------------------------------------------------------------------------
+def run_mixed(args, threads):
+    """Mixed-entropy stress in one gzip member: 1-8 MiB segments of random
+    bytes (stored), byte runs, period-3..300 repeats and Markov text at levels
+    1/9, switched inside the member."""
+    import numpy as np
+
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import hipmem, synth
+
+    n = int(args.mixed_gib * (1 << 30))
+    raw, lv, ends = synth.mixed(n, threads=threads)
+    t0 = time.time()
+    gz = synth.gzip_segments(raw, lv, ends, threads=threads, piece=64 << 20)
+    C = int(gz.size)
+    log(f"mixed {n} -> {C} bytes ({len(ends)} segments) in {time.time() - t0:.1f}s")
+    d_in = hipmem.DeviceBuffer.from_host(gz)
+    cfg = make_cfg(args)
+    dec = rrg.Decoder(0)
+
+    def step():
+        return dec.decode_device(d_in.ptr, C, gz, cfg)
+
+    for w in range(max(1, args.warmup)):
+        ptr, out_len, ends_out, st = step()
+        if w == 0:
+            assert out_len == n and np.array_equal(hipmem.to_host(ptr, n), raw)
+            log(f"warmup verified: {len(ends_out)} chunks, {st.speculation_failures} speculation failures")
+        log(f"warmup {w} {dec.profile().as_dict()}")
+    prof = []
+
+    def step_p():
+        step()
+        prof.append(dec.profile().as_dict())
+
+    elapsed, lat = timed_steps(step_p, args.steps)
+    result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, mixed entropy)",
+                   n * args.steps / elapsed / 1e9, "GB/s", args.steps, args.warmup, elapsed * 1e3 / args.steps)
+    result["config"] = {"workload": f"mixed-entropy {args.mixed_gib:g} GiB single member: 1-8 MiB segments of "
+                                    "random bytes (stored), runs, period-3..300 repeats, Markov text at levels 1/9",
+                        "compressed_bytes": C, "uncompressed_bytes": n, "chunk_size_bytes": cfg.chunk_size_bytes,
+                        "segments": int(prof[-1]["n_segments"]), "input": "resident in HBM"}
+    result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
+    result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3)
+                           for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+    result["inflate_launches"] = int(prof[-1]["n_inflate_launches"])
+    d_in.free()
+    return result
+
+
+# ---------------------------------------------------------------------------
 def run_batch(args, threads):
     import numpy as np
 
@@ -369,7 +421,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="fastq4g", choices=("fastq4g", "bgzf", "batch", "serve"))
+    ap.add_argument("--workload", default="fastq4g", choices=("fastq4g", "bgzf", "mixed", "batch", "serve"))
+    ap.add_argument("--mixed-gib", type=float, default=2.0, help="uncompressed size of the mixed workload")
     ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
     ap.add_argument("--chunk-mib", type=int, default=4)
     ap.add_argument("--segment-kib", type=int, default=0, help="0 = automatic")
@@ -390,7 +443,8 @@ def main() -> int:
         log("only --gpus 1 is supported: the decoder runs on a single MI355X")
         return 2
     threads = min(16, os.cpu_count() or 1)
-    fn = {"fastq4g": run_fastq, "bgzf": run_bgzf, "batch": run_batch, "serve": run_serve}[args.workload]
+    fn = {"fastq4g": run_fastq, "bgzf": run_bgzf, "mixed": run_mixed, "batch": run_batch,
+          "serve": run_serve}[args.workload]
     result = fn(args, threads)
     print(json.dumps(result), flush=True)
     return 0

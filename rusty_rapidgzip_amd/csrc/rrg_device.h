@@ -89,6 +89,14 @@ struct ResolveTile {
     uint32_t n;          // cells in the tile
 };
 
+// An exact block-finder target continued in legs (k_leg_fixup).
+struct LegTarget {
+    uint64_t p;      // true chain position where the legs begin
+    uint64_t limit;  // the end of the target's file (candidates and verification)
+    uint64_t end;    // where the last leg stops
+    uint32_t first, n_legs;
+};
+
 // A segment whose markers may name bytes of the previous member: `reach`
 // bytes of its own member precede it (single-worker paths).
 struct ReachScan {

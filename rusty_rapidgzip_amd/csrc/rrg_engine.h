@@ -65,7 +65,7 @@ struct Ctx {
     hipStream_t s = nullptr;
     int n_cu = 0;
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
-        pcrc, err, reach, first;
+        pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
     uint8_t* pin = nullptr;
@@ -75,7 +75,8 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Ctx() {
         DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
-                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first};
+                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
+                         &stops,  &exits,   &recs,   &legtg, &lfound, &lnext};
         for (DevBuf* b : all) b->release();
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);

@@ -219,6 +219,8 @@ void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& m
     }
 }
 
+inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
+
 // One launch of the inflate kernel over the segment indices in `work`; results
 // for every segment are copied back into `res`.
 int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vector<SegResult>& res,
@@ -245,16 +247,29 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
         t.stop();
     }
     c.prof.n_inflate_launches++;
-    if (c.verbose)
-        std::fprintf(stderr, "[rrg-gpu] inflate launch %llu: %zu segments in %.3f ms\n",
-                     (unsigned long long)c.prof.n_inflate_launches, work.size(), c.prof.ms_inflate - before);
     res.resize(S);
     HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
+    if (c.verbose) {
+        // the launch's largest outputs and its status counts
+        std::vector<std::pair<uint64_t, uint32_t>> big;
+        size_t n_over = 0, n_err = 0;
+        for (uint32_t j : work) {
+            big.push_back({res[j].n_cells, j});
+            n_over += overflowed(res[j].status);
+            n_err += res[j].status != ST_OK && !overflowed(res[j].status);
+        }
+        std::sort(big.rbegin(), big.rend());
+        std::fprintf(stderr, "[rrg-gpu] inflate launch %llu: %zu segments in %.3f ms (%zu out of room, %zu failed); "
+                     "largest outputs:", (unsigned long long)c.prof.n_inflate_launches, work.size(),
+                     c.prof.ms_inflate - before, n_over, n_err);
+        for (size_t k = 0; k < big.size() && k < 4; ++k)
+            std::fprintf(stderr, " %llu cells from %llu B", (unsigned long long)big[k].first,
+                         (unsigned long long)((res[big[k].second].end_bit - desc[big[k].second].start_bit) / 8));
+        std::fprintf(stderr, "\n");
+    }
     return RRG_OK;
 }
-
-inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
 
 // Tail chain + resolve + narrow for the chain segments `cs` into `out`;
 // out_off[k] is chain segment k's first output byte.  flags[k] != 0 when a
@@ -468,7 +483,7 @@ struct Redo {
     uint64_t start;
     uint32_t status;  // ST_OK: false start; ST_OVERFLOW / ST_MB_OVERFLOW: region too small
     uint64_t needed;  // cells the segment needs (ST_OVERFLOW)
-    uint64_t end_bit = UINT64_MAX;  // where its decode ends, when known (ST_OVERFLOW at a true start)
+    uint64_t end_bit = UINT64_MAX;  // where its decode ended (ST_OVERFLOW; the start may be unverified)
 };
 
 // Anchored verification chain of one job (pipeline.rs:774-809): accept a
@@ -499,11 +514,13 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
             if (redo.size() == before) J.fatal = (long)k;
             break;
         }
+        const bool verified = redo.size() == before;  // the chain before it holds no re-decode
         redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK, r.n_cells});
-        if (desc[j].start_bit == expected && r.status == ST_OVERFLOW) {
-            // out of cells, the lane decoded on without storing: where it ends is
-            // known, so the next segment is checked in this same round
-            redo.back().end_bit = r.end_bit;
+        if (desc[j].start_bit == expected && r.status == ST_OVERFLOW) redo.back().end_bit = r.end_bit;
+        if (verified && desc[j].start_bit == expected && r.status == ST_OVERFLOW) {
+            // out of cells at a verified start, the lane decoded on without
+            // storing: where it ends is known, so the next segment is checked
+            // in this same round
             if (r.final_block) break;
             expected = r.end_bit;
         } else if (k + 1 < n) {
@@ -774,8 +791,13 @@ int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc
 }
 
 // One finder launch; every find must lie in [target, scan end) of its file.
+// Exact targets may carry `stops` (their walk ends at the first window
+// starting there) and report `exits` (the chain position where it ended) and,
+// with want_recs, their first 256 chain positions (left on the device).
 int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
-               const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err) {
+               const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err,
+               const std::vector<uint64_t>* stops = nullptr, std::vector<uint64_t>* exits = nullptr,
+               bool want_recs = false) {
     found.assign(targets.size(), UINT64_MAX);
     if (targets.empty()) return RRG_OK;
     const size_t n = targets.size();
@@ -786,13 +808,35 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
     HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), n * 8, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), n * 8, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.scan_ends.p, scan_ends.data(), n * 8, hipMemcpyHostToDevice, c.s));
+    if (stops) {
+        HIPCHK(c.stops.ensure(n * 8));
+        HIPCHK(hipMemcpyAsync(c.stops.p, stops->data(), n * 8, hipMemcpyHostToDevice, c.s));
+    }
+    if (exits) HIPCHK(c.exits.ensure(n * 8));
+    if (want_recs) HIPCHK(c.recs.ensure(n * 32));
     StageTimer t(c, &c.prof.ms_find);
     HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
-                              c.limits.as<uint64_t>(), c.scan_ends.as<uint64_t>(), c.found.as<uint64_t>(),
+                              c.limits.as<uint64_t>(), c.scan_ends.as<uint64_t>(),
+                              stops ? c.stops.as<uint64_t>() : nullptr, c.found.as<uint64_t>(),
+                              exits ? c.exits.as<uint64_t>() : nullptr, want_recs ? c.recs.as<uint64_t>() : nullptr,
                               (uint32_t)n, c.s));
     HIPCHK(hipMemcpyAsync(found.data(), c.found.p, n * 8, hipMemcpyDeviceToHost, c.s));
+    if (exits) {
+        exits->assign(n, 0);
+        HIPCHK(hipMemcpyAsync(exits->data(), c.exits.p, n * 8, hipMemcpyDeviceToHost, c.s));
+    }
     t.stop();
     HIPCHK(hipStreamSynchronize(c.s));
+    if (c.verbose) {
+        uint64_t longest = 0, none = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const uint64_t e = found[i] != UINT64_MAX ? found[i] : exits ? (*exits)[i] : scan_ends[i];
+            longest = std::max(longest, e - targets[i]);
+            none += found[i] == UINT64_MAX;
+        }
+        std::fprintf(stderr, "[rrg-gpu] finder: %zu targets, %llu without a find, longest scan %llu KiB\n", n,
+                     (unsigned long long)none, (unsigned long long)(longest / 8192));
+    }
     // a find is a bit offset inside the target's scan range; anything else is a device fault
     for (size_t i = 0; i < n; ++i)
         if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= scan_ends[i])) {
@@ -803,12 +847,107 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
     return RRG_OK;
 }
 
+// An exact target whose bounded first walk ended at chain position `p`
+// without a find, before the end of its file (`limit`, in bits).
+struct OpenTarget {
+    size_t idx;  // index into the caller's found[]
+    uint64_t p, limit;
+    uint64_t span = 0;  // input (bits) the next round of legs covers
+};
+
+// Exact finds over long stretches without a verified candidate (stored or
+// incompressible data can run for MiBs): the next part of each open target's
+// range is cut into legs of kLegBits, all legs of all targets walked at once
+// (one wavefront each, recording their first chain positions), and
+// k_leg_fixup stitches each target's legs along its true chain.  The part
+// grows 4x per round (1 MiB, 4 MiB, ... 64 MiB) until every target has its
+// find or reached its file end.
+int resolve_open(Ctx& c, std::vector<OpenTarget> open, std::vector<uint64_t>& found, rrg_error* err) {
+    constexpr uint64_t kLegBits = 8ull * 16384;
+    constexpr uint64_t kFirstSpan = 8ull * (1ull << 20), kMaxSpan = 8ull * (64ull << 20);
+    for (OpenTarget& o : open) o.span = kFirstSpan;
+    while (!open.empty()) {
+        std::vector<uint64_t> targets, limits, scan_ends, stops;
+        std::vector<LegTarget> tg;
+        for (const OpenTarget& o : open) {
+            LegTarget T{};
+            T.p = o.p;
+            T.limit = o.limit;
+            T.end = std::min(o.limit, o.p + o.span);
+            T.first = (uint32_t)targets.size();
+            for (uint64_t a = o.p; a < T.end; a += kLegBits) {
+                targets.push_back(a);
+                limits.push_back(o.limit);
+                scan_ends.push_back(o.limit);
+                stops.push_back(std::min(T.end, a + kLegBits));
+            }
+            T.n_legs = (uint32_t)(targets.size() - T.first);
+            tg.push_back(T);
+        }
+        std::vector<uint64_t> lf, lx;
+        int rc = run_finder(c, targets, limits, scan_ends, lf, err, &stops, &lx, true);
+        if (rc) return rc;
+        const size_t n = tg.size();
+        HIPCHK(c.legtg.ensure(n * sizeof(LegTarget)));
+        HIPCHK(c.lfound.ensure(n * 8));
+        HIPCHK(c.lnext.ensure(n * 8));
+        HIPCHK(hipMemcpyAsync(c.legtg.p, tg.data(), n * sizeof(LegTarget), hipMemcpyHostToDevice, c.s));
+        std::vector<uint64_t> of(n), on(n);
+        {
+            StageTimer t(c, &c.prof.ms_find);
+            HIPCHK(launch_leg_fixup(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.legtg.as<LegTarget>(),
+                                    c.targets.as<uint64_t>(), c.found.as<uint64_t>(), c.exits.as<uint64_t>(),
+                                    c.recs.as<uint64_t>(), c.lfound.as<uint64_t>(), c.lnext.as<uint64_t>(),
+                                    (uint32_t)n, c.s));
+            HIPCHK(hipMemcpyAsync(of.data(), c.lfound.p, n * 8, hipMemcpyDeviceToHost, c.s));
+            HIPCHK(hipMemcpyAsync(on.data(), c.lnext.p, n * 8, hipMemcpyDeviceToHost, c.s));
+            t.stop();
+        }
+        HIPCHK(hipStreamSynchronize(c.s));
+        std::vector<OpenTarget> still;
+        for (size_t i = 0; i < n; ++i) {
+            const OpenTarget& o = open[i];
+            if (of[i] != UINT64_MAX) {
+                if (of[i] < o.p || of[i] >= o.limit) {
+                    set_err(err, RRG_E_DEVICE, "leg fix-up returned %llu for %llu", (unsigned long long)of[i],
+                            (unsigned long long)o.p);
+                    return RRG_E_DEVICE;
+                }
+                found[o.idx] = of[i];
+            } else if (on[i] + 13 <= o.limit && on[i] > o.p) {
+                still.push_back(OpenTarget{o.idx, on[i], o.limit, std::min(kMaxSpan, 4 * o.span)});
+            }
+        }
+        if (c.verbose)
+            std::fprintf(stderr, "[rrg-gpu] finder legs: %zu targets, %zu legs, %zu still open\n", n, targets.size(),
+                         still.size());
+        open.swap(still);
+    }
+    return RRG_OK;
+}
+
+// The first walk of an exact target covers this much input; targets without
+// a find by then are continued in parallel legs (resolve_open).
+constexpr uint64_t kExactCapBits = 8ull * 131072;
+
+// Runs of a finder launch whose exact targets (limits without kAnyStart) had
+// `stops`: continue those that ended open.
+int finish_open(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
+                const std::vector<uint64_t>& exits, std::vector<uint64_t>& found, rrg_error* err) {
+    std::vector<OpenTarget> open;
+    for (size_t i = 0; i < targets.size(); ++i)
+        if (!(limits[i] & kAnyStart) && found[i] == UINT64_MAX && exits[i] > targets[i] && exits[i] + 13 <= limits[i])
+            open.push_back(OpenTarget{i, exits[i], limits[i]});
+    return resolve_open(c, open, found, err);
+}
+
 // Turn the re-decode requests of one settle round into segments to decode:
 // each requested segment restarts at its verified start and, when it spans
 // enough input, is cut at the blocks the finder reports inside it (one finder
 // launch for all requests); the pieces join the job's chain after it.
 int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vector<SegDesc>& desc,
-               uint64_t& cells_used, uint64_t& mb_used, std::vector<uint32_t>& again, rrg_error* err) {
+               const std::vector<SegResult>& res, uint64_t& cells_used, uint64_t& mb_used,
+               std::vector<uint32_t>& again, rrg_error* err) {
     if (redo.empty()) return RRG_OK;
     constexpr uint64_t kPieceBits = 8 * 2048;  // at least 2 KiB of input per piece
     constexpr uint64_t kMaxPieces = 32;
@@ -832,6 +971,10 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
     std::vector<uint64_t> found;
     const int frc = run_finder(c, targets, limits, scan_ends, found, err);
     if (frc) return frc;
+    // cells granted beyond the default sizes in this round (a start that is
+    // not verified yet may be a false one, whose counts mean nothing)
+    uint64_t guessed = 0;
+    constexpr uint64_t kGuessBudget = 1ull << 31;
     // later positions first, so earlier insertions do not move them
     std::vector<size_t> idx(redo.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
@@ -849,6 +992,28 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         // start in [hint, end_bit): cuts stay below the hint, and the last
         // piece's region covers the decode up to end_bit
         const uint64_t last_end = R.end_bit != UINT64_MAX && R.end_bit > end ? R.end_bit : end;
+        // expected output cells per input byte in this range: what the
+        // segment itself counted when it ran out of room, else what its chain
+        // predecessor produced (pieces of a high-ratio stretch would
+        // otherwise overflow again)
+        double ratio = 0;
+        if (R.status == ST_OVERFLOW && R.end_bit != UINT64_MAX && R.end_bit > R.start + 8 * 1024) {
+            ratio = (double)R.needed / (double)((R.end_bit - R.start) / 8);
+        } else if (R.pos > 0) {
+            const uint32_t q = J.order[R.pos - 1];
+            if (res[q].status == ST_OK && res[q].end_bit == R.start && res[q].end_bit > desc[q].start_bit + 8 * 1024)
+                ratio = (double)res[q].n_cells / (double)((res[q].end_bit - desc[q].start_bit) / 8);
+        }
+        // at most DEFLATE's largest expansion; the region of one piece at most
+        // kMaxGuess cells (beyond it the kernel counts and the piece is regrown)
+        constexpr uint64_t kMaxGuess = 1ull << 26;
+        ratio = std::min(ratio, 1032.0);
+        auto guess = [&](uint64_t span) -> uint64_t {
+            const uint64_t g = std::min<uint64_t>(kMaxGuess, ((uint64_t)((double)span * ratio * 1.25) + 65536 + 7) & ~7ull);
+            if (guessed + g > kGuessBudget) return 0;
+            guessed += g;
+            return g;
+        };
         if (overflowed(R.status)) {
             grow_segment(d, R.status, cells_used, mb_used, R.needed);
             ++c.prof.n_grown;
@@ -858,6 +1023,16 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         for (size_t t = t_first[i]; t < t_first[i + 1]; ++t)
             if (found[t] != UINT64_MAX && found[t] > (cuts.empty() ? R.start : cuts.back()) && found[t] < end)
                 cuts.push_back(found[t]);
+        {
+            // the segment's own region, for its (new) range
+            const uint64_t span0 = ((cuts.empty() ? last_end : cuts[0]) - R.start) / 8;
+            const uint64_t want = guess(span0);
+            if (ratio > 0 && !overflowed(R.status) && d.cells_cap < want) {
+                d.cells_base = (cells_used + 7) & ~7ull;
+                d.cells_cap = want;
+                cells_used = d.cells_base + want;
+            }
+        }
         again.push_back(j);
         if (cuts.empty()) continue;
         d.hint_bit = cuts[0];
@@ -869,7 +1044,7 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
             p.end_byte = d.end_byte;
             const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : last_end) - cuts[q]) / 8;
             p.cells_base = cells_used;
-            p.cells_cap = seg_cells(span);
+            p.cells_cap = std::max<uint64_t>(seg_cells(span), ratio > 0 ? guess(span) : 0);
             cells_used += p.cells_cap;
             p.flags = SEG_SPEC | (d.flags & SEG_BLOCK_ENDS);
             p.mb_base = (uint32_t)mb_used;
@@ -907,8 +1082,10 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
         J.stream_B.assign(1, J.abs_bits(J.body));
         for (uint64_t t = J.stream_B.back() + chunk_bits; t < J.end_bits(); t = J.stream_B.back() + chunk_bits) {
-            std::vector<uint64_t> f;
-            const int frc = run_finder(c, {t}, {J.end_bits()}, {J.end_bits()}, f, err);
+            std::vector<uint64_t> f, x;
+            const std::vector<uint64_t> tv{t}, lv{J.end_bits()}, sv{t + kExactCapBits};
+            int frc = run_finder(c, tv, lv, lv, f, err, &sv, &x);
+            if (!frc) frc = finish_open(c, tv, lv, x, f, err);
             if (frc) return frc;
             if (f[0] == UINT64_MAX) break;
             J.stream_B.push_back(f[0]);
@@ -921,8 +1098,12 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (J.body != UINT64_MAX) make_targets(J, G, targets, limits, scan_ends, is_chunk);
         J.t_hi = targets.size();
     }
+    // chunk targets walk at most kExactCapBits here and continue in legs
+    std::vector<uint64_t> stops(targets.size()), exits;
+    for (size_t i = 0; i < targets.size(); ++i) stops[i] = targets[i] + kExactCapBits;
     std::vector<uint64_t> found;
-    int rc = run_finder(c, targets, limits, scan_ends, found, err);
+    int rc = run_finder(c, targets, limits, scan_ends, found, err, &stops, &exits);
+    if (!rc) rc = finish_open(c, targets, limits, exits, found, err);
     if (rc) return rc;
     // 3. segment descriptors
     std::vector<SegDesc> desc;
@@ -1041,7 +1222,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             std::fprintf(stderr, "[rrg-gpu] settle round %d: %zu false starts, %zu out of room, %zu regrown alone\n",
                          round, n_false, n_over, again.size());
         }
-        rc = split_redo(c, jobs, redo, desc, cells_used, mb_used, again, err);
+        rc = split_redo(c, jobs, redo, desc, res, cells_used, mb_used, again, err);
         if (rc) return rc;
         c.prof.n_redecoded += again.size();
         rc = run_inflate(c, G, desc, res, cells_used, mb_used, again, err);

@@ -331,14 +331,147 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
 // distance codes, and the first weaker one only if its range holds none.
 constexpr uint32_t kWeakHdist = 4;
 
-__global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
-                                                                const uint64_t* __restrict__ targets,
-                                                                const uint64_t* __restrict__ limits,
-                                                                const uint64_t* __restrict__ scan_ends,
-                                                                uint64_t* __restrict__ found, uint32_t n_targets) {
-    __shared__ uint8_t s_next[8192];
-    __shared__ FindWave s_wave[FIND_WAVES];
-    for (int i = (int)threadIdx.x; i < 8192; i += 64 * FIND_WAVES) {
+// Lanes of one 64-position window that the reference's skip chain visits when
+// it enters at the window's first position: binary lifting over the doubling
+// tables J (J[k][l] = the position 2^k chain steps after l, 64 = left the
+// window) from position 0 toward each lane.
+__device__ __forceinline__ uint64_t chain_lanes(const uint32_t (&J)[6], int lane) {
+    uint32_t p = 0;
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+        const uint32_t q = (uint32_t)__shfl((int)J[k], (int)p, 64);
+        if (q <= (uint32_t)lane) p = q;
+    }
+    return __ballot(p == (uint32_t)lane);
+}
+
+// One exact walk (block_finder.rs:109-141): candidates p need p + 13 <= limit,
+// verification reads up to nbits, and windows start below `stop`.  With
+// `rec`, the chain positions in [rec_base, rec_base + 256) are OR-ed into
+// rec0..rec3.  With `mrg`, the walk ends at the first chain position that is
+// set in the bitmap m0..m3 based at mrg_base (its own candidates before that
+// point are checked; that position and later ones are the other chain's).
+struct ExactScan {
+    uint64_t stop, limit, nbits;
+    bool rec;
+    uint64_t rec_base;
+    bool mrg;
+    uint64_t mrg_base, m0, m1, m2, m3;
+};
+
+// Returns the first verified candidate on the chain from `cur` (~0 if none
+// before the walk ends); `cur` is left on the chain position after the last
+// window walked (every candidate before it was checked), or on the merge
+// position, reported in *merged (~0 if none).
+__device__ __forceinline__ uint64_t exact_walk(const uint64_t* __restrict__ in64, uint64_t nvalid, FindWave& fw,
+                               const uint8_t* s_next, int lane, uint64_t& rbase, uint64_t& cur, const ExactScan& sc,
+                               uint64_t& rec0, uint64_t& rec1, uint64_t& rec2, uint64_t& rec3, uint64_t* merged) {
+    *merged = ~0ull;
+    while (true) {
+        const uint64_t base = cur;
+        if (base + 13 > sc.limit || base >= sc.stop) return ~0ull;  // the reference's `while pos + 13 <= limit`
+        // the window needs bits [base, base + 63 + 128)
+        if (rbase == ~0ull || (base >> 6) + 3 >= rbase + REGION_WORDS || (base >> 6) < rbase) {
+            rbase = base >> 6;
+            for (int k = lane; k < REGION_WORDS; k += 64) fw.reg[k] = (rbase + k < nvalid) ? in64[rbase + k] : 0ull;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        const uint64_t p = base + (uint64_t)lane;
+        const uint32_t wi = (uint32_t)((p >> 6) - rbase);
+        const uint32_t sh = (uint32_t)(p & 63);
+        const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
+        const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
+        const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
+        uint32_t step = 255;
+        bool cand = false;
+        if (p + 13 <= sc.limit) {
+            step = s_next[(uint32_t)(lo & 0x1fffu)];
+            if (step == 0) cand = precode_ok(lo, hi, p, sc.nbits);
+        }
+        const uint64_t cm = __ballot(cand);
+        // Jump tables over the window: J[k][l] = the position 2^k chain steps
+        // after l (64 = left the window).  Whether a candidate lies on the
+        // chain, and where the chain leaves the window, then take six
+        // readlanes each instead of a walk of up to 64 dependent steps.
+        const uint32_t nxu = (step == 0) ? 1u : step;
+        uint32_t J[6];
+        J[0] = min((uint32_t)lane + nxu, 64u);
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+            const uint32_t prev = J[k - 1];
+            const uint32_t nxt = (uint32_t)__shfl((int)prev, (int)min(prev, 63u), 64);
+            J[k] = prev >= 64u ? 64u : nxt;
+        }
+        const uint64_t room = sc.limit - 13 - base;  // base + 13 <= limit here
+        const bool rec_now = sc.rec && base < sc.rec_base + 256;
+        const bool need_on = rec_now || sc.mrg;
+        const uint64_t onm = need_on ? uni64(chain_lanes(J, lane)) : 0ull;
+        uint64_t cands = cm;
+        uint32_t ml = 64;
+        if (sc.mrg) {
+            const uint64_t idx = p - sc.mrg_base;  // base >= mrg_base on every walk that merges
+            bool hit = false;
+            if (p >= sc.mrg_base && idx < 256) {
+                const uint32_t w = (uint32_t)(idx >> 6);
+                const uint64_t word = w == 0 ? sc.m0 : w == 1 ? sc.m1 : w == 2 ? sc.m2 : sc.m3;
+                hit = ((word >> (idx & 63)) & 1ull) != 0;
+            }
+            const uint64_t hm = __ballot(hit) & onm;
+            if (hm) {
+                ml = (uint32_t)__builtin_ctzll(hm);
+                cands &= (1ull << ml) - 1;  // from the merge position on, the other walk checked them
+            }
+        }
+        uint32_t pth = 0;  // last chain position visited
+        while (cands) {
+            const uint32_t c = (uint32_t)__builtin_ctzll(cands);
+            cands &= cands - 1;
+            if (need_on) {
+                if (!((onm >> c) & 1ull)) continue;
+            } else {
+#pragma unroll
+                for (int k = 5; k >= 0; --k) {
+                    const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
+                    if (q <= c) pth = q;
+                }
+                if (pth != c) continue;  // the chain jumps over this candidate
+            }
+            uint32_t hd = 0;
+            if (uni32(deep_verify(in64, fw, rbase, sc.nbits, base + c, lane, &hd) ? 1u : 0u)) return base + c;
+        }
+        if (rec_now) {
+            const uint64_t d = base - sc.rec_base;
+            const uint32_t w0 = (uint32_t)(d >> 6), s0 = (uint32_t)(d & 63);
+            const uint64_t lo_bits = onm << s0, hi_bits = s0 ? onm >> (64 - s0) : 0ull;
+            if (w0 == 0) { rec0 |= lo_bits; rec1 |= hi_bits; }
+            else if (w0 == 1) { rec1 |= lo_bits; rec2 |= hi_bits; }
+            else if (w0 == 2) { rec2 |= lo_bits; rec3 |= hi_bits; }
+            else { rec3 |= lo_bits; }
+        }
+        if (ml < 64) {
+            cur = base + ml;
+            *merged = cur;
+            return ~0ull;
+        }
+        uint32_t last;
+        if (need_on) {
+            last = 63u - (uint32_t)__builtin_clzll(onm);  // position 0 is always on the chain
+        } else {
+#pragma unroll
+            for (int k = 5; k >= 0; --k) {
+                const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
+                if (q < 64u) pth = q;
+            }
+            last = pth;
+        }
+        cur = base + last + uni32((uint32_t)__builtin_amdgcn_readlane((int)nxu, (int)last));
+        if (room < 64) return ~0ull;  // every later chain position is past the limit
+    }
+}
+
+__device__ void fill_next_lut(uint8_t* s_next) {
+    for (int i = (int)threadIdx.x; i < 8192; i += (int)blockDim.x) {
         uint32_t w = (uint32_t)i, step = 0;
         while (step < 13 && !is_cand13(w)) {
             w >>= 1;
@@ -346,6 +479,24 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         }
         s_next[i] = (uint8_t)step;
     }
+}
+
+// One wavefront per target.  limits[t] = the end of the target's file, bit 63
+// set for a segment target (any verified block start will do); scan_ends[t]
+// bounds the candidates.  Exact targets also take stops[t] (windows start
+// below it; null = no bound) and report exits[t], the chain position where
+// the walk ended; recs[4t..4t+4) (if not null) receives the chain positions
+// of the first 256 bits after the target.
+__global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
+                                                                const uint64_t* __restrict__ targets,
+                                                                const uint64_t* __restrict__ limits,
+                                                                const uint64_t* __restrict__ scan_ends,
+                                                                const uint64_t* __restrict__ stops,
+                                                                uint64_t* __restrict__ found, uint64_t* __restrict__ exits,
+                                                                uint64_t* __restrict__ recs, uint32_t n_targets) {
+    __shared__ uint8_t s_next[8192];
+    __shared__ FindWave s_wave[FIND_WAVES];
+    fill_next_lut(s_next);
     __syncthreads();
     const int lane = (int)(threadIdx.x & 63);
     const uint32_t t = blockIdx.x * FIND_WAVES + (threadIdx.x >> 6);
@@ -365,8 +516,30 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
     const uint64_t nbits = nbits_file;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
-    uint64_t weak = ~0ull;  // any_start: the first verified candidate with a weak header
     uint64_t rbase = ~0ull;
+    if (!any_start) {
+        ExactScan sc{};
+        sc.stop = stops ? uni64(stops[t]) : ~0ull;
+        sc.limit = limit;
+        sc.nbits = nbits;
+        sc.rec = recs != nullptr;
+        sc.rec_base = cur;
+        sc.mrg = false;
+        uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, merged;
+        result = exact_walk(in64, nvalid, fw, s_next, lane, rbase, cur, sc, r0, r1, r2, r3, &merged);
+        if (lane == 0) {
+            found[t] = result;
+            if (exits) exits[t] = cur;
+            if (recs) {
+                recs[4ull * t] = r0;
+                recs[4ull * t + 1] = r1;
+                recs[4ull * t + 2] = r2;
+                recs[4ull * t + 3] = r3;
+            }
+        }
+        return;
+    }
+    uint64_t weak = ~0ull;  // the first verified candidate with a weak header
     bool end = (cur + 13 > limit);
     while (!end) {
         const uint64_t base = cur;
@@ -384,107 +557,154 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
         const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
         const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
-        uint32_t step = 255;
         bool cand = false;
-        if (p + 13 <= limit) {
-            step = s_next[(uint32_t)(lo & 0x1fffu)];
-            if (step == 0) cand = precode_ok(lo, hi, p, nbits);
-        }
+        if (p + 13 <= limit && s_next[(uint32_t)(lo & 0x1fffu)] == 0) cand = precode_ok(lo, hi, p, nbits);
         const uint64_t cm = __ballot(cand);
-        if (any_start) {
-            // Stored blocks are block starts too (decoding from one is as good as
-            // from a dynamic header), and long stored stretches hold no dynamic
-            // block at all.  A byte-aligned non-final stored header (low 3 bits
-            // 0, LEN == ~NLEN) whose block is followed by another stored header
-            // is accepted without further checks (2^-32 per byte on random data).
-            bool st = false;
-            if ((p & 7) == 0 && p + 40 <= limit && (lo & 7u) == 0 && ((((lo >> 8) ^ (lo >> 24)) & 0xffffu) == 0xffffu)) {
-                const uint64_t nb = (p >> 3) + 5 + ((lo >> 8) & 0xffffu);  // next header byte
-                if ((nb + 5) * 8 <= nbits) {
-                    const uint32_t h = byte_at(in64, nb), l0 = byte_at(in64, nb + 1), l1 = byte_at(in64, nb + 2),
-                                   n0 = byte_at(in64, nb + 3), n1 = byte_at(in64, nb + 4);
-                    st = (h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu;
-                }
+        // Stored blocks are block starts too (decoding from one is as good as
+        // from a dynamic header), and long stored stretches hold no dynamic
+        // block at all.  A byte-aligned non-final stored header (low 3 bits
+        // 0, LEN == ~NLEN) whose block is followed by another stored header
+        // is accepted without further checks (2^-32 per byte on random data).
+        bool st = false;
+        if ((p & 7) == 0 && p + 40 <= limit && (lo & 7u) == 0 && ((((lo >> 8) ^ (lo >> 24)) & 0xffffu) == 0xffffu)) {
+            const uint64_t nb = (p >> 3) + 5 + ((lo >> 8) & 0xffffu);  // next header byte
+            if ((nb + 5) * 8 <= nbits) {
+                const uint32_t h = byte_at(in64, nb), l0 = byte_at(in64, nb + 1), l1 = byte_at(in64, nb + 2),
+                               n0 = byte_at(in64, nb + 3), n1 = byte_at(in64, nb + 4);
+                st = (h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu;
             }
-            const uint64_t sm = __ballot(st);
-            // candidates of this window in order; cand already implies p + 13 <= limit
-            uint64_t m = cm | sm;
-            while (m) {
-                const uint32_t rel = (uint32_t)__builtin_ctzll(m);
-                if ((sm >> rel) & 1ull) {
+        }
+        const uint64_t sm = __ballot(st);
+        // candidates of this window in order; cand already implies p + 13 <= limit
+        uint64_t m = cm | sm;
+        while (m) {
+            const uint32_t rel = (uint32_t)__builtin_ctzll(m);
+            if ((sm >> rel) & 1ull) {
+                result = base + rel;
+                end = true;
+                break;
+            }
+            uint32_t hd = 0;
+            if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane, &hd) ? 1u : 0u)) {
+                if (uni32(hd) > kWeakHdist) {
                     result = base + rel;
                     end = true;
                     break;
                 }
-                uint32_t hd = 0;
-                if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane, &hd) ? 1u : 0u)) {
-                    if (uni32(hd) > kWeakHdist) {
-                        result = base + rel;
-                        end = true;
-                        break;
-                    }
-                    if (weak == ~0ull) weak = base + rel;
-                }
-                m &= m - 1;
+                if (weak == ~0ull) weak = base + rel;
             }
-            cur = base + 64;
-            continue;
-        }
-        // The reference's skip chain (block_finder.rs:109-141) over this window,
-        // followed with jump tables: J[k][l] = the position 2^k chain steps after
-        // l (64 = left the window).  Whether a candidate lies on the chain, and
-        // where the chain leaves the window, then take six readlanes each
-        // instead of a walk of up to 64 dependent steps.
-        const uint32_t nxu = (step == 0) ? 1u : step;
-        uint32_t J[6];
-        J[0] = min((uint32_t)lane + nxu, 64u);
-#pragma unroll
-        for (int k = 1; k < 6; ++k) {
-            const uint32_t prev = J[k - 1];
-            const uint32_t nxt = (uint32_t)__shfl((int)prev, (int)min(prev, 63u), 64);
-            J[k] = prev >= 64u ? 64u : nxt;
-        }
-        const uint64_t room = limit - 13 - base;  // base + 13 <= limit here
-        uint32_t pth = 0;                           // last chain position visited
-        uint64_t m = cm;
-        while (m) {
-            const uint32_t c = (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
-#pragma unroll
-            for (int k = 5; k >= 0; --k) {
-                const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
-                if (q <= c) pth = q;
-            }
-            if (pth != c) continue;  // the chain jumps over this candidate
-            uint32_t hd = 0;
-            if (uni32(deep_verify(in64, fw, rbase, nbits, base + c, lane, &hd) ? 1u : 0u)) {
-                result = base + c;
-                end = true;
-                break;
-            }
         }
-        if (end) break;
-        if (room < 64) {  // every later chain position is past the limit
-            end = true;
-            break;
-        }
-#pragma unroll
-        for (int k = 5; k >= 0; --k) {
-            const uint32_t q = uni32((uint32_t)__builtin_amdgcn_readlane((int)J[k], (int)pth));
-            if (q < 64u) pth = q;
-        }
-        cur = base + pth + uni32((uint32_t)__builtin_amdgcn_readlane((int)nxu, (int)pth));
+        cur = base + 64;
     }
     if (result == ~0ull) result = weak;
-    if (lane == 0) found[t] = result;
+    if (lane == 0) {
+        found[t] = result;
+        if (exits) exits[t] = cur;
+    }
+}
+
+// Exact targets that a bounded first scan left open, continued in legs: leg
+// j of a target starts at a_j (leg 0 at the target's true chain position P)
+// and was walked by its own wavefront, recording its first verified
+// candidate, its exit and its first 256 chain positions.  The skip chain is a
+// function of the position, so two walks that share one position agree from
+// there on.  One wavefront per target walks the true chain from P into each
+// leg until it meets that leg's recorded positions (a few bits, usually),
+// checking the candidates on the way, and then takes over the leg's result.
+
+__global__ __launch_bounds__(64) void k_leg_fixup(const uint64_t* __restrict__ in64, uint64_t total_bits,
+                                                  const LegTarget* __restrict__ tg, const uint64_t* __restrict__ leg_a,
+                                                  const uint64_t* __restrict__ leg_found,
+                                                  const uint64_t* __restrict__ leg_exit,
+                                                  const uint64_t* __restrict__ leg_rec, uint64_t* __restrict__ out_found,
+                                                  uint64_t* __restrict__ out_next, uint32_t n) {
+    __shared__ uint8_t s_next[8192];
+    __shared__ FindWave fw;
+    fill_next_lut(s_next);
+    __syncthreads();
+    const uint32_t t = blockIdx.x;
+    if (t >= n) return;
+    const int lane = (int)threadIdx.x;
+    const LegTarget T = tg[t];
+    const uint64_t nvalid = (total_bits >> 6) + 2;
+    uint64_t rbase = ~0ull, P = uni64(T.p), result = ~0ull;
+    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    for (uint32_t j = 0; j < T.n_legs && P + 13 <= T.limit; ++j) {
+        const uint32_t k = T.first + j;
+        const uint64_t a = uni64(leg_a[k]);
+        const uint64_t leg_end = j + 1 < T.n_legs ? uni64(leg_a[k + 1]) : T.end;
+        const uint64_t lf = uni64(leg_found[k]);
+        if (P == a) {  // this leg's walk was the true chain
+            if (lf != ~0ull) {
+                result = lf;
+                break;
+            }
+            P = uni64(leg_exit[k]);
+            continue;
+        }
+        ExactScan sc{};
+        sc.stop = a + 256;
+        sc.limit = T.limit;
+        sc.nbits = T.limit;
+        sc.rec = false;
+        sc.mrg = true;
+        sc.mrg_base = a;
+        sc.m0 = uni64(leg_rec[4ull * k]);
+        sc.m1 = uni64(leg_rec[4ull * k + 1]);
+        sc.m2 = uni64(leg_rec[4ull * k + 2]);
+        sc.m3 = uni64(leg_rec[4ull * k + 3]);
+        uint64_t cur = P, merged;
+        uint64_t f = exact_walk(in64, nvalid, fw, s_next, lane, rbase, cur, sc, r0, r1, r2, r3, &merged);
+        if (f != ~0ull) {
+            result = f;
+            break;
+        }
+        if (merged != ~0ull) {
+            // from here the true chain is the leg's: its first verified
+            // candidate (at or after the merge position) or its exit
+            if (lf != ~0ull) {
+                result = lf;
+                break;
+            }
+            P = uni64(leg_exit[k]);
+            continue;
+        }
+        // no merge within the recorded positions: walk the rest of the leg
+        ExactScan sc2{};
+        sc2.stop = leg_end;
+        sc2.limit = T.limit;
+        sc2.nbits = T.limit;
+        f = exact_walk(in64, nvalid, fw, s_next, lane, rbase, cur, sc2, r0, r1, r2, r3, &merged);
+        if (f != ~0ull) {
+            result = f;
+            break;
+        }
+        P = cur;
+    }
+    if (lane == 0) {
+        out_found[t] = result;
+        out_next[t] = P;
+    }
 }
 
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
-                              const uint64_t* d_limits, const uint64_t* d_scan_ends, uint64_t* d_found,
-                              uint32_t n_targets, hipStream_t s) {
+                              const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
+                              uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
+                              hipStream_t s) {
     if (n_targets == 0) return hipSuccess;
     hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
-                       d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_found, n_targets);
+                       d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_stops, d_found, d_exits, d_recs,
+                       n_targets);
+    return hipGetLastError();
+}
+
+hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const LegTarget* d_tg, const uint64_t* d_leg_a,
+                            const uint64_t* d_leg_found, const uint64_t* d_leg_exit, const uint64_t* d_leg_rec,
+                            uint64_t* d_out_found, uint64_t* d_out_next, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leg_fixup, dim3(n), dim3(64), 0, s, d_in64, total_bits, d_tg, d_leg_a, d_leg_found,
+                       d_leg_exit, d_leg_rec, d_out_found, d_out_next, n);
     return hipGetLastError();
 }
 
@@ -828,26 +1048,55 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     out_n += length;
                 } else if (length) {
                     // o[out_n + t] = o[src + (t mod distance)]: every load reads a
-                    // cell written before its group (all of them for distance < 8;
-                    // earlier groups for distance >= 8)
+                    // cell written before this match, so the loads of a batch
+                    // never wait for the match's own stores.  A wait for loads
+                    // also waits for every earlier store (one in-order counter),
+                    // so after the first group of 8 (issued with the refill) a
+                    // long match goes on in batches of 32 cells: one wait per
+                    // batch instead of one per group.
                     const uint32_t src = out_n - distance;
                     uint32_t jm = 0;
-                    for (uint32_t k = 0; k < length; k += 8) {
+                    {
                         uint16_t v[8];
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             v[i] = o[src + jm];
                             jm = (jm + 1 == distance) ? 0u : jm + 1;
                         }
-                        if (k == 0) {
-                            bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
-                            bitcnt = 64;
-                            // complete the refill here, before the stores (and so
-                            // not as a full wait at the top of the next symbol)
-                            asm volatile("" : "+v"(bitbuf));
+                        bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
+                        bitcnt = 64;
+                        // complete the refill here, before the stores (and so
+                        // not as a full wait at the top of the next symbol)
+                        asm volatile("" : "+v"(bitbuf));
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) o[out_n + i] = v[i];
+                    }
+                    if (length <= 24) {
+                        // short matches (the common case): plain groups of 8
+                        for (uint32_t k = 8; k < length; k += 8) {
+                            uint16_t v[8];
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                v[i] = o[src + jm];
+                                jm = (jm + 1 == distance) ? 0u : jm + 1;
+                            }
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) o[out_n + k + i] = v[i];
+                        }
+                    } else
+                    for (uint32_t k = 8; k < length; k += 32) {
+                        uint16_t v[32];
+#pragma unroll
+                        for (int i = 0; i < 32; ++i) {
+                            v[i] = o[src + jm];
+                            jm = (jm + 1 == distance) ? 0u : jm + 1;
                         }
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) o[out_n + k + i] = v[i];
+                        for (int g = 0; g < 4; ++g)
+                            if (k + 8u * (uint32_t)g < length) {
+#pragma unroll
+                                for (int i = 0; i < 8; ++i) o[out_n + k + 8 * g + i] = v[8 * g + i];
+                            }
                     }
                     out_n += length;
                 } else {

@@ -14,10 +14,19 @@ namespace rrg {
 // visited in the reference's NEXT_LUT skip order (its exact choice); with bit
 // 63 every candidate is tried in bit order and verified stored-block headers
 // count as starts too (some block start).  One wavefront per target.
-// total_bits: size of the whole (padded) input buffer.
+// total_bits: size of the whole (padded) input buffer.  Exact targets may
+// take d_stops (windows start below it) and report d_exits (the chain position
+// where the walk ended) and d_recs (4 words per target: the chain positions of
+// the first 256 bits after the target); each may be null.
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
-                              const uint64_t* d_limits, const uint64_t* d_scan_ends, uint64_t* d_found,
-                              uint32_t n_targets, hipStream_t s);
+                              const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
+                              uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
+                              hipStream_t s);
+// Exact targets continued in legs: per target, the first verified candidate
+// on its chain over its legs (~0 if none) and the chain position after them.
+hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const LegTarget* d_tg, const uint64_t* d_leg_a,
+                            const uint64_t* d_leg_found, const uint64_t* d_leg_exit, const uint64_t* d_leg_rec,
+                            uint64_t* d_out_found, uint64_t* d_out_next, uint32_t n, hipStream_t s);
 
 // Persistent inflate over the segment indices in d_work[0..n_work).
 struct InflateGeom {
