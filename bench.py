@@ -174,6 +174,29 @@ def run_fastq(args, threads):
     result["cpu_baseline"] = None
     d_in.free()
     del raw
+    if not args.no_e2e:
+        # the same decode through the host-buffer entry point (rrg_gpu_decode):
+        # H2D of the file, decode, one D2H per chunk into a callback; never the value
+        class _Count:
+            n = 0
+            chunks = 0
+
+            def __call__(self, b):
+                self.n += len(b)
+                self.chunks += 1
+
+        hb = gz.tobytes()
+        rates = []
+        for _ in range(3):
+            sink = _Count()
+            t1 = time.perf_counter()
+            dec.decode_bytes(hb, sink, cfg)
+            rates.append(n / (time.perf_counter() - t1) / 1e9)
+            assert sink.n == n
+        result["host_to_host"] = {"value": round(sorted(rates)[1], 3), "unit": "GB/s",
+                                  "how": "rrg_gpu_decode: host file in, every chunk copied to a host callback "
+                                         "(PCIe both ways, pinned staging); median of 3"}
+        log(f"host to host {result['host_to_host']}")
 
     if not args.no_cpu:
         from oracle import oracle
@@ -430,6 +453,7 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
     ap.add_argument("--batch-files", type=int, default=64)
     ap.add_argument("--batch-mib", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=400)
