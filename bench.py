@@ -105,6 +105,12 @@ def run_fastq(args, threads):
     hipmem.synchronize()
     cfg = make_cfg(args)
     dec = rrg.Decoder(0)
+    tuned_ms = None
+    if args.autotune:
+        # untimed: the autotuner's pick replaces the command-line decomposition
+        cfg, tuned_ms = dec.autotune(d_in.ptr, C, gz, cfg, reps=1)
+        log(f"autotuned: lanes {cfg.lanes_per_block}, lut {cfg.lut_bits}, segment {cfg.segment_bytes} "
+            f"({tuned_ms:.1f} ms)")
 
     def step():
         return dec.decode_device(d_in.ptr, C, gz, cfg)
@@ -149,8 +155,10 @@ def run_fastq(args, threads):
         "compressed_bytes": C,
         "uncompressed_bytes": n,
         "segments": int(avg("n_segments")),
-        "lut_bits": args.lut_bits,
-        "lanes_per_block": args.lanes,
+        "lut_bits": cfg.lut_bits,
+        "lanes_per_block": cfg.lanes_per_block,
+        "segment_bytes": cfg.segment_bytes or "automatic",
+        "autotuned": tuned_ms is not None,
         "input": "resident in HBM",
     }
     result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
@@ -454,6 +462,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
+    ap.add_argument("--autotune", action="store_true", help="pick lanes / LUT width / segment size with the autotuner")
     ap.add_argument("--batch-files", type=int, default=64)
     ap.add_argument("--batch-mib", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=400)

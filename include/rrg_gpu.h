@@ -18,6 +18,7 @@
  *   rrg_chunk_cb           Sender::send of one in-order Arc<Vec<u8>>      pipeline.rs:921-937
  *   rrg_gpu_create/destroy the opaque per-worker handle pattern of
  *                          isal_shim.c:191-261 (rrg_isal_alloc/free)
+ *   rrg_gpu_autotune       no reference counterpart: picks the GPU decomposition
  *   rrg_gpu_decode_device  no reference counterpart: the device-resident
  *                          form (input and output in HBM) used to measure
  *                          the kernels without PCIe transfers
@@ -133,6 +134,17 @@ int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cf
  * end. */
 int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user, rrg_stats* stats,
                       rrg_error* err);
+
+/* Autotuner (no reference counterpart; the tuning the reference leaves to
+ * Config.num_threads).  Decodes the device-resident sample d_file (a complete
+ * gzip file) under each combination of lanes_per_block {8, 16}, lut_bits
+ * {9, 10} and segment_bytes {automatic, 40 KiB, 56 KiB}, on top of `base`
+ * (chunk_size_bytes and num_threads are kept), reps timed runs after one
+ * warm-up each, and returns the fastest configuration in *best and its time
+ * in *best_ms.  Every configuration must produce the same chunk boundaries
+ * (the output does not depend on them); a difference is RRG_E_DEVICE. */
+int rrg_gpu_autotune(void* ctx, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* base, int reps,
+                     rrg_cfg* best, double* best_ms, rrg_error* err);
 
 /* Device-resident form.  d_file: the gzip file in device memory.  h_file:
  * optional host copy (used only to walk BGZF member headers; NULL = copy the

@@ -137,6 +137,7 @@ EXPORTED_SYMBOLS = (
     "rrg_gpu_decode",
     "rrg_gpu_decode_device",
     "rrg_gpu_decode_fd",
+    "rrg_gpu_autotune",
     "rrg_gpu_last_profile",
     "rrg_gpu_decode_batch",
     "rrg_server_create",
@@ -162,6 +163,9 @@ def _lib() -> ctypes.CDLL:
                                         ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
         lib.rrg_gpu_decode.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(_Cfg), _CB,
                                        ctypes.c_void_p, ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
+        lib.rrg_gpu_autotune.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(_Cfg), ctypes.c_int,
+            ctypes.POINTER(_Cfg), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_Err)]
         lib.rrg_gpu_decode_device.argtypes = [
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(_Cfg),
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
@@ -442,6 +446,25 @@ class Decoder:
             _raise(rc, er)
         stats = DecodeStats(st.compressed_bytes, st.uncompressed_bytes, st.chunks_decoded, st.speculation_failures)
         return d_out.value or 0, out_len.value, [ends[i] for i in range(min(n.value, max_chunks))], stats
+
+    def autotune(self, d_ptr: int, length: int, host: bytes | None = None, config: Config | None = None,
+                 reps: int = 2) -> tuple[Config, float]:
+        """Pick lanes_per_block / lut_bits / segment_bytes for this GPU by decoding the
+        device-resident sample (a complete gzip file) under each combination
+        (rrg_gpu_autotune).  Returns (the fastest Config, its time in ms)."""
+        config = config or Config()
+        best = _Cfg()
+        ms = ctypes.c_double()
+        er = _Err()
+        hptr, _keep = _host_ptr(host)
+        rc = self._lib.rrg_gpu_autotune(self._ctx, ctypes.c_void_p(d_ptr), length, hptr, ctypes.byref(config._c()),
+                                        int(reps), ctypes.byref(best), ctypes.byref(ms), ctypes.byref(er))
+        if rc != RRG_OK:
+            _raise(rc, er)
+        tuned = Config(num_threads=best.num_threads, chunk_size_bytes=best.chunk_size_bytes,
+                       verbose=config.verbose, segment_bytes=best.segment_bytes, lut_bits=best.lut_bits,
+                       lanes_per_block=best.lanes_per_block)
+        return tuned, ms.value
 
     def decode_batch(self, files, config: Config | None = None, device_ptrs=None, device_lens=None,
                      copy: bool = True):

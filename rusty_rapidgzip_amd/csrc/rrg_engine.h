@@ -73,6 +73,7 @@ struct Ctx {
     rrg_profile prof{};
     bool verbose = false;  // Config.verbose of the current call: per-round lines on stderr
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evc[2] = {nullptr, nullptr};  // staging copies of the chunk emission
     ~Ctx() {
         DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
                          &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
@@ -81,6 +82,8 @@ struct Ctx {
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : evc)
+            if (e) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
     }
 };

@@ -1408,7 +1408,9 @@ bool init_ctx(Ctx& c, int device) {
     c.device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c.s) != hipSuccess ||
-        hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess)
+        hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evc[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess)
         return false;
     c.n_cu = prop.multiProcessorCount;
     return true;
@@ -1513,6 +1515,66 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
     return RRG_OK;
 }
 
+int rrg_gpu_autotune(void* ctx, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* base, int reps,
+                     rrg_cfg* best, double* best_ms, rrg_error* err) {
+    if (err) std::memset(err, 0, sizeof *err);
+    if (!ctx || !best) {
+        set_err(err, RRG_E_ARG, "null context or result");
+        return RRG_E_ARG;
+    }
+    rrg_cfg b;
+    if (base) b = *base;
+    else rrg_cfg_default(&b);
+    if (reps < 1) reps = 1;
+    // lanes per workgroup x literal LUT width x segment size (0 = automatic)
+    static const int kLanes[] = {8, 16};
+    static const int kLut[] = {9, 10};
+    static const size_t kSeg[] = {0, 40960, 57344};
+    std::vector<uint64_t> ref_ends, ends(1u << 20);
+    double best_t = 1e300;
+    bool first = true;
+    for (int lanes : kLanes)
+        for (int lut : kLut)
+            for (size_t seg : kSeg) {
+                rrg_cfg cfg = b;
+                cfg.lanes_per_block = lanes;
+                cfg.lut_bits = lut;
+                cfg.segment_bytes = seg;
+                cfg.verbose = 0;
+                double t = 1e300;
+                for (int r = 0; r <= reps; ++r) {
+                    const uint8_t* d_out = nullptr;
+                    uint64_t out_len = 0;
+                    size_t n = 0;
+                    const double t0 = now_ms();
+                    const int rc = rrg_gpu_decode_device(ctx, d_file, len, h_file, &cfg, &d_out, &out_len, ends.data(),
+                                                         ends.size(), &n, nullptr, err);
+                    const double dt = now_ms() - t0;
+                    if (rc) return rc;  // the sample must decode
+                    if (n > ends.size()) {
+                        set_err(err, RRG_E_ARG, "autotune: more than %zu chunks", ends.size());
+                        return RRG_E_ARG;
+                    }
+                    const std::vector<uint64_t> got(ends.begin(), ends.begin() + (long)n);
+                    if (first) {
+                        ref_ends = got;
+                        first = false;
+                    } else if (got != ref_ends) {
+                        set_err(err, RRG_E_DEVICE, "autotune: chunk boundaries differ between configurations");
+                        return RRG_E_DEVICE;
+                    }
+                    if (r) t = std::min(t, dt);  // the first run of each configuration warms up
+                }
+                if (t < best_t) {
+                    best_t = t;
+                    *best = cfg;
+                }
+            }
+    best->verbose = b.verbose;
+    if (best_ms) *best_ms = best_t;
+    return RRG_OK;
+}
+
 static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg_in, rrg_chunk_cb cb, void* user,
                        rrg_stats* stats, rrg_error* err, bool stream) {
     if (err) std::memset(err, 0, sizeof *err);
@@ -1548,38 +1610,51 @@ static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg
         biggest = std::max(biggest, e - prev);
         prev = e;
     }
-    if (biggest > c.pin_cap) {
+    prev = 0;
+    // two pinned halves: the copy of chunk k+1 runs while the callback takes chunk k
+    const size_t half = (biggest + 64 + 255) & ~(size_t)255;
+    if (2 * half > c.pin_cap) {
         if (c.pin) (void)hipHostFree(c.pin);
         c.pin = nullptr;
         c.pin_cap = 0;
-        HIPCHK(hipHostMalloc((void**)&c.pin, biggest + 64, hipHostMallocDefault));
-        c.pin_cap = biggest + 64;
+        HIPCHK(hipHostMalloc((void**)&c.pin, 2 * half, hipHostMallocDefault));
+        c.pin_cap = 2 * half;
     }
+    uint8_t* const buf[2] = {c.pin, c.pin + half};
     // a closed sink: the speculative path returns Ok counting the refused
     // chunk's bytes (pipeline.rs:1028-1030), the serial path without them
     // (pipeline.rs:1264-1266); the streaming serial path ignores it and
     // decodes on, counting every chunk (lib.rs:296-298)
     const bool serial = cfg.num_threads == 1;
     const bool stream_serial = stream && serial;
-    prev = 0;
+    const std::vector<uint64_t>& E = dec.chunk_ends;
+    auto start_of = [&](size_t k) -> uint64_t { return k ? E[k - 1] : 0; };
+    auto issue = [&](size_t k) -> hipError_t {
+        const uint64_t a = start_of(k), n = E[k] - a;
+        hipError_t e = hipSuccess;
+        if (n) e = hipMemcpyAsync(buf[k & 1], c.out.as<uint8_t>() + a, n, hipMemcpyDeviceToHost, c.s);
+        if (e == hipSuccess) e = hipEventRecord(c.evc[k & 1], c.s);
+        return e;
+    };
     bool closed = false;
-    for (uint64_t e : dec.chunk_ends) {
-        const uint64_t n = e - prev;
+    if (!E.empty()) HIPCHK(issue(0));
+    for (size_t k = 0; k < E.size(); ++k) {
+        const uint64_t n = E[k] - start_of(k);
         if (!closed) {
-            if (n) HIPCHK(hipMemcpyAsync(c.pin, c.out.as<uint8_t>() + prev, n, hipMemcpyDeviceToHost, c.s));
-            HIPCHK(hipStreamSynchronize(c.s));
-        }
-        prev = e;
-        if (!closed && cb && cb(user, c.pin, n)) {
-            closed = true;
-            if (!stream_serial) {
-                if (!serial) total += n;
-                break;
+            if (k + 1 < E.size()) HIPCHK(issue(k + 1));
+            HIPCHK(hipEventSynchronize(c.evc[k & 1]));
+            if (cb && cb(user, buf[k & 1], n)) {
+                closed = true;
+                if (!stream_serial) {
+                    if (!serial) total += n;
+                    break;
+                }
             }
         }
         total += n;
         ++sent;
     }
+    HIPCHK(hipStreamSynchronize(c.s));  // no copy into the staging buffer outlives the call
     c.prof.uncompressed_bytes = total;
     c.prof.ms_total = now_ms() - t0;
     if (cfg.verbose) print_profile(c);

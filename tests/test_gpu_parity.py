@@ -348,3 +348,26 @@ def test_streaming_pipe(decoder, tmp_path):
     with pytest.raises(rrg.GzipError) as ei:
         rrg.read_gz(empty, [], cfg)
     assert ei.value.kind == "Truncated"
+
+
+def test_autotune_picks_an_equivalent_configuration(decoder):
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import hipmem, synth
+
+    raw = synth.fastq(24 << 20, seed=5)
+    g = synth.gzip(raw, 6, piece=8 << 20)
+    d = hipmem.DeviceBuffer.from_host(g)
+    try:
+        base = rrg.Config(chunk_size_bytes=1 << 20)
+        _say("autotune on", g.size, "bytes")
+        tuned, ms = decoder.autotune(d.ptr, g.size, g, base, reps=1)
+        assert ms > 0
+        assert tuned.lanes_per_block in (8, 16) and tuned.lut_bits in (9, 10)
+        assert tuned.segment_bytes in (0, 40960, 57344)
+        assert tuned.chunk_size_bytes == base.chunk_size_bytes
+        ptr, n, ends, st = decoder.decode_device(d.ptr, g.size, g, tuned)
+        assert n == raw.size and np.array_equal(hipmem.to_host(ptr, n), raw)
+        _, _, ends0, st0 = decoder.decode_device(d.ptr, g.size, g, base)
+        assert ends == ends0 and st.speculation_failures == st0.speculation_failures
+    finally:
+        d.free()
