@@ -95,6 +95,7 @@ constexpr int REGION_WORDS = 66;
 
 struct FindWave {
     uint64_t reg[REGION_WORDS];
+    uint32_t cand[128];  // any-start scan: pending candidate offsets from rbase*64 (bit 31: stored header)
     uint8_t lens[320];
     uint8_t clut[128];     // precode LUT: (symbol << 3) | length
     uint16_t cnt[32];      // code-length counts: literal/length [0..16), distance [16..32)
@@ -540,12 +541,67 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         return;
     }
     uint64_t weak = ~0ull;  // the first verified candidate with a weak header
+    // Candidates are collected in bit order into a per-wave list and tested
+    // 64 at a time, one per lane: the precode test then runs once per 64
+    // candidates instead of once per 64-offset window (most windows hold a
+    // few offsets that pass the 13-bit test).  The list is emptied before the
+    // LDS region moves, so an offset always names bits of the current region.
+    uint32_t ncand = 0;
     bool end = (cur + 13 > limit);
+    auto flush = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t c0 = 0; c0 < ncand && !end; c0 += 64) {
+            const uint32_t k = c0 + (uint32_t)lane;
+            bool ok = false;
+            uint32_t e = 0;
+            if (k < ncand) {
+                e = fw.cand[k];
+                if (e >> 31) {
+                    ok = true;  // a verified stored header
+                } else {
+                    const uint32_t off = e;
+                    const uint32_t wi = off >> 6, sh = off & 63;
+                    const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
+                    const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
+                    const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
+                    ok = precode_ok(lo, hi, rbase * 64 + off, nbits);
+                }
+            }
+            uint64_t m = __ballot(ok);
+            const uint64_t stored = __ballot(ok && (e >> 31));
+            while (m) {
+                const uint32_t rel = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t er = uni32((uint32_t)__builtin_amdgcn_readlane((int)e, (int)rel));
+                const uint64_t pos = rbase * 64 + (er & 0x7fffffffu);
+                if ((stored >> rel) & 1ull) {
+                    result = pos;
+                    end = true;
+                    break;
+                }
+                uint32_t hd = 0;
+                if (uni32(deep_verify(in64, fw, rbase, nbits, pos, lane, &hd) ? 1u : 0u)) {
+                    if (uni32(hd) > kWeakHdist) {
+                        result = pos;
+                        end = true;
+                        break;
+                    }
+                    if (weak == ~0ull) weak = pos;
+                }
+            }
+        }
+        ncand = 0;
+    };
     while (!end) {
         const uint64_t base = cur;
         if (base + 13 > limit) break;  // the reference's `while pos + 13 <= limit`
         // the window needs bits [base, base + 63 + 128)
         if (rbase == ~0ull || (base >> 6) + 3 >= rbase + REGION_WORDS) {
+            if (ncand) {
+                flush();
+                if (end) break;
+            }
             rbase = base >> 6;
             for (int k = lane; k < REGION_WORDS; k += 64) fw.reg[k] = (rbase + k < nvalid) ? in64[rbase + k] : 0ull;
             __builtin_amdgcn_wave_barrier();
@@ -554,12 +610,11 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         const uint64_t p = base + (uint64_t)lane;
         const uint32_t wi = (uint32_t)((p >> 6) - rbase);
         const uint32_t sh = (uint32_t)(p & 63);
-        const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
+        const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1];
         const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
-        const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
-        bool cand = false;
-        if (p + 13 <= limit && s_next[(uint32_t)(lo & 0x1fffu)] == 0) cand = precode_ok(lo, hi, p, nbits);
-        const uint64_t cm = __ballot(cand);
+        // every offset is a candidate here, so the 13-bit test is done in
+        // registers (NEXT_LUT[w] == 0 is exactly is_cand13(w))
+        const bool c13 = p + 13 <= limit && is_cand13((uint32_t)lo);
         // Stored blocks are block starts too (decoding from one is as good as
         // from a dynamic header), and long stored stretches hold no dynamic
         // block at all.  A byte-aligned non-final stored header (low 3 bits
@@ -574,29 +629,17 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
                 st = (h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu;
             }
         }
-        const uint64_t sm = __ballot(st);
-        // candidates of this window in order; cand already implies p + 13 <= limit
-        uint64_t m = cm | sm;
-        while (m) {
-            const uint32_t rel = (uint32_t)__builtin_ctzll(m);
-            if ((sm >> rel) & 1ull) {
-                result = base + rel;
-                end = true;
-                break;
-            }
-            uint32_t hd = 0;
-            if (uni32(deep_verify(in64, fw, rbase, nbits, base + rel, lane, &hd) ? 1u : 0u)) {
-                if (uni32(hd) > kWeakHdist) {
-                    result = base + rel;
-                    end = true;
-                    break;
-                }
-                if (weak == ~0ull) weak = base + rel;
-            }
-            m &= m - 1;
+        const bool take = c13 || st;
+        const uint64_t tm = __ballot(take);
+        if (take) {
+            const uint32_t idx = (uint32_t)__popcll(tm & ((1ull << lane) - 1ull));
+            fw.cand[ncand + idx] = (uint32_t)(p - rbase * 64) | (st ? 0x80000000u : 0u);
         }
+        ncand += (uint32_t)__popcll(tm);
+        if (ncand >= 64) flush();
         cur = base + 64;
     }
+    if (!end && ncand) flush();
     if (result == ~0ull) result = weak;
     if (lane == 0) {
         found[t] = result;
