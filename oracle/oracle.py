@@ -75,6 +75,8 @@ def lib() -> ctypes.CDLL:
         L.orc_parse_bgzf_block_size.restype = ctypes.c_uint32
         L.orc_find_next_dynamic_block.argtypes = [vp, sz, ctypes.c_uint64, ctypes.c_uint64,
                                                   ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_strict_header_ok.argtypes = [vp, sz, ctypes.c_uint64]
+        L.orc_strict_header_ok.restype = ctypes.c_int
         L.orc_next_lut.argtypes = [ctypes.c_uint32]
         L.orc_next_lut.restype = ctypes.c_uint8
         L.orc_safe_inflate.argtypes = [vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]
@@ -120,6 +122,12 @@ def find_next_dynamic_block(data, from_bit: int, until_bit: int | None = None):
     out = ctypes.c_uint64()
     ub = a.size * 8 if until_bit is None else until_bit
     return out.value if lib().orc_find_next_dynamic_block(p, a.size, from_bit, ub, ctypes.byref(out)) else None
+
+
+def strict_header_ok(data, pos: int) -> bool:
+    """block_finder.rs:196-262: the strict dynamic header parse at bit `pos`."""
+    a, p = _buf(data)
+    return bool(lib().orc_strict_header_ok(p, a.size, pos))
 
 
 def next_lut(w: int) -> int:

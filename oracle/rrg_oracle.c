@@ -364,6 +364,18 @@ static int verify_candidate(const uint8_t* data, size_t len, uint64_t pos, Huff*
     return trial_decode(&b, &pool[1], &pool[2]);
 }
 
+/* block_finder.rs:196-262: the strict dynamic header alone (BFINAL/BTYPE at
+ * `pos` are skipped, not checked).  1 = the header parses in strict mode. */
+int orc_strict_header_ok(const uint8_t* data, size_t len, uint64_t pos) {
+    Huff* pool = (Huff*)malloc(3 * sizeof(Huff));
+    BR b = {data, len, (uint64_t)len * 8, pos};
+    uint32_t v;
+    const char* msg;
+    int ok = !br_read(&b, 3, &v) && !read_dynamic_header(&b, &pool[0], &pool[1], &pool[2], 1, &msg);
+    free(pool);
+    return ok;
+}
+
 /* block_finder.rs:109-141 */
 int orc_find_next_dynamic_block(const uint8_t* data, size_t len, uint64_t from_bit, uint64_t until_bit,
                                 uint64_t* out) {

@@ -57,6 +57,9 @@ uint32_t orc_parse_bgzf_block_size(const uint8_t* p, size_t n);
 /* block_finder.rs:109-141.  Returns 1 and sets *out when found. */
 int orc_find_next_dynamic_block(const uint8_t* data, size_t len, uint64_t from_bit,
                                 uint64_t until_bit, uint64_t* out);
+/* block_finder.rs:196-262: 1 when the bits at `pos` (a block's BFINAL bit)
+ * hold a dynamic header that parses in the finder's strict mode. */
+int orc_strict_header_ok(const uint8_t* data, size_t len, uint64_t pos);
 /* block_finder.rs:57-101 prefilter table (13-bit window -> skip). */
 uint8_t orc_next_lut(uint32_t window13);
 

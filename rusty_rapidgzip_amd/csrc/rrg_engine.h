@@ -63,6 +63,8 @@ struct DevBuf {
 struct Ctx {
     int device = 0;
     hipStream_t s = nullptr;
+    hipStream_t s2 = nullptr;  // a second stream (default priority) for work that overlaps s
+    hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
         pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext;
@@ -84,6 +86,9 @@ struct Ctx {
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : evc)
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : evf)
+            if (e) (void)hipEventDestroy(e);
+        if (s2) (void)hipStreamDestroy(s2);
         if (s) (void)hipStreamDestroy(s);
     }
 };

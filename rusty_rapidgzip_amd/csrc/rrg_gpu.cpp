@@ -819,7 +819,7 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
                               c.limits.as<uint64_t>(), c.scan_ends.as<uint64_t>(),
                               stops ? c.stops.as<uint64_t>() : nullptr, c.found.as<uint64_t>(),
                               exits ? c.exits.as<uint64_t>() : nullptr, want_recs ? c.recs.as<uint64_t>() : nullptr,
-                              (uint32_t)n, c.s));
+                              (uint32_t)n, c.s, c.s2, c.evf[0], c.evf[1]));
     HIPCHK(hipMemcpyAsync(found.data(), c.found.p, n * 8, hipMemcpyDeviceToHost, c.s));
     if (exits) {
         exits->assign(n, 0);
@@ -1410,7 +1410,9 @@ bool init_ctx(Ctx& c, int device) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c.s) != hipSuccess ||
         hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess || hipStreamCreate(&c.s2) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evf[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evf[1], hipEventDisableTiming) != hipSuccess)
         return false;
     c.n_cu = prop.multiProcessorCount;
     return true;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rrg_device.h"
+#include "rrg_hdr.h"
 #include "rrg_launch.h"
 
 namespace rrg {
@@ -95,7 +96,6 @@ constexpr int REGION_WORDS = 66;
 
 struct FindWave {
     uint64_t reg[REGION_WORDS];
-    uint32_t cand[128];  // any-start scan: pending candidate offsets from rbase*64 (bit 31: stored header)
     uint8_t lens[320];
     uint8_t clut[128];     // precode LUT: (symbol << 3) | length
     uint16_t cnt[32];      // code-length counts: literal/length [0..16), distance [16..32)
@@ -482,12 +482,12 @@ __device__ void fill_next_lut(uint8_t* s_next) {
     }
 }
 
-// One wavefront per target.  limits[t] = the end of the target's file, bit 63
-// set for a segment target (any verified block start will do); scan_ends[t]
-// bounds the candidates.  Exact targets also take stops[t] (windows start
-// below it; null = no bound) and report exits[t], the chain position where
-// the walk ended; recs[4t..4t+4) (if not null) receives the chain positions
-// of the first 256 bits after the target.
+// One wavefront per exact target (limits[t] bit 63 clear; segment targets,
+// bit 63 set, are k_find_any's).  limits[t] = the end of the target's file;
+// scan_ends[t] bounds the candidates, stops[t] the windows (null = no
+// bound).  Reports found[t], exits[t] (the chain position where the walk
+// ended) and, if recs is not null, recs[4t..4t+4) = the chain positions of
+// the first 256 bits after the target.
 __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t* __restrict__ in64, uint64_t total_bits,
                                                                 const uint64_t* __restrict__ targets,
                                                                 const uint64_t* __restrict__ limits,
@@ -497,28 +497,24 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
                                                                 uint64_t* __restrict__ recs, uint32_t n_targets) {
     __shared__ uint8_t s_next[8192];
     __shared__ FindWave s_wave[FIND_WAVES];
+    const uint32_t t = blockIdx.x * FIND_WAVES + (threadIdx.x >> 6);
+    const bool mine = t < n_targets && (limits[t] >> 63) == 0;
+    if (!__syncthreads_or(mine)) return;  // no exact target in this workgroup
     fill_next_lut(s_next);
     __syncthreads();
     const int lane = (int)(threadIdx.x & 63);
-    const uint32_t t = blockIdx.x * FIND_WAVES + (threadIdx.x >> 6);
-    if (t >= n_targets) return;
+    if (!mine) return;
     FindWave& fw = s_wave[threadIdx.x >> 6];
     uint64_t cur = uni64(targets[t]);
     // search limit: the end of the target's file (several files may share the
-    // buffer).  Bit 63 set: any verified block start will do -- the candidates
-    // are tried in bit order without the reference's NEXT_LUT skip chain,
-    // which is only needed where the exact candidate it picks matters.
-    const uint64_t lim_word = uni64(limits[t]);
-    const bool any_start = (lim_word >> 63) != 0;
-    // candidates lie below `limit` (where the scan stops); verification may
-    // read up to `nbits`, the end of the file
-    const uint64_t nbits_file = lim_word & ~(1ull << 63);
-    const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits_file) : nbits_file;
-    const uint64_t nbits = nbits_file;
+    // buffer); candidates lie below `limit` (where the scan stops) and
+    // verification may read up to `nbits`, the end of the file
+    const uint64_t nbits = uni64(limits[t]);
+    const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits) : nbits;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     uint64_t result = ~0ull;
     uint64_t rbase = ~0ull;
-    if (!any_start) {
+    {
         ExactScan sc{};
         sc.stop = stops ? uni64(stops[t]) : ~0ull;
         sc.limit = limit;
@@ -540,48 +536,78 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
         }
         return;
     }
-    uint64_t weak = ~0ull;  // the first verified candidate with a weak header
-    // Candidates are collected in bit order into a per-wave list and tested
-    // 64 at a time, one per lane: the precode test then runs once per 64
-    // candidates instead of once per 64-offset window (most windows hold a
-    // few offsets that pass the 13-bit test).  The list is emptied before the
-    // LDS region moves, so an offset always names bits of the current region.
-    uint32_t ncand = 0;
-    bool end = (cur + 13 > limit);
-    auto flush = [&]() {
+}
+
+// Segment targets (limits[t] bit 63 set): any verified block start will do,
+// so the candidates are tried in bit order without the reference's NEXT_LUT
+// skip chain, which is only needed where the exact candidate it picks
+// matters (k_find_blocks).  One wavefront per workgroup, so a wave that
+// finishes early frees its slot at once.
+//
+// The scan runs in rounds of 4096 positions: lane l takes the 64 offsets of
+// word w0 + l.  The 13-bit test runs bit-parallel on the lane's own words:
+// bit j of X_k is bit k of offset j, and is_cand13 = !b0 & !b1 & b2 &
+// !(b4 & b5 & b6 & b7) & !(b9 & b10 & b11 & b12).  Each lane then runs the
+// precode test on its own candidates; the survivors (about one in 2000
+// positions) are listed in bit order and, kPendBatch at a time, get the
+// strict header parse one per lane; only what passes that gets the wave's
+// cooperative verification, in bit order.  The scan therefore runs ahead of
+// its find by at most kPendBatch precode survivors.
+struct AnyWave : FindWave {
+    uint64_t pend[128];  // precode survivors, bit order (bit 63: verified stored header)
+};
+
+__device__ __forceinline__ uint64_t zero_bytes(uint64_t v) {  // high bit of each zero byte (and maybe of bytes above one)
+    return (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
+}
+
+__global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in64, uint64_t total_bits,
+                                                 const uint64_t* __restrict__ targets,
+                                                 const uint64_t* __restrict__ limits,
+                                                 const uint64_t* __restrict__ scan_ends, uint64_t* __restrict__ found,
+                                                 uint64_t* __restrict__ exits, uint32_t n_targets) {
+    __shared__ AnyWave fw;
+    const uint32_t t = blockIdx.x;
+    if (t >= n_targets) return;
+    const uint64_t lim_word = uni64(limits[t]);
+    if ((lim_word >> 63) == 0) return;  // an exact target
+    const int lane = (int)threadIdx.x;
+    const uint64_t nbits = lim_word & ~(1ull << 63);
+    const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits) : nbits;
+    const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
+    const uint64_t first = uni64(targets[t]);
+    constexpr uint32_t kPendBatch = 32, kPendCap = 128;
+    constexpr uint64_t kNoRegion = ~0ull;  // deep_verify reads global memory
+    uint64_t result = ~0ull, weak = ~0ull;
+    uint32_t npend = 0;
+    bool end = false;
+    auto fetch = [&](uint64_t q) __attribute__((always_inline)) -> uint64_t { return q < nvalid ? in64[q] : 0ull; };
+    auto flush2 = [&]() __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (uint32_t c0 = 0; c0 < ncand && !end; c0 += 64) {
+        for (uint32_t c0 = 0; c0 < npend && !end; c0 += 64) {
             const uint32_t k = c0 + (uint32_t)lane;
             bool ok = false;
-            uint32_t e = 0;
-            if (k < ncand) {
-                e = fw.cand[k];
-                if (e >> 31) {
-                    ok = true;  // a verified stored header
-                } else {
-                    const uint32_t off = e;
-                    const uint32_t wi = off >> 6, sh = off & 63;
-                    const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1], a2 = fw.reg[wi + 2];
-                    const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
-                    const uint64_t hi = sh ? (a1 >> sh) | (a2 << (64 - sh)) : a1;
-                    ok = precode_ok(lo, hi, rbase * 64 + off, nbits);
-                }
+            uint64_t e = 0;
+            if (k < npend) {
+                e = fw.pend[k];
+                ok = (e >> 63) != 0 || strict_header_lane(fetch, nbits, e);
             }
             uint64_t m = __ballot(ok);
-            const uint64_t stored = __ballot(ok && (e >> 31));
+            const uint64_t stored = __ballot(ok && (e >> 63));
             while (m) {
                 const uint32_t rel = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1;
-                const uint32_t er = uni32((uint32_t)__builtin_amdgcn_readlane((int)e, (int)rel));
-                const uint64_t pos = rbase * 64 + (er & 0x7fffffffu);
+                const uint32_t lo = uni32((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e, (int)rel));
+                const uint32_t hi = uni32((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e >> 32), (int)rel));
+                const uint64_t pos = ((uint64_t)(hi & 0x7fffffffu) << 32) | lo;
                 if ((stored >> rel) & 1ull) {
                     result = pos;
                     end = true;
                     break;
                 }
                 uint32_t hd = 0;
-                if (uni32(deep_verify(in64, fw, rbase, nbits, pos, lane, &hd) ? 1u : 0u)) {
+                if (uni32(deep_verify(in64, fw, kNoRegion, nbits, pos, lane, &hd) ? 1u : 0u)) {
                     if (uni32(hd) > kWeakHdist) {
                         result = pos;
                         end = true;
@@ -591,59 +617,95 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
                 }
             }
         }
-        ncand = 0;
+        npend = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     };
-    while (!end) {
-        const uint64_t base = cur;
-        if (base + 13 > limit) break;  // the reference's `while pos + 13 <= limit`
-        // the window needs bits [base, base + 63 + 128)
-        if (rbase == ~0ull || (base >> 6) + 3 >= rbase + REGION_WORDS) {
-            if (ncand) {
-                flush();
-                if (end) break;
-            }
-            rbase = base >> 6;
-            for (int k = lane; k < REGION_WORDS; k += 64) fw.reg[k] = (rbase + k < nvalid) ? in64[rbase + k] : 0ull;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
-        const uint64_t p = base + (uint64_t)lane;
-        const uint32_t wi = (uint32_t)((p >> 6) - rbase);
-        const uint32_t sh = (uint32_t)(p & 63);
-        const uint64_t a0 = fw.reg[wi], a1 = fw.reg[wi + 1];
-        const uint64_t lo = sh ? (a0 >> sh) | (a1 << (64 - sh)) : a0;
-        // every offset is a candidate here, so the 13-bit test is done in
-        // registers (NEXT_LUT[w] == 0 is exactly is_cand13(w))
-        const bool c13 = p + 13 <= limit && is_cand13((uint32_t)lo);
+    // positions p of a round: first <= p and p + 13 <= limit
+    uint64_t w0 = first >> 6;
+    uint64_t N0 = fetch(w0 + lane), N1 = fetch(w0 + lane + 1), N2 = fetch(w0 + lane + 2);
+    while (!end && w0 * 64 + 13 <= limit) {
+        const uint64_t W0 = N0, W1 = N1, W2 = N2;
+        // the next round's words, in flight during this one
+        N0 = fetch(w0 + 64 + lane);
+        N1 = fetch(w0 + 65 + lane);
+        N2 = fetch(w0 + 66 + lane);
+        const uint64_t wb = (w0 + (uint64_t)lane) * 64;  // this lane's first offset
+        auto X = [&](uint32_t k) __attribute__((always_inline)) -> uint64_t { return (W0 >> k) | (W1 << (64 - k)); };
+        uint64_t cm = X(2) & ~(W0 | X(1) | (X(4) & X(5) & X(6) & X(7)) | (X(9) & X(10) & X(11) & X(12)));
+        uint64_t valid = ~0ull;
+        if (wb < first) valid = first - wb < 64 ? ~0ull << (first - wb) : 0ull;
+        if (wb + 13 > limit)
+            valid = 0;
+        else if (limit - 13 - wb < 63)
+            valid &= (2ull << (limit - 13 - wb)) - 1ull;
+        cm &= valid;
         // Stored blocks are block starts too (decoding from one is as good as
         // from a dynamic header), and long stored stretches hold no dynamic
         // block at all.  A byte-aligned non-final stored header (low 3 bits
         // 0, LEN == ~NLEN) whose block is followed by another stored header
         // is accepted without further checks (2^-32 per byte on random data).
-        bool st = false;
-        if ((p & 7) == 0 && p + 40 <= limit && (lo & 7u) == 0 && ((((lo >> 8) ^ (lo >> 24)) & 0xffffu) == 0xffffu)) {
-            const uint64_t nb = (p >> 3) + 5 + ((lo >> 8) & 0xffffu);  // next header byte
-            if ((nb + 5) * 8 <= nbits) {
+        // Bytes whose LEN/NLEN agree are found bit-parallel first.
+        uint64_t sm = 0;
+        {
+            const uint64_t a_lo = (W0 >> 8) | (W1 << 56), a_hi = W1 >> 8;    // bytes k + 1
+            const uint64_t b_lo = (W0 >> 24) | (W1 << 40), b_hi = W1 >> 24;  // bytes k + 3
+            const uint64_t t_lo = a_lo ^ b_lo, t_hi = a_hi ^ b_hi;
+            const uint64_t u = t_lo & ((t_lo >> 8) | (t_hi << 56));  // byte k: (b[k+1]^b[k+3]) & (b[k+2]^b[k+4])
+            uint64_t hits = zero_bytes(~u) & zero_bytes(W0 & 0x0707070707070707ull);
+            while (hits) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(hits) - 7;  // the byte's bit offset
+                hits &= hits - 1;
+                const uint64_t p = wb + j;
+                if (!((valid >> j) & 1ull) || p + 40 > limit) continue;
+                const uint64_t v = (W0 >> j) | (j ? W1 << (64 - j) : 0ull);
+                if ((v & 7u) != 0 || (((v >> 8) ^ (v >> 24)) & 0xffffu) != 0xffffu) continue;
+                const uint64_t nb = (p >> 3) + 5 + ((v >> 8) & 0xffffu);  // next header byte
+                if ((nb + 5) * 8 > nbits) continue;
                 const uint32_t h = byte_at(in64, nb), l0 = byte_at(in64, nb + 1), l1 = byte_at(in64, nb + 2),
                                n0 = byte_at(in64, nb + 3), n1 = byte_at(in64, nb + 4);
-                st = (h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu;
+                if ((h & 6u) == 0 && ((l0 | (l1 << 8)) ^ (n0 | (n1 << 8))) == 0xffffu) sm |= 1ull << j;
             }
         }
-        const bool take = c13 || st;
-        const uint64_t tm = __ballot(take);
-        if (take) {
-            const uint32_t idx = (uint32_t)__popcll(tm & ((1ull << lane) - 1ull));
-            fw.cand[ncand + idx] = (uint32_t)(p - rbase * 64) | (st ? 0x80000000u : 0u);
+        // the precode test of this lane's candidates (bits [p, p + 128) are
+        // in W0..W2); a verified stored header passes as it is
+        uint64_t pm = sm;
+        for (uint64_t m = cm & ~sm; m; m &= m - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(m);
+            const uint64_t lo = j ? (W0 >> j) | (W1 << (64 - j)) : W0;
+            const uint64_t hi = j ? (W1 >> j) | (W2 << (64 - j)) : W1;
+            if (precode_ok(lo, hi, wb + j, nbits)) pm |= 1ull << j;
         }
-        ncand += (uint32_t)__popcll(tm);
-        if (ncand >= 64) flush();
-        cur = base + 64;
+        // list the survivors in bit order (lane order, then offset order),
+        // at most kPendCap at a time
+        const uint32_t mine = (uint32_t)__popcll(pm);
+        uint32_t before = mine;  // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)before, d, 64);
+            if (lane >= d) before += o;
+        }
+        const uint32_t total = uni32((uint32_t)__shfl((int)before, 63, 64));
+        before -= mine;
+        for (uint32_t done = 0; done < total && !end;) {
+            const uint32_t take = min(total - done, kPendCap - npend);
+            uint32_t r = before;
+            for (uint64_t m = pm; m; m &= m - 1, ++r) {
+                if (r < done || r >= done + take) continue;
+                const uint32_t j = (uint32_t)__builtin_ctzll(m);
+                fw.pend[npend + (r - done)] = (wb + j) | (((sm >> j) & 1ull) << 63);
+            }
+            npend += take;
+            done += take;
+            if (npend >= kPendBatch) flush2();
+        }
+        w0 += 64;
     }
-    if (!end && ncand) flush();
+    if (!end && npend) flush2();
     if (result == ~0ull) result = weak;
     if (lane == 0) {
         found[t] = result;
-        if (exits) exits[t] = cur;
+        if (exits) exits[t] = min(w0 * 64, limit);
     }
 }
 
@@ -734,12 +796,29 @@ __global__ __launch_bounds__(64) void k_leg_fixup(const uint64_t* __restrict__ i
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
                               const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
                               uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
-                              hipStream_t s) {
+                              hipStream_t s, hipStream_t s_seg, hipEvent_t fork, hipEvent_t join) {
     if (n_targets == 0) return hipSuccess;
+    hipError_t e;
+    // the exact walks go first: they are few but long, and their workgroups
+    // must not queue behind the many segment scans
+    hipStream_t ss = s;
+    if (s_seg) {
+        if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
+        ss = s_seg;
+    }
     hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
                        d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_stops, d_found, d_exits, d_recs,
                        n_targets);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (s_seg && (e = hipStreamWaitEvent(s_seg, fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_find_any, dim3(n_targets), dim3(64), 0, ss, d_in64, total_bits, d_targets, d_limits,
+                       d_scan_ends, d_found, d_exits, n_targets);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (s_seg) {
+        if ((e = hipEventRecord(join, s_seg)) != hipSuccess || (e = hipStreamWaitEvent(s, join, 0)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const LegTarget* d_tg, const uint64_t* d_leg_a,

@@ -17,11 +17,14 @@ namespace rrg {
 // total_bits: size of the whole (padded) input buffer.  Exact targets may
 // take d_stops (windows start below it) and report d_exits (the chain position
 // where the walk ended) and d_recs (4 words per target: the chain positions of
-// the first 256 bits after the target); each may be null.
+// the first 256 bits after the target); each may be null.  Exact targets
+// are walked on stream s; with s_seg non-null the segment targets are
+// scanned concurrently on s_seg (forked from and joined back into s with the
+// two events), else on s after them.
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
                               const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
                               uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
-                              hipStream_t s);
+                              hipStream_t s, hipStream_t s_seg, hipEvent_t fork, hipEvent_t join);
 // Exact targets continued in legs: per target, the first verified candidate
 // on its chain over its legs (~0 if none) and the chain position after them.
 hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const LegTarget* d_tg, const uint64_t* d_leg_a,

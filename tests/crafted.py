@@ -97,3 +97,66 @@ def reach_before_member(a_len, z_len, y_ref, y_new, seed=1):
     m2 = HEADER + sz + sy + trailer(Z + Y)
     y_at = len(m1) + len(HEADER) + len(sz)
     return m1 + m2, A + Z + Y, y_at, (len(A), len(Z), len(Y))
+
+
+CL_ORDER = (16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15)
+
+
+class BitWriter:
+    """Bits LSB-first, as DEFLATE packs them (RFC 1951 3.1.1)."""
+
+    def __init__(self, lead=0):
+        self.bits = [0] * lead
+
+    def put(self, value, n):
+        self.bits += [(value >> k) & 1 for k in range(n)]
+
+    def put_code(self, code, n):  # Huffman codes go most significant bit first
+        self.bits += [(code >> (n - 1 - k)) & 1 for k in range(n)]
+
+    def tobytes(self, pad_bits=0):
+        bits = self.bits + [0] * pad_bits
+        bits += [0] * (-len(bits) % 8)
+        return bytes(sum(b << k for k, b in enumerate(bits[i:i + 8])) for i in range(0, len(bits), 8))
+
+
+def canonical_codes(lengths):
+    """RFC 1951 3.2.2 codes for a list of code lengths (0 = unused)."""
+    count = [0] * 16
+    for L in lengths:
+        if L:
+            count[L] += 1
+    nxt, code = [0] * 16, 0
+    for L in range(1, 16):
+        code = (code + count[L - 1]) << 1 if L > 1 else 0
+        nxt[L] = code
+    out = []
+    for L in lengths:
+        out.append(nxt[L] if L else None)
+        if L:
+            nxt[L] += 1
+    return out
+
+
+def dynamic_header(precode, seq, hlit, hdist, hclen=19, lead=0, bfinal=0):
+    """A dynamic block header: `precode` = 19 code-length-code lengths in
+    symbol order, `seq` = (code-length symbol, extra bits value) pairs.
+    Returns (BitWriter, number of header bits after the lead)."""
+    w = BitWriter(lead)
+    w.put(bfinal, 1)
+    w.put(2, 2)
+    w.put(hlit - 257, 5)
+    w.put(hdist - 1, 5)
+    w.put(hclen - 4, 4)
+    for k in range(hclen):
+        w.put(precode[CL_ORDER[k]], 3)
+    codes = canonical_codes(precode)
+    for sym, extra in seq:
+        w.put_code(codes[sym], precode[sym])
+        if sym == 16:
+            w.put(extra, 2)
+        elif sym == 17:
+            w.put(extra, 3)
+        elif sym == 18:
+            w.put(extra, 7)
+    return w, len(w.bits) - lead
