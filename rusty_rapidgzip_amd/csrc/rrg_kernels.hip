@@ -709,6 +709,153 @@ __global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in
     }
 }
 
+// Exact targets without chain records (first walks): the reference's skip
+// chain (block_finder.rs:109-141) in rounds of 4096 positions, lane l
+// walking the chain through word w0 + l by NEXT_LUT steps.  A lane's entry
+// is the previous lane's exit; the lanes first walk from guessed entries (the
+// start of their word) and walk again from the previous lane's exit until no
+// entry changes -- chains from nearby entries merge within a few steps, so
+// this takes two or three walks, and the fixed point is the true chain (lane
+// 0 enters at the true position, and each lane's entry equals its
+// predecessor's exit).  The chain positions with a jump of 0 are the
+// candidates; each lane runs the precode test on its own, the survivors are
+// listed in chain order and, kPendBatch at a time, get the strict header
+// parse one per lane and then the wave's verification in order: the first
+// that verifies is the find.  The walk ends where the reference's loop would
+// (p + 13 > limit) or at the first chain position at or past stops[t];
+// exits[t] is that position.
+__global__ __launch_bounds__(64) void k_find_exact(const uint64_t* __restrict__ in64, uint64_t total_bits,
+                                                   const uint64_t* __restrict__ targets,
+                                                   const uint64_t* __restrict__ limits,
+                                                   const uint64_t* __restrict__ scan_ends,
+                                                   const uint64_t* __restrict__ stops, uint64_t* __restrict__ found,
+                                                   uint64_t* __restrict__ exits, uint32_t n_targets) {
+    __shared__ uint8_t s_next[8192];
+    __shared__ AnyWave fw;
+    const uint32_t t = blockIdx.x;
+    if (t >= n_targets) return;
+    const uint64_t nbits = uni64(limits[t]);
+    if (nbits >> 63) return;  // a segment target
+    fill_next_lut(s_next);
+    __syncthreads();
+    const int lane = (int)threadIdx.x;
+    const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits) : nbits;
+    const uint64_t stop = stops ? uni64(stops[t]) : ~0ull;
+    const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
+    constexpr uint32_t kPendBatch = 32, kPendCap = 128;
+    constexpr uint64_t kNoRegion = ~0ull;  // deep_verify reads global memory
+    auto fetch = [&](uint64_t q) __attribute__((always_inline)) -> uint64_t { return q < nvalid ? in64[q] : 0ull; };
+    uint64_t result = ~0ull;
+    uint32_t npend = 0;
+    bool end = false;
+    auto flush = [&]() __attribute__((always_inline)) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t c0 = 0; c0 < npend && !end; c0 += 64) {
+            const uint32_t k = c0 + (uint32_t)lane;
+            bool ok = false;
+            uint64_t e = 0;
+            if (k < npend) {
+                e = fw.pend[k];
+                ok = strict_header_lane(fetch, nbits, e);
+            }
+            uint64_t m = __ballot(ok);
+            while (m) {
+                const uint32_t rel = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t lo = uni32((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e, (int)rel));
+                const uint32_t hi = uni32((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e >> 32), (int)rel));
+                const uint64_t pos = ((uint64_t)hi << 32) | lo;
+                uint32_t hd = 0;
+                if (uni32(deep_verify(in64, fw, kNoRegion, nbits, pos, lane, &hd) ? 1u : 0u)) {
+                    result = pos;
+                    end = true;
+                    break;
+                }
+            }
+        }
+        npend = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    };
+    uint64_t P = uni64(targets[t]);  // the chain position entering the round
+    while (!end) {
+        if (P + 13 > limit || P >= stop) break;  // the walk ends at P
+        const uint64_t w0 = P >> 6;
+        const uint64_t W0 = fetch(w0 + lane), W1 = fetch(w0 + lane + 1), W2 = fetch(w0 + lane + 2);
+        const uint64_t ws = (w0 + (uint64_t)lane) * 64, we = ws + 64;
+        // the chain through this lane's word from `entry`: visited offsets V,
+        // candidates C (jump 0), and X = the first position past the word or
+        // where the walk ends
+        uint64_t V = 0, C = 0, X = 0;
+        auto walk = [&](uint64_t entry) __attribute__((always_inline)) {
+            V = 0;
+            C = 0;
+            uint64_t pos = entry;
+            while (pos < we && pos + 13 <= limit && pos < stop) {
+                const uint32_t o = (uint32_t)(pos - ws);  // entry >= ws on every walk that reaches here
+                const uint32_t w13 = (uint32_t)((o ? (W0 >> o) | (W1 << (64 - o)) : W0) & 0x1fffu);
+                const uint32_t step = s_next[w13];
+                V |= 1ull << o;
+                if (step == 0) C |= 1ull << o;
+                pos += step ? step : 1;
+            }
+            X = pos;
+        };
+        uint64_t entry = lane == 0 ? P : ws;
+        walk(entry);
+        while (true) {
+            const uint64_t px = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(X >> 32), 1, 64) << 32) |
+                                (uint32_t)__shfl_up((int)(uint32_t)X, 1, 64);
+            const uint64_t ne = lane == 0 ? P : px;
+            const bool changed = ne != entry;
+            if (!__ballot(changed)) break;
+            if (changed) {
+                entry = ne;
+                walk(entry);
+            }
+        }
+        // candidates in chain order: the precode test per lane, survivors listed
+        uint64_t pm = 0;
+        for (uint64_t m = C; m; m &= m - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(m);
+            const uint64_t lo = j ? (W0 >> j) | (W1 << (64 - j)) : W0;
+            const uint64_t hi = j ? (W1 >> j) | (W2 << (64 - j)) : W1;
+            if (precode_ok(lo, hi, ws + j, nbits)) pm |= 1ull << j;
+        }
+        const uint32_t mine = (uint32_t)__popcll(pm);
+        uint32_t before = mine;  // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)before, d, 64);
+            if (lane >= d) before += o;
+        }
+        const uint32_t total = uni32((uint32_t)__shfl((int)before, 63, 64));
+        before -= mine;
+        for (uint32_t done = 0; done < total && !end;) {
+            const uint32_t take = min(total - done, kPendCap - npend);
+            uint32_t r = before;
+            for (uint64_t m = pm; m; m &= m - 1, ++r) {
+                if (r < done || r >= done + take) continue;
+                fw.pend[npend + (r - done)] = ws + (uint64_t)__builtin_ctzll(m);
+            }
+            npend += take;
+            done += take;
+            if (npend >= kPendBatch) flush();
+        }
+        // the chain leaves the round at lane 63's exit (or ended earlier, in
+        // which case every later lane's exit is that same position)
+        P = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(X >> 32), 63, 64) << 32) |
+            (uint32_t)__shfl((int)(uint32_t)X, 63, 64);
+        P = uni64(P);
+    }
+    if (!end && npend) flush();
+    if (lane == 0) {
+        found[t] = result;
+        if (exits) exits[t] = P;
+    }
+}
+
 // Exact targets that a bounded first scan left open, continued in legs: leg
 // j of a target starts at a_j (leg 0 at the target's true chain position P)
 // and was walked by its own wavefront, recording its first verified
@@ -806,9 +953,13 @@ hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const
         if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
         ss = s_seg;
     }
-    hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
-                       d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_stops, d_found, d_exits, d_recs,
-                       n_targets);
+    if (d_recs)  // legs: chain records and merges need the window walk
+        hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
+                           d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_stops, d_found, d_exits, d_recs,
+                           n_targets);
+    else
+        hipLaunchKernelGGL(k_find_exact, dim3(n_targets), dim3(64), 0, s, d_in64, total_bits, d_targets, d_limits,
+                           d_scan_ends, d_stops, d_found, d_exits, n_targets);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (s_seg && (e = hipStreamWaitEvent(s_seg, fork, 0)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_find_any, dim3(n_targets), dim3(64), 0, ss, d_in64, total_bits, d_targets, d_limits,
