@@ -132,3 +132,21 @@ def crafted_header(pre, seq, hlit, hdist, hclen, lead):
     import crafted
 
     return crafted.dynamic_header(pre, seq, hlit, hdist, hclen, lead=lead)
+
+
+def test_crafted_valid_header_decodes_with_zlib():
+    """The crafted encoder is right: the valid header, two literals and EOB
+    make a final block that zlib decodes."""
+    import crafted
+    from oracle import oracle
+
+    pre, seq, hlit, hdist, hclen, want = _cases()["valid"]
+    assert want
+    w, nh = crafted.dynamic_header(pre, seq, hlit, hdist, hclen, bfinal=1)
+    lit = [8] * 252 + [0] * 4 + [7, 7] + [0, 0]  # the lengths `seq` encodes
+    codes = crafted.canonical_codes(lit)
+    for sym, n in ((65, 8), (66, 8), (256, 7)):
+        w.put_code(codes[sym], n)
+    body = w.tobytes()
+    assert zlib.decompress(body, -15) == b"AB"
+    assert oracle.strict_header_ok(body, 0)
