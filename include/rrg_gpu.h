@@ -85,8 +85,8 @@ typedef struct {
     int verbose;              /* Config.verbose: 0 = Off, 1 = On (stage timings on stderr) */
     /* GPU decomposition (no reference counterpart; 0 = automatic) */
     size_t segment_bytes;     /* compressed bytes per lane-decoded segment */
-    int lut_bits;             /* literal/length LUT bits: 8, 9 or 10 */
-    int lanes_per_block;      /* lanes (segments in flight) per workgroup */
+    int lut_bits;             /* literal/length LUT bits: 8 (default), 9 or 10 */
+    int lanes_per_block;      /* lanes (segments in flight) per workgroup (default 16) */
 } rrg_cfg;
 
 /* Return nonzero to close the stream (the reference's dropped Receiver). */
@@ -138,7 +138,7 @@ int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, vo
 /* Autotuner (no reference counterpart; the tuning the reference leaves to
  * Config.num_threads).  Decodes the device-resident sample d_file (a complete
  * gzip file) under each combination of lanes_per_block {8, 16}, lut_bits
- * {9, 10} and segment_bytes {automatic, 40 KiB, 56 KiB}, on top of `base`
+ * {8, 9} and segment_bytes {automatic, 40 KiB, 56 KiB}, on top of `base`
  * (chunk_size_bytes and num_threads are kept), reps timed runs after one
  * warm-up each, and returns the fastest configuration in *best and its time
  * in *best_ms.  Every configuration must produce the same chunk boundaries

@@ -174,8 +174,8 @@ double now_ms() {
 
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
-    G.g.lut_bits = (cfg.lut_bits == 8 || cfg.lut_bits == 10) ? cfg.lut_bits : 9;
-    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 8;
+    G.g.lut_bits = (cfg.lut_bits == 9 || cfg.lut_bits == 10) ? cfg.lut_bits : 8;
+    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 16;
     if (lanes > 64) lanes = 64;
     while (lanes > 1 && (size_t)lanes * inflate_lds_per_lane(G.g.lut_bits) > 65536) lanes /= 2;
     G.g.lanes_per_block = lanes;
@@ -232,7 +232,9 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), c.cells.cap, c.s));
     HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
     HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));
-    HIPCHK(c.res.ensure(S * sizeof(SegResult)));
+    // results of segments decoded by earlier launches of this decode stay in
+    // the buffer (every launch copies all S back), so growing keeps them
+    HIPCHK(c.res.grow_keep(S * sizeof(SegResult), c.res.cap, c.s));
     HIPCHK(c.work.ensure(work.size() * sizeof(uint32_t)));
     HIPCHK(c.counter.ensure(16));
     HIPCHK(hipMemcpyAsync(c.segs.p, desc.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice, c.s));
@@ -433,12 +435,13 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
     uint64_t seg_bytes = J.segment_bytes;
     if (!seg_bytes) {
-        // about 0.8 segments per lane in flight (a lane decodes one segment in
-        // roughly the time the launch takes), between 32 and 64 KiB: shorter
-        // segments cost finder work and more false starts (each a re-decode
-        // round), longer ones idle lanes
-        seg_bytes = (total / 8) * 5 / std::max<uint64_t>(1, 4 * G.lanes_in_flight);
-        seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 32 * 1024), 64 * 1024);
+        // about one segment per lane in flight (a lane decodes one segment in
+        // roughly the time the launch takes), between 24 and 64 KiB: shorter
+        // segments cost finder work (a segment cannot be shorter than a
+        // deflate block, so targets closer than the blocks are wasted scans),
+        // longer ones idle lanes
+        seg_bytes = (total / 8) / std::max<uint64_t>(1, G.lanes_in_flight);
+        seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 24 * 1024), 64 * 1024);
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
     const uint64_t seg_bits = J.serial ? seg_bytes * 8 : chunk_bits / D;
@@ -1434,8 +1437,8 @@ int rrg_gpu_abi_version(void) { return RRG_GPU_ABI_VERSION; }
 void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);
     cfg->chunk_size_bytes = 4u << 20;
-    cfg->lut_bits = 9;
-    cfg->lanes_per_block = 8;
+    cfg->lut_bits = 8;
+    cfg->lanes_per_block = 16;
 }
 
 void* rrg_gpu_create(int device) {
@@ -1530,7 +1533,7 @@ int rrg_gpu_autotune(void* ctx, const void* d_file, size_t len, const uint8_t* h
     if (reps < 1) reps = 1;
     // lanes per workgroup x literal LUT width x segment size (0 = automatic)
     static const int kLanes[] = {8, 16};
-    static const int kLut[] = {9, 10};
+    static const int kLut[] = {8, 9};
     static const size_t kSeg[] = {0, 40960, 57344};
     std::vector<uint64_t> ref_ends, ends(1u << 20);
     double best_t = 1e300;

@@ -33,7 +33,7 @@ def gz(data, level=6):
     return pygzip.compress(data, compresslevel=level, mtime=0)
 
 
-def gpu_decode(decoder, data, chunk, seg=0, lut_bits=9):
+def gpu_decode(decoder, data, chunk, seg=0, lut_bits=8):
     import rusty_rapidgzip_amd as rrg
 
     out = []
@@ -41,7 +41,7 @@ def gpu_decode(decoder, data, chunk, seg=0, lut_bits=9):
     return out, st
 
 
-def check_same(decoder, data, chunk, seg=0, lut_bits=9, expect=None):
+def check_same(decoder, data, chunk, seg=0, lut_bits=8, expect=None):
     rc, ochunks, ost, oer = oracle.read_gz(data, chunk_size=chunk, threads=4)
     assert rc == 0, oer.msg
     out, st = gpu_decode(decoder, data, chunk, seg, lut_bits)
