@@ -145,7 +145,10 @@ def run_fastq(args, threads):
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         t = json.load(open(TRAFFIC_FILE))
-        if t.get("compressed_bytes") == C and t.get("uncompressed_bytes") == n:
+        # counters measured on this workload with this launch geometry
+        if (t.get("compressed_bytes") == C and t.get("uncompressed_bytes") == n
+                and t.get("lut_bits") == cfg.lut_bits and t.get("lanes_per_block") == cfg.lanes_per_block
+                and not cfg.segment_bytes):
             traffic = t.get("k_inflate_bytes_per_launch")
     result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, CRC-checked)", value, "GB/s",
                    args.steps, args.warmup, elapsed * 1e3 / args.steps)
