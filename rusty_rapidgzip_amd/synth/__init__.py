@@ -41,6 +41,11 @@ def _lib():
         lib.rrg_gzip_segments.restype = ctypes.c_size_t
         lib.rrg_bgzf.argtypes = [u8p, ctypes.c_size_t, ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_int]
         lib.rrg_bgzf.restype = ctypes.c_size_t
+        lib.rrg_members_bound.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+        lib.rrg_members_bound.restype = ctypes.c_size_t
+        lib.rrg_members.argtypes = [u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, u8p, ctypes.c_size_t,
+                                    ctypes.c_int]
+        lib.rrg_members.restype = ctypes.c_size_t
         _typed = True
     return lib
 
@@ -104,6 +109,18 @@ def gzip_segments(raw: np.ndarray, levels: np.ndarray, ends: np.ndarray, threads
                                  _threads(threads), piece)
     if n == 0:
         raise RuntimeError("gzip writer failed")
+    return out[:n].copy()
+
+
+def members(raw: np.ndarray, member: int = 1 << 20, level: int = 6, threads: int | None = None) -> np.ndarray:
+    """pigz-style multi-member gzip: consecutive `member`-byte pieces, each its own
+    member with a plain header (no BGZF extra field)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    cap = _lib().rrg_members_bound(raw.size, member)
+    out = np.empty(cap, dtype=np.uint8)
+    n = _lib().rrg_members(_ptr(raw), raw.size, member, level, _ptr(out), cap, _threads(threads))
+    if n == 0 and raw.size:
+        raise RuntimeError("member writer failed")
     return out[:n].copy()
 
 

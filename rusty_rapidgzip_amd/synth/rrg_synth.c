@@ -453,6 +453,74 @@ size_t rrg_bgzf(const uint8_t* in, size_t n, int level, uint8_t* out, size_t cap
     return bad ? 0 : p;
 }
 
+/* ---- pigz-style members -------------------------------------------------------- */
+/* Consecutive `member`-byte pieces, each an independent gzip member with a plain
+ * 10-byte header (no extra field, so not BGZF), as parallel gzip writers emit. */
+typedef struct {
+    const uint8_t* in;
+    size_t n, member, slot;
+    int level;
+    uint8_t* out;
+    size_t* olen;
+    int* err;
+} MemCtx;
+static void mem_job(void* c, size_t i) {
+    MemCtx* b = (MemCtx*)c;
+    size_t off = i * b->member, len = b->n - off < b->member ? b->n - off : b->member;
+    uint8_t* o = b->out + i * b->slot;
+    z_stream s;
+    memset(&s, 0, sizeof s);
+    if (deflateInit2(&s, b->level, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) {
+        b->err[i] = 1;
+        return;
+    }
+    s.next_in = (Bytef*)(b->in + off);
+    s.avail_in = (uInt)len;
+    s.next_out = o + 10;
+    s.avail_out = (uInt)(b->slot - 18);
+    int rc = deflate(&s, Z_FINISH);
+    size_t clen = b->slot - 18 - s.avail_out;
+    deflateEnd(&s);
+    if (rc != Z_STREAM_END) {
+        b->err[i] = 1;
+        return;
+    }
+    static const uint8_t h[10] = {0x1f, 0x8b, 8, 0, 0, 0, 0, 0, 0, 0xff};
+    memcpy(o, h, 10);
+    uint32_t crc = (uint32_t)crc32(0, b->in + off, (uInt)len);
+    uint8_t* t = o + 10 + clen;
+    for (int k = 0; k < 4; k++) t[k] = (uint8_t)(crc >> (8 * k));
+    for (int k = 0; k < 4; k++) t[4 + k] = (uint8_t)((uint32_t)len >> (8 * k));
+    b->olen[i] = 10 + clen + 8;
+}
+
+size_t rrg_members_bound(size_t n, size_t member) {
+    const size_t nb = (n + member - 1) / member;
+    return (nb ? nb : 1) * (compressBound((uLong)member) + 64) + 64;
+}
+
+size_t rrg_members(const uint8_t* in, size_t n, size_t member, int level, uint8_t* out, size_t cap, int nthreads) {
+    if (member == 0 || member > (1u << 30)) return 0;
+    const size_t nb = (n + member - 1) / member, slot = compressBound((uLong)member) + 64;
+    uint8_t* tmp = (uint8_t*)malloc(nb * slot + 1);
+    MemCtx b = {in, n, member, slot, level, tmp, (size_t*)calloc(nb + 1, sizeof(size_t)), (int*)calloc(nb + 1, sizeof(int))};
+    par_for(nb, nthreads, mem_job, &b);
+    size_t p = 0;
+    int bad = 0;
+    for (size_t i = 0; i < nb; i++) {
+        if (b.err[i] || p + b.olen[i] > cap) {
+            bad = 1;
+            break;
+        }
+        memmove(out + p, tmp + i * slot, b.olen[i]);
+        p += b.olen[i];
+    }
+    free(tmp);
+    free(b.olen);
+    free(b.err);
+    return bad ? 0 : p;
+}
+
 /* ---- mixed-entropy single member --------------------------------------------- */
 /* Fills raw[n]; writes per-segment levels into seg_level/seg_end (cap entries);
  * returns the number of segments. */
