@@ -115,6 +115,19 @@ def test_multi_member(decoder):
     check_same(decoder, tiny, 65536)
 
 
+@pytest.mark.parametrize("member_kib", [64, 256, 1024])
+def test_pigz_style_members(decoder, member_kib):
+    """Many members with plain headers (not BGZF): member ends fall inside
+    segments and chunks; chunk boundaries, stats and bytes as the oracle's."""
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(12 << 20, seed=17)
+    g = synth.members(raw, member_kib << 10, 6).tobytes()
+    _say("pigz members", member_kib, len(g))
+    check_same(decoder, g, 1 << 20, expect=raw.tobytes())
+    check_same(decoder, g, 1 << 20, seg=16 << 10, expect=raw.tobytes())
+
+
 def test_nested_gzip_speculation_failures(decoder):
     inner = gz(lcg_payload(2 << 20, 42), 6)
     outer = gz(inner, 1)
