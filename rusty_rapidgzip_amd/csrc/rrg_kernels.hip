@@ -1654,9 +1654,27 @@ __device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, u
     return true;
 }
 
+// Eight consecutive cells from any cell index (gfx950 global memory takes
+// 16-byte accesses at 2-byte alignment).
+__device__ __forceinline__ uint4 load_cells8(const uint16_t* p) {
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+__device__ __forceinline__ void store_cells8(uint16_t* p, uint4 v) { __builtin_memcpy(p, &v, 16); }
+
+// Two packed cells with each marker replaced by the window cell it names
+// (tab[WIN - 1 - offset]); literal cells stay.
+__device__ __forceinline__ uint32_t sym2(uint32_t w, const uint16_t* tab) {
+    uint32_t lo = w & 0xffffu, hi = w >> 16;
+    if (lo & 0x8000u) lo = tab[WIN - 1 - (lo & 0x7fffu)];
+    if (hi & 0x8000u) hi = tab[WIN - 1 - (hi & 0x7fffu)];
+    return lo | (hi << 16);
+}
+
 __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict__ cs, uint32_t n_seg, uint32_t G,
                                                      uint16_t* __restrict__ cells, uint16_t* __restrict__ exit_tail) {
-    __shared__ uint16_t win[WIN];
+    __shared__ __align__(16) uint16_t win[WIN];
     const uint32_t B = blockIdx.x, tid = threadIdx.x;
     const uint32_t first = B * G, end = min(n_seg, first + G);
     const bool fresh = first < n_seg && cs[first].first;
@@ -1671,19 +1689,40 @@ __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict_
         const uint32_t tl = (uint32_t)(len < WIN ? len : WIN);
         if (tl == 0) continue;
         const uint64_t tz0 = cs[s].cells_base + len - tl;  // first tail cell
-        const uint32_t keep = WIN - tl;                      // window cells shifted down
+        if (tl == WIN) {
+            // the whole window is this segment's tail: four runs of 8 cells
+            // per thread, all four loads in flight before the first lookup
+            uint4 q[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) q[r] = load_cells8(cells + tz0 + 8u * (tid + 1024u * r));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) q[r] = make_uint4(sym2(q[r].x, win), sym2(q[r].y, win), sym2(q[r].z, win),
+                                                          sym2(q[r].w, win));
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = 8u * (tid + 1024u * r);
+                *reinterpret_cast<uint4*>(win + i) = q[r];
+                store_cells8(cells + tz0 + i, q[r]);
+            }
+            __syncthreads();
+            continue;
+        }
+        const uint32_t keep = WIN - tl;  // window cells shifted down
         uint16_t v[WIN / 1024];
+        // loads first (all in flight), then the lookups
 #pragma unroll
         for (int r = 0; r < (int)(WIN / 1024); ++r) {
             const uint32_t i = tid + 1024u * r;
-            uint16_t c;
-            if (i < keep) {
-                c = win[i + tl];
-            } else {
-                c = cells[tz0 + (i - keep)];
-                if (c & 0x8000u) c = win[WIN - 1 - (c & 0x7fffu)];
-            }
-            v[r] = c;
+            v[r] = i >= keep ? cells[tz0 + (i - keep)] : (uint16_t)0;
+        }
+#pragma unroll
+        for (int r = 0; r < (int)(WIN / 1024); ++r) {
+            const uint32_t i = tid + 1024u * r;
+            if (i < keep)
+                v[r] = win[i + tl];
+            else if (v[r] & 0x8000u)
+                v[r] = win[WIN - 1 - (v[r] & 0x7fffu)];
         }
         __syncthreads();
 #pragma unroll
@@ -1694,8 +1733,9 @@ __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict_
         }
         __syncthreads();
     }
-    uint16_t* out = exit_tail + (size_t)B * WIN;
-    for (uint32_t i = tid; i < WIN; i += 1024) out[i] = win[i];
+    uint4* out = reinterpret_cast<uint4*>(exit_tail + (size_t)B * WIN);
+    const uint4* w4 = reinterpret_cast<const uint4*>(win);
+    for (uint32_t i = tid; i < WIN / 8; i += 1024) out[i] = w4[i];
 }
 
 // Composition of symbolic windows: out[i] = X[i] when it is a literal (or
@@ -1705,27 +1745,27 @@ __global__ __launch_bounds__(1024) void k_tail_block(const ChainSeg* __restrict_
 // blocks into one window relative to the entry of the first of them.
 __global__ __launch_bounds__(1024) void k_tail_compose(const uint16_t* __restrict__ exit_tail, uint32_t n_blocks,
                                                        uint32_t K, uint16_t* __restrict__ group_exit) {
-    __shared__ uint16_t acc[WIN];
+    __shared__ __align__(16) uint16_t acc[WIN];
     const uint32_t tid = threadIdx.x, g = blockIdx.x;
     const uint32_t first = g * K, end = min(n_blocks, first + K);
     for (uint32_t i = tid; i < WIN; i += 1024) acc[i] = (uint16_t)(0x8000u | (WIN - 1 - i));  // identity
     __syncthreads();
+    uint4* a4 = reinterpret_cast<uint4*>(acc);
     for (uint32_t B = first; B < end; ++B) {
-        uint16_t v[WIN / 1024];
-        const uint16_t* ex = exit_tail + (size_t)B * WIN;
+        const uint4* ex = reinterpret_cast<const uint4*>(exit_tail + (size_t)B * WIN);
+        uint4 q[4];
 #pragma unroll
-        for (int r = 0; r < (int)(WIN / 1024); ++r) {
-            uint16_t c = ex[tid + 1024u * r];
-            if (c & 0x8000u) c = acc[WIN - 1 - (c & 0x7fffu)];
-            v[r] = c;
-        }
+        for (int r = 0; r < 4; ++r) q[r] = ex[tid + 1024u * r];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = make_uint4(sym2(q[r].x, acc), sym2(q[r].y, acc), sym2(q[r].z, acc),
+                                                      sym2(q[r].w, acc));
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < (int)(WIN / 1024); ++r) acc[tid + 1024u * r] = v[r];
+        for (int r = 0; r < 4; ++r) a4[tid + 1024u * r] = q[r];
         __syncthreads();
     }
-    uint16_t* out = group_exit + (size_t)g * WIN;
-    for (uint32_t i = tid; i < WIN; i += 1024) out[i] = acc[i];
+    uint4* out = reinterpret_cast<uint4*>(group_exit + (size_t)g * WIN);
+    for (uint32_t i = tid; i < WIN / 8; i += 1024) out[i] = a4[i];
 }
 
 // k_tail_entry: workgroup g walks blocks [gK, gK + K) in order, starting from
@@ -1734,28 +1774,44 @@ __global__ __launch_bounds__(1024) void k_tail_compose(const uint16_t* __restric
 __global__ __launch_bounds__(1024) void k_tail_entry(const uint16_t* __restrict__ exit_tail, uint32_t n_blocks,
                                                      uint32_t K, const uint16_t* __restrict__ init,
                                                      uint16_t* __restrict__ entry) {
-    __shared__ uint16_t ent[WIN];
+    __shared__ __align__(16) uint16_t ent[WIN];
     const uint32_t tid = threadIdx.x, g = blockIdx.x;
     const uint32_t first = g * K, end = min(n_blocks, first + K);
     for (uint32_t i = tid; i < WIN; i += 1024) ent[i] = init ? init[(size_t)g * WIN + i] : (uint16_t)0x100;
     __syncthreads();
+    uint4* e4 = reinterpret_cast<uint4*>(ent);
     for (uint32_t B = first; B < end; ++B) {
-        uint16_t v[WIN / 1024];
-        const uint16_t* ex = exit_tail + (size_t)B * WIN;
-        uint16_t* en = entry + (size_t)B * WIN;
+        const uint4* ex = reinterpret_cast<const uint4*>(exit_tail + (size_t)B * WIN);
+        uint4* en = reinterpret_cast<uint4*>(entry + (size_t)B * WIN);
+        uint4 q[4];
 #pragma unroll
-        for (int r = 0; r < (int)(WIN / 1024); ++r) {
-            const uint32_t i = tid + 1024u * r;
-            en[i] = ent[i];
-            uint16_t c = ex[i];
-            if (c & 0x8000u) c = ent[WIN - 1 - (c & 0x7fffu)];
-            v[r] = c;
+        for (int r = 0; r < 4; ++r) q[r] = ex[tid + 1024u * r];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            en[tid + 1024u * r] = e4[tid + 1024u * r];
+            q[r] = make_uint4(sym2(q[r].x, ent), sym2(q[r].y, ent), sym2(q[r].z, ent), sym2(q[r].w, ent));
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < (int)(WIN / 1024); ++r) ent[tid + 1024u * r] = v[r];
+        for (int r = 0; r < 4; ++r) e4[tid + 1024u * r] = q[r];
         __syncthreads();
     }
+}
+
+// Two packed tail cells made literal: a marker (relative to the block entry)
+// becomes the entry window's byte; a 0x100 cell (a byte before the output
+// start) sets `bad`.  The entry cell is read whatever the cell holds (the
+// index is below WIN either way), so a run's reads are all in flight at once.
+__device__ __forceinline__ uint32_t lit2(uint32_t w, const uint16_t* __restrict__ en, bool& bad) {
+    uint32_t c[2] = {w & 0xffffu, w >> 16};
+    const uint32_t e[2] = {en[WIN - 1 - (c[0] & 0x7fffu)], en[WIN - 1 - (c[1] & 0x7fffu)]};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t b = (c[h] & 0x8000u) ? e[h] : c[h];
+        bad |= (b & 0x100u) != 0;
+        c[h] = b & 0xffu;
+    }
+    return c[0] | (c[1] << 16);
 }
 
 __global__ __launch_bounds__(256) void k_tail_patch(const ChainSeg* __restrict__ cs, uint32_t n_seg, uint32_t G,
@@ -1767,6 +1823,23 @@ __global__ __launch_bounds__(256) void k_tail_patch(const ChainSeg* __restrict__
     const uint32_t tl = (uint32_t)(len < WIN ? len : WIN);
     const uint64_t tz0 = cs[s].cells_base + len - tl;
     const uint16_t* en = entry + (size_t)(s / G) * WIN;
+    if (tl == WIN) {
+        // sixteen runs of 8 cells per thread, all loads in flight before the
+        // first test; runs without a marker or 0x100 cell are not rewritten
+        bool bad = false;
+        uint4 q[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) q[r] = load_cells8(cells + tz0 + 8u * (threadIdx.x + 256u * r));
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            if ((q[r].x | q[r].y | q[r].z | q[r].w) & 0x81008100u) {
+                q[r] = make_uint4(lit2(q[r].x, en, bad), lit2(q[r].y, en, bad), lit2(q[r].z, en, bad),
+                                  lit2(q[r].w, en, bad));
+                store_cells8(cells + tz0 + 8u * (threadIdx.x + 256u * r), q[r]);
+            }
+        if (bad) err[s] = 1u;
+        return;
+    }
     for (uint32_t k = threadIdx.x; k < tl; k += 256) {
         const uint16_t c = cells[tz0 + k];
         if (c & 0x8100u) {
