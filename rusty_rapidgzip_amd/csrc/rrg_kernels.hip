@@ -172,6 +172,34 @@ __device__ __forceinline__ bool precode_ok(uint64_t lo, uint64_t hi, uint64_t p,
     return nsym == 1 || (nsym >= 2 && sum == 128u);
 }
 
+// The same test with a table of three code lengths per entry: entry k holds
+// the Kraft sum (units of 2^-7, bits 0-15) and the count (bits 16-31) of the
+// nonzero 3-bit lengths packed in k.  Seven lookups cover the 19 lengths
+// (the lengths past hclen are masked to zero first).
+__device__ __forceinline__ uint32_t precode3_entry(uint32_t k) {
+    uint32_t sum = 0, n = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const uint32_t l = (k >> (3 * f)) & 7u;
+        if (l) {
+            sum += 128u >> l;
+            ++n;
+        }
+    }
+    return sum | (n << 16);
+}
+__device__ __forceinline__ bool precode_ok_lut(const uint32_t* __restrict__ tab, uint64_t lo, uint64_t hi, uint64_t p,
+                                               uint64_t nbits) {
+    const uint32_t hclen = 4u + (uint32_t)((lo >> 13) & 15u);
+    if (p + 17 + 3ull * hclen > nbits) return false;
+    const uint64_t pc = ((lo >> 17) | (hi << 47)) & ((1ull << (3 * hclen)) - 1ull);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) acc += tab[(uint32_t)(pc >> (9 * i)) & 511u];
+    const uint32_t sum = acc & 0xffffu, n = acc >> 16;
+    return n == 1u || (n >= 2u && sum == 128u);
+}
+
 // Strict dynamic header (block_finder.rs:196-262) at `p`, the block's first
 // bit (BFINAL/BTYPE are the caller's), as wave-uniform code; the lanes
 // cooperate to build the precode LUT and to fill runs of code lengths.  On
@@ -555,6 +583,7 @@ __global__ __launch_bounds__(64 * FIND_WAVES) void k_find_blocks(const uint64_t*
 // its find by at most kPendBatch precode survivors.
 struct AnyWave : FindWave {
     uint64_t pend[128];  // precode survivors, bit order (bit 63: verified stored header)
+    uint32_t pre3[512];  // precode3_entry(k)
 };
 
 __device__ __forceinline__ uint64_t zero_bytes(uint64_t v) {  // high bit of each zero byte (and maybe of bytes above one)
@@ -576,6 +605,9 @@ __global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in
     const uint64_t limit = scan_ends ? min(uni64(scan_ends[t]), nbits) : nbits;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     const uint64_t first = uni64(targets[t]);
+    for (int k = lane; k < 512; k += 64) fw.pre3[k] = precode3_entry((uint32_t)k);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     constexpr uint32_t kPendBatch = 32, kPendCap = 128;
     constexpr uint64_t kNoRegion = ~0ull;  // deep_verify reads global memory
     uint64_t result = ~0ull, weak = ~0ull;
@@ -674,7 +706,7 @@ __global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in
             const uint32_t j = (uint32_t)__builtin_ctzll(m);
             const uint64_t lo = j ? (W0 >> j) | (W1 << (64 - j)) : W0;
             const uint64_t hi = j ? (W1 >> j) | (W2 << (64 - j)) : W1;
-            if (precode_ok(lo, hi, wb + j, nbits)) pm |= 1ull << j;
+            if (precode_ok_lut(fw.pre3, lo, hi, wb + j, nbits)) pm |= 1ull << j;
         }
         // list the survivors in bit order (lane order, then offset order),
         // at most kPendCap at a time
