@@ -27,6 +27,51 @@ RRG_HD uint32_t rev7(uint32_t x) {
     return (x & 0xffu) >> 1;
 }
 
+// Cheap stage (block_finder.rs:196-213): header fields + strict precode Kraft.
+// `lo`/`hi` hold the 128 bits at p; branch-free over the 19 possible fields.
+RRG_HD bool precode_ok(uint64_t lo, uint64_t hi, uint64_t p, uint64_t nbits) {
+    const uint32_t hclen = 4u + (uint32_t)((lo >> 13) & 15u);
+    if (p + 17 + 3ull * hclen > nbits) return false;
+    const uint64_t pc = (lo >> 17) | (hi << 47);  // precode lengths, 3 bits each
+    uint32_t sum = 0, nsym = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 19; ++i) {
+        uint32_t l = (uint32_t)(pc >> (3 * i)) & 7u;
+        l = i < hclen ? l : 0u;
+        sum += (128u >> l) & (0u - (uint32_t)(l != 0));
+        nsym += (uint32_t)(l != 0);
+    }
+    return nsym == 1 || (nsym >= 2 && sum == 128u);
+}
+
+// The same test with a table of three code lengths per entry: entry k holds
+// the Kraft sum (units of 2^-7, bits 0-15) and the count (bits 16-31) of the
+// nonzero 3-bit lengths packed in k.  Seven lookups cover the 19 lengths
+// (the lengths past hclen are masked to zero first).
+RRG_HD uint32_t precode3_entry(uint32_t k) {
+    uint32_t sum = 0, n = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const uint32_t l = (k >> (3 * f)) & 7u;
+        if (l) {
+            sum += 128u >> l;
+            ++n;
+        }
+    }
+    return sum | (n << 16);
+}
+RRG_HD bool precode_ok_lut(const uint32_t* __restrict__ tab, uint64_t lo, uint64_t hi, uint64_t p,
+                                               uint64_t nbits) {
+    const uint32_t hclen = 4u + (uint32_t)((lo >> 13) & 15u);
+    if (p + 17 + 3ull * hclen > nbits) return false;
+    const uint64_t pc = ((lo >> 17) | (hi << 47)) & ((1ull << (3 * hclen)) - 1ull);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) acc += tab[(uint32_t)(pc >> (9 * i)) & 511u];
+    const uint32_t sum = acc & 0xffffu, n = acc >> 16;
+    return n == 1u || (n >= 2u && sum == 128u);
+}
+
 // `fetch(q)` returns 64-bit word q of the stream (zero past its end); bits
 // are little-endian within words.  `p` is the block's first bit (BFINAL);
 // BFINAL and BTYPE are the caller's.  `nbits` bounds every read as the
