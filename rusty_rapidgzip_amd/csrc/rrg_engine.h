@@ -67,7 +67,7 @@ struct Ctx {
     hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
-        pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext;
+        pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
     uint8_t* pin = nullptr;
@@ -79,7 +79,7 @@ struct Ctx {
     ~Ctx() {
         DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
                          &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
-                         &stops,  &exits,   &recs,   &legtg, &lfound, &lnext};
+                         &stops,  &exits,   &recs,   &legtg, &lfound, &lnext, &exact_idx};
         for (DevBuf* b : all) b->release();
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);

@@ -817,12 +817,20 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
     }
     if (exits) HIPCHK(c.exits.ensure(n * 8));
     if (want_recs) HIPCHK(c.recs.ensure(n * 32));
+    std::vector<uint32_t> exact;  // targets that keep the reference's candidate order
+    for (size_t i = 0; i < n; ++i)
+        if (!(limits[i] & kAnyStart)) exact.push_back((uint32_t)i);
+    if (!exact.empty()) {
+        HIPCHK(c.exact_idx.ensure(exact.size() * 4));
+        HIPCHK(hipMemcpyAsync(c.exact_idx.p, exact.data(), exact.size() * 4, hipMemcpyHostToDevice, c.s));
+    }
     StageTimer t(c, &c.prof.ms_find);
     HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
                               c.limits.as<uint64_t>(), c.scan_ends.as<uint64_t>(),
                               stops ? c.stops.as<uint64_t>() : nullptr, c.found.as<uint64_t>(),
                               exits ? c.exits.as<uint64_t>() : nullptr, want_recs ? c.recs.as<uint64_t>() : nullptr,
-                              (uint32_t)n, c.s, c.s2, c.evf[0], c.evf[1]));
+                              (uint32_t)n, c.exact_idx.as<uint32_t>(), (uint32_t)exact.size(), c.s, c.s2, c.evf[0],
+                              c.evf[1]));
     HIPCHK(hipMemcpyAsync(found.data(), c.found.p, n * 8, hipMemcpyDeviceToHost, c.s));
     if (exits) {
         exits->assign(n, 0);

@@ -716,10 +716,15 @@ __global__ __launch_bounds__(64) void k_find_exact(const uint64_t* __restrict__ 
                                                    const uint64_t* __restrict__ limits,
                                                    const uint64_t* __restrict__ scan_ends,
                                                    const uint64_t* __restrict__ stops, uint64_t* __restrict__ found,
-                                                   uint64_t* __restrict__ exits, uint32_t n_targets) {
+                                                   uint64_t* __restrict__ exits, uint32_t n_targets,
+                                                   const uint32_t* __restrict__ exact_idx, uint32_t n_exact) {
     __shared__ uint8_t s_next[8192];
     __shared__ AnyWave fw;
-    const uint32_t t = blockIdx.x;
+    if (blockIdx.x >= n_exact) return;
+    // these few latency-bound walks share the SIMDs with the segment scans
+    // (VALU bound): the wave priority lets them issue first when ready
+    __builtin_amdgcn_s_setprio(3);
+    const uint32_t t = exact_idx[blockIdx.x];
     if (t >= n_targets) return;
     const uint64_t nbits = uni64(limits[t]);
     if (nbits >> 63) return;  // a segment target
@@ -930,7 +935,8 @@ __global__ __launch_bounds__(64) void k_leg_fixup(const uint64_t* __restrict__ i
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
                               const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
                               uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
-                              hipStream_t s, hipStream_t s_seg, hipEvent_t fork, hipEvent_t join) {
+                              const uint32_t* d_exact_idx, uint32_t n_exact, hipStream_t s, hipStream_t s_seg,
+                              hipEvent_t fork, hipEvent_t join) {
     if (n_targets == 0) return hipSuccess;
     hipError_t e;
     // the exact walks go first: they are few but long, and their workgroups
@@ -944,9 +950,9 @@ hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const
         hipLaunchKernelGGL(k_find_blocks, dim3((n_targets + FIND_WAVES - 1) / FIND_WAVES), dim3(64 * FIND_WAVES), 0, s,
                            d_in64, total_bits, d_targets, d_limits, d_scan_ends, d_stops, d_found, d_exits, d_recs,
                            n_targets);
-    else
-        hipLaunchKernelGGL(k_find_exact, dim3(n_targets), dim3(64), 0, s, d_in64, total_bits, d_targets, d_limits,
-                           d_scan_ends, d_stops, d_found, d_exits, n_targets);
+    else if (n_exact)
+        hipLaunchKernelGGL(k_find_exact, dim3(n_exact), dim3(64), 0, s, d_in64, total_bits, d_targets, d_limits,
+                           d_scan_ends, d_stops, d_found, d_exits, n_targets, d_exact_idx, n_exact);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (s_seg && (e = hipStreamWaitEvent(s_seg, fork, 0)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_find_any, dim3(n_targets), dim3(64), 0, ss, d_in64, total_bits, d_targets, d_limits,

@@ -20,11 +20,15 @@ namespace rrg {
 // the first 256 bits after the target); each may be null.  Exact targets
 // are walked on stream s; with s_seg non-null the segment targets are
 // scanned concurrently on s_seg (forked from and joined back into s with the
-// two events), else on s after them.
+// two events), else on s after them.  d_exact_idx lists the n_exact exact
+// targets (their indices, ascending); the exact walk launches one workgroup
+// per entry, so its few workgroups do not wait behind the segment scans for
+// dispatch.
 hipError_t launch_find_blocks(const uint64_t* d_in64, uint64_t total_bits, const uint64_t* d_targets,
                               const uint64_t* d_limits, const uint64_t* d_scan_ends, const uint64_t* d_stops,
                               uint64_t* d_found, uint64_t* d_exits, uint64_t* d_recs, uint32_t n_targets,
-                              hipStream_t s, hipStream_t s_seg, hipEvent_t fork, hipEvent_t join);
+                              const uint32_t* d_exact_idx, uint32_t n_exact, hipStream_t s, hipStream_t s_seg,
+                              hipEvent_t fork, hipEvent_t join);
 // Exact targets continued in legs: per target, the first verified candidate
 // on its chain over its legs (~0 if none) and the chain position after them.
 hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const LegTarget* d_tg, const uint64_t* d_leg_a,
