@@ -1160,10 +1160,15 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         if (J.stream || J.serial) B = J.stream_B;
         for (size_t i = J.t_lo; i < J.t_hi; ++i)
             if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
-        std::vector<uint64_t> M(B);
+        // segment finds come in target order, which is almost always their
+        // order too: merge them with the chunk starts (sort only when not)
+        std::vector<uint64_t> F;
+        F.reserve(J.t_hi - J.t_lo);
         for (size_t i = J.t_lo; i < J.t_hi; ++i)
-            if (!is_chunk[i] && found[i] != UINT64_MAX) M.push_back(found[i]);
-        std::sort(M.begin(), M.end());
+            if (!is_chunk[i] && found[i] != UINT64_MAX) F.push_back(found[i]);
+        if (!std::is_sorted(F.begin(), F.end())) std::sort(F.begin(), F.end());
+        std::vector<uint64_t> M(B.size() + F.size());
+        std::merge(B.begin(), B.end(), F.begin(), F.end(), M.begin());
         M.erase(std::unique(M.begin(), M.end()), M.end());
         const size_t S = M.size();
         J.seg_is_chunk.assign(S, 0);
