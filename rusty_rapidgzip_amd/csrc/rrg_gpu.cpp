@@ -150,15 +150,34 @@ const X2N& x2n() {
     static X2N x;
     return x;
 }
-// x^(8n) mod P
-uint32_t x8n(uint64_t n) {
-    uint32_t r = 0x80000000u;
-    int k = 3;
-    while (n) {
-        if (n & 1) r = gf2_mul(x2n().t[k], r);
-        n >>= 1;
-        ++k;
+// x^(8 d 16^k) mod P for every hexadecimal digit d of n at position k
+struct X8Nib {
+    uint32_t t[16][16];
+    X8Nib() {
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t base = x2n().t[3 + 4 * k];  // x^(8 * 16^k) = x^(2^(3 + 4k))
+            t[k][0] = 0x80000000u;                     // 1
+            for (int d = 1; d < 16; ++d) t[k][d] = gf2_mul(t[k][d - 1], base);
+        }
     }
+};
+const X8Nib& x8nib() {
+    static X8Nib x;
+    return x;
+}
+// x^(8n) mod P: one product per nonzero hexadecimal digit of n.  Lengths
+// repeat (BGZF members, full CRC pieces), so the last result is kept.
+uint32_t x8n(uint64_t n) {
+    thread_local uint64_t last_n = ~0ull;
+    thread_local uint32_t last_r = 0;
+    if (n == last_n) return last_r;
+    const X8Nib& tab = x8nib();
+    uint32_t r = 0x80000000u;
+    const uint64_t n0 = n;
+    for (int k = 0; n; ++k, n >>= 4)
+        if (n & 15) r = gf2_mul(tab.t[k][n & 15], r);
+    last_n = n0;
+    last_r = r;
     return r;
 }
 
