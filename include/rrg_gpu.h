@@ -227,6 +227,13 @@ uint64_t rrg_server_pending(void* srv);
 /* Finish the queued requests, stop the threads, free device memory. */
 void rrg_server_destroy(void* srv);
 
+/* ---- testing hook (no reference counterpart) ---------------------------- */
+
+/* Make the n-th next device allocation of the library fail with HIP's
+ * out-of-memory error (once; n <= 0 turns the hook off).  Used by the tests
+ * of the server's recovery from an allocation failure. */
+void rrg_gpu_debug_fail_alloc(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -148,6 +148,7 @@ EXPORTED_SYMBOLS = (
     "rrg_server_release",
     "rrg_server_pending",
     "rrg_server_destroy",
+    "rrg_gpu_debug_fail_alloc",
 )
 
 
@@ -189,6 +190,7 @@ def _lib() -> ctypes.CDLL:
         lib.rrg_server_pending.restype = ctypes.c_uint64
         lib.rrg_server_pending.argtypes = [ctypes.c_void_p]
         lib.rrg_server_destroy.argtypes = [ctypes.c_void_p]
+        lib.rrg_gpu_debug_fail_alloc.argtypes = [ctypes.c_int]
         lib._rrg_typed = True
     return lib
 
@@ -602,6 +604,12 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def debug_fail_alloc(n: int) -> None:
+    """Testing hook: the n-th next device allocation of the library fails with
+    HIP's out-of-memory error (n <= 0 turns it off)."""
+    _lib().rrg_gpu_debug_fail_alloc(int(n))
 
 
 _default: dict[int, Decoder] = {}

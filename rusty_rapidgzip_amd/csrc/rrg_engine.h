@@ -16,16 +16,52 @@
 namespace rrg {
 namespace host {
 
+// A fixed region of device memory that DevBufs are carved from instead of
+// being allocated (bump allocation; a grown buffer abandons its old block).
+// A request that does not fit fails the call with `exhausted` set.
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0, high = 0;
+    bool exhausted = false;
+    void* take(size_t n) {
+        const size_t a = (used + 255) & ~(size_t)255;
+        if (a > cap || n > cap - a) {
+            exhausted = true;
+            return nullptr;
+        }
+        used = a + n;
+        if (used > high) high = used;
+        return base + a;
+    }
+};
+
+// hipMalloc, or hipErrorOutOfMemory when the allocation-failure hook
+// (rrg_gpu_debug_fail_alloc) says so.
+hipError_t dev_malloc(void** p, size_t n);
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;  // carve from this arena instead of allocating
+    bool fixed = false;      // caller-owned memory of capacity cap: never reallocated
+    hipError_t take(void** q, size_t n) {
+        if (fixed) return hipErrorOutOfMemory;
+        if (arena) {
+            *q = arena->take(n);
+            return *q ? hipSuccess : hipErrorOutOfMemory;
+        }
+        return dev_malloc(q, n);
+    }
+    void drop() {
+        if (p && !arena && !fixed) (void)hipFree(p);
+    }
     hipError_t ensure(size_t n) {
         if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
+        drop();
         p = nullptr;
         cap = 0;
-        const size_t c = n + n / 8 + 4096;
-        hipError_t e = hipMalloc(&p, c);
+        const size_t c = arena ? n : n + n / 8 + 4096;
+        hipError_t e = take(&p, c);
         if (e == hipSuccess) cap = c;
         return e;
     }
@@ -33,18 +69,18 @@ struct DevBuf {
     hipError_t grow_keep(size_t n, size_t used, hipStream_t s) {
         if (n <= cap) return hipSuccess;
         void* q = nullptr;
-        const size_t c = n + n / 4 + 4096;
-        hipError_t e = hipMalloc(&q, c);
+        const size_t c = arena ? n + n / 4 : n + n / 4 + 4096;
+        hipError_t e = take(&q, c);
         if (e != hipSuccess) return e;
         if (p && used) {
             e = hipMemcpyAsync(q, p, used, hipMemcpyDeviceToDevice, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) {
-                (void)hipFree(q);
+                if (!arena) (void)hipFree(q);
                 return e;
             }
         }
-        if (p) (void)hipFree(p);
+        drop();
         p = q;
         cap = c;
         return hipSuccess;
@@ -54,7 +90,7 @@ struct DevBuf {
         return static_cast<T*>(p);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        drop();
         p = nullptr;
         cap = 0;
     }
@@ -66,6 +102,7 @@ struct Ctx {
     hipStream_t s2 = nullptr;  // a second stream (default priority) for work that overlaps s
     hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
+    uint64_t max_lanes = 0;  // cap on inflate lanes in flight (0 = as many as fit)
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
         pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
@@ -76,11 +113,14 @@ struct Ctx {
     bool verbose = false;  // Config.verbose of the current call: per-round lines on stderr
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evc[2] = {nullptr, nullptr};  // staging copies of the chunk emission
+    // every device buffer of the context
+    std::vector<DevBuf*> bufs() {
+        return {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
+                &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
+                &stops,  &exits,   &recs,   &legtg, &lfound, &lnext, &exact_idx};
+    }
     ~Ctx() {
-        DevBuf* all[] = {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
-                         &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
-                         &stops,  &exits,   &recs,   &legtg, &lfound, &lnext, &exact_idx};
-        for (DevBuf* b : all) b->release();
+        for (DevBuf* b : bufs()) b->release();
         if (pin) (void)hipHostFree(pin);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -93,10 +133,15 @@ struct Ctx {
     }
 };
 
+// The HIP error behind the last RRG_E_DEVICE returned on this thread (the
+// server tells an allocation failure from a device fault by it).
+extern thread_local hipError_t t_last_hip;
+
 #define HIPCHK(expr)                                                                         \
     do {                                                                                     \
         hipError_t _e = (expr);                                                              \
         if (_e != hipSuccess) {                                                              \
+            t_last_hip = _e;                                                                 \
             set_err(err, RRG_E_DEVICE, "HIP error %s at %s:%d", hipGetErrorString(_e), __FILE__, \
                     __LINE__);                                                               \
             return RRG_E_DEVICE;                                                             \
@@ -183,6 +228,8 @@ long parse_header(const uint8_t* p, size_t n, uint32_t* method);
 uint32_t bgzf_block_size(const uint8_t* p, size_t n);
 double now_ms();
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg);
+// Device bytes held at the peak of a decode of len compressed bytes into out bytes.
+uint64_t engine_footprint(uint64_t len, uint64_t out);
 void begin_profile(Ctx& c, uint64_t len);
 void print_profile(const Ctx& c);
 // Decode every job (inputs already in c.in[0, c.in_used)) into `out`.

@@ -23,6 +23,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -37,6 +38,26 @@ namespace host {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+thread_local hipError_t t_last_hip = hipSuccess;
+
+// allocation-failure hook: > 0 = device allocations that still succeed
+// before the next one fails (once); 0 = off
+static std::atomic<long> g_fail_alloc{0};
+
+hipError_t dev_malloc(void** p, size_t n) {
+    long v = g_fail_alloc.load();
+    while (v > 0) {
+        if (g_fail_alloc.compare_exchange_weak(v, v - 1)) {
+            if (v == 1) {
+                *p = nullptr;
+                return hipErrorOutOfMemory;
+            }
+            break;
+        }
+    }
+    return hipMalloc(p, n);
+}
+
 void set_err(rrg_error* e, int kind, const char* fmt, ...) {
     if (!e) return;
     std::memset(e, 0, sizeof *e);
@@ -201,6 +222,8 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     int per_cu = inflate_blocks_per_cu(G.g.lut_bits, lanes);
     if (per_cu < 1) per_cu = 1;
     G.g.blocks = per_cu * c.n_cu;
+    if (c.max_lanes && (uint64_t)G.g.blocks * (uint64_t)lanes > c.max_lanes)
+        G.g.blocks = std::max<int>(1, (int)(c.max_lanes / (uint64_t)lanes));
     G.lanes_in_flight = (uint64_t)G.g.blocks * (uint64_t)lanes;
     return G;
 }
@@ -214,6 +237,15 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
 // segment only.  Beyond it the kernel counts the cells needed and the segment
 // is regrown.
 uint64_t seg_cells(uint64_t span) { return ((span + 32768) * 4 + 16384 + 7) & ~7ull; }
+
+// Device bytes a decode of `len` compressed bytes holds at its peak: the
+// input, the u16 cell regions of its segments (seg_cells over segments of at
+// least 24 KiB) and `out` bytes of output.
+uint64_t engine_footprint(uint64_t len, uint64_t out) {
+    const uint64_t n_seg = len / (24 * 1024) + 1;
+    const uint64_t cells = 4 * len + n_seg * (4 * 32768 + 16384 + 8);
+    return len + 64 + 2 * cells + out;
+}
 
 // Member-record slots of a segment spanning `span` input bytes: a few member
 // ends, or, when block ends are recorded, about one block per 4 KiB of input
@@ -1465,6 +1497,8 @@ using namespace rrg::host;
 extern "C" {
 
 int rrg_gpu_abi_version(void) { return RRG_GPU_ABI_VERSION; }
+
+void rrg_gpu_debug_fail_alloc(int n) { g_fail_alloc.store(n > 0 ? n : 0); }
 
 void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);

@@ -2028,8 +2028,12 @@ hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const Ch
 // The 256 partial values are folded in a tree with x^(64*2^k); the last
 // (unaligned) bytes of the piece follow byte by byte.
 // ===========================================================================
-__constant__ uint32_t c_crc_shift[8];  // x^(64*2^k) mod P, reflected (2^k words of 8 bytes)
-__constant__ uint32_t c_crc_step;      // x^(8*2048) mod P (256 words)
+// GF(2) constants of the fold, passed by value: shift[k] = x^(64*2^k) mod P
+// (2^k words of 8 bytes), step = x^(8*2048) mod P (256 words), reflected
+struct CrcConsts {
+    uint32_t shift[8];
+    uint32_t step;
+};
 
 __device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
     // a*b mod P over GF(2), reflected bit order (bit 31 = x^0)
@@ -2044,7 +2048,8 @@ __device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
 
 __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ out,
                                                     const uint64_t* __restrict__ piece_off,
-                                                    const uint32_t* __restrict__ piece_len, uint32_t* __restrict__ crc) {
+                                                    const uint32_t* __restrict__ piece_len, uint32_t* __restrict__ crc,
+                                                    const CrcConsts K) {
     __shared__ uint32_t T[8][256];  // slice-by-8
     __shared__ uint32_t M[4][256];  // v -> v * x^(8*2048), by bytes of v
     __shared__ uint32_t part[256];
@@ -2053,7 +2058,7 @@ __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ 
         uint32_t c = tid;
         for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
         T[0][tid] = c;
-        const uint32_t X = c_crc_step;
+        const uint32_t X = K.step;
 #pragma unroll
         for (int j = 0; j < 4; ++j) M[j][tid] = gf2_mul(X, tid << (8 * j));
     }
@@ -2087,7 +2092,7 @@ __global__ __launch_bounds__(256) void k_crc_pieces(const uint8_t* __restrict__ 
     __syncthreads();
     for (int k = 0; k < 8; ++k) {
         const uint32_t span = 1u << k;
-        if ((tid & ((span << 1) - 1)) == 0) part[tid] = gf2_mul(c_crc_shift[k], part[tid]) ^ part[tid + span];
+        if ((tid & ((span << 1) - 1)) == 0) part[tid] = gf2_mul(K.shift[k], part[tid]) ^ part[tid + span];
         __syncthreads();
     }
     if (tid == 0) {
@@ -2134,33 +2139,32 @@ hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint1
     return hipGetLastError();
 }
 
+// The fold constants, computed once (thread-safe static initialisation).
+static CrcConsts make_crc_consts() {
+    auto mul = [](uint32_t a, uint32_t b) {
+        uint32_t prod = 0;
+        for (int i = 0; i < 32; ++i) {
+            if (a & (0x80000000u >> i)) prod ^= b;
+            b = (b & 1u) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+        }
+        return prod;
+    };
+    CrcConsts K{};
+    uint32_t x = 0x40000000u;                   // x^1
+    for (int i = 0; i < 6; ++i) x = mul(x, x);  // x^64: one 8-byte word
+    for (int k = 0; k < 8; ++k) {
+        K.shift[k] = x;
+        x = mul(x, x);
+    }
+    K.step = x;  // x^(64*256) = x^(8*2048)
+    return K;
+}
+
 hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
                              uint32_t n_pieces, uint32_t* d_crc, hipStream_t s) {
     if (n_pieces == 0) return hipSuccess;
-    static bool init = false;
-    if (!init) {
-        auto mul = [](uint32_t a, uint32_t b) {
-            uint32_t prod = 0;
-            for (int i = 0; i < 32; ++i) {
-                if (a & (0x80000000u >> i)) prod ^= b;
-                b = (b & 1u) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
-            }
-            return prod;
-        };
-        uint32_t x = 0x40000000u;                   // x^1
-        for (int i = 0; i < 6; ++i) x = mul(x, x);  // x^64: one 8-byte word
-        uint32_t k8[8];
-        for (int k = 0; k < 8; ++k) {
-            k8[k] = x;
-            x = mul(x, x);
-        }
-        const uint32_t step = x;  // x^(64*256) = x^(8*2048)
-        hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_shift), k8, sizeof k8);
-        if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc_step), &step, sizeof step);
-        if (e != hipSuccess) return e;
-        init = true;
-    }
-    hipLaunchKernelGGL(k_crc_pieces, dim3(n_pieces), dim3(256), 0, s, d_out, d_piece_off, d_piece_len, d_crc);
+    static const CrcConsts K = make_crc_consts();
+    hipLaunchKernelGGL(k_crc_pieces, dim3(n_pieces), dim3(256), 0, s, d_out, d_piece_off, d_piece_len, d_crc, K);
     return hipGetLastError();
 }
 

@@ -167,6 +167,8 @@ struct Staged {
     DevBuf in;
     size_t used = 0;
     std::vector<uint64_t> in_off;
+    int rc = RRG_OK;  // staging failed (an allocation failure: only this batch fails)
+    rrg_error err{};
 };
 
 struct Server {
@@ -189,15 +191,22 @@ struct Server {
     rrg_error fail_err{};
     std::thread stager, decoder;
 
-    // admission charge of a request: its input plus an output estimate (the
-    // trailer's ISIZE of the last member, at least 4x the input)
+    // admission charge of a request: the device bytes the engine holds for
+    // it while it decodes (input, cell regions and output; the output size
+    // is estimated from the last member's ISIZE, at least 4x the input)
     static uint64_t estimate(const std::vector<uint8_t>& d) {
         uint64_t isz = 0;
         const size_t n = d.size();
         if (n >= 4)
             isz = (uint64_t)d[n - 4] | ((uint64_t)d[n - 3] << 8) | ((uint64_t)d[n - 2] << 16) |
                   ((uint64_t)d[n - 1] << 24);
-        return n + std::max<uint64_t>(isz, 4 * (uint64_t)n) + kAlign + kPad;
+        return engine_footprint(n, std::max<uint64_t>(isz, 4 * (uint64_t)n)) + kAlign + kPad;
+    }
+
+    // After an allocation failure: give back every cached device buffer so
+    // that the next batch starts from free memory.
+    void drop_cached() {
+        for (DevBuf* b : c.bufs()) b->release();
     }
 
     void stage_loop() {
@@ -231,11 +240,17 @@ struct Server {
             for (Request* r : st->reqs) src.push_back(Src{nullptr, r->data.data(), r->data.size()});
             std::vector<std::vector<uint8_t>> hcopy;
             rrg_error e{};
+            t_last_hip = hipSuccess;
             const int rc = load_files(c, st->in, up, src, st->in_off, hcopy, st->used, &e);
             std::lock_guard<std::mutex> lk(m);
             if (rc) {
-                failed = true;
-                fail_err = e;
+                st->rc = rc;
+                st->err = e;
+                if (t_last_hip == hipErrorOutOfMemory) st->in.release();
+                else {  // a device fault: nothing decodes after it
+                    failed = true;
+                    fail_err = e;
+                }
             }
             staged.push_back(std::move(st));
             cv.notify_all();
@@ -269,15 +284,25 @@ struct Server {
             std::vector<Src> src;
             for (Request* r : st->reqs) src.push_back(Src{nullptr, r->data.data(), r->data.size()});
             std::vector<FileResult> results;
-            rrg_error e{};
-            int rc = was_failed ? RRG_E_DEVICE : RRG_OK;
+            rrg_error e = st->err;
+            int rc = was_failed ? RRG_E_DEVICE : st->rc;
+            bool oom = false;
             if (!rc) {
                 // the staged buffer becomes the context's input for this pass
                 std::swap(c.in, st->in);
                 c.in_used = st->used;
                 begin_profile(c, 0);
+                t_last_hip = hipSuccess;
                 rc = run_batch(c, cfg, src, st->in_off, {}, ob->buf, results, &e);
                 std::swap(c.in, st->in);
+                if (rc && t_last_hip == hipErrorOutOfMemory) {
+                    // this batch did not fit: it fails alone, later batches go on
+                    oom = true;
+                    drop_cached();
+                    ob->buf.release();
+                }
+            } else if (!was_failed) {
+                oom = true;  // staging ran out of memory
             }
             std::lock_guard<std::mutex> lk(m);
             const uint64_t seq = ++batches;
@@ -286,7 +311,7 @@ struct Server {
                 held -= r->est;
                 if (rc) {
                     r->res = FileResult{};
-                    r->res.error = failed ? fail_err : e;
+                    r->res.error = (failed && !oom) ? fail_err : e;
                     if (!r->res.error.kind) r->res.error.kind = rc;
                 } else {
                     r->res = std::move(results[k]);
@@ -303,11 +328,13 @@ struct Server {
                 --pending;
             }
             if (ob->live == 0) free_out.push_back(ob);
-            if (rc && !failed) {
+            if (rc && !failed && !oom) {
                 failed = true;
                 fail_err = e;
             }
             st->reqs.clear();
+            st->rc = RRG_OK;
+            st->err = rrg_error{};
             free_in.push_back(std::move(st));
             cv.notify_all();
         }
@@ -423,7 +450,9 @@ int rrg_server_poll(void* srv, rrg_completion* out, int max, int timeout_ms) {
     while (k < max && !S.completed.empty()) {
         const uint64_t id = S.completed.front();
         S.completed.pop_front();
-        const Request* r = S.reqs[id];
+        auto it = S.reqs.find(id);
+        if (it == S.reqs.end()) continue;  // released before it was polled
+        const Request* r = it->second;
         rrg_completion& o = out[k++];
         std::memset(&o, 0, sizeof o);
         o.id = r->id;
@@ -479,6 +508,9 @@ int rrg_server_release(void* srv, uint64_t id) {
     S.held -= std::min<uint64_t>(S.held, r->res.out_len);
     if (r->block && --r->block->live == 0) S.free_out.push_back(r->block);
     S.reqs.erase(it);
+    // a completion not yet polled is dropped with it
+    auto cit = std::find(S.completed.begin(), S.completed.end(), id);
+    if (cit != S.completed.end()) S.completed.erase(cit);
     delete r;
     S.cv.notify_all();
     return RRG_OK;

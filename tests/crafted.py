@@ -160,3 +160,242 @@ def dynamic_header(precode, seq, hlit, hdist, hclen=19, lead=0, bfinal=0):
         elif sym == 18:
             w.put(extra, 7)
     return w, len(w.bits) - lead
+
+
+# ---------------------------------------------------------------------------
+# Streams that pin the block finder's candidate choice
+# ---------------------------------------------------------------------------
+# A complete precode over all 19 code-length symbols: 13 of 4 bits, 6 of 5.
+# The 5-bit codes (the all-ones one among them) go to symbols 10-14, which no
+# header here uses, so no header has nine one bits in a row.
+PRECODE_19 = [4] * 9 + [5] * 6 + [4] * 4
+
+
+def length_seq(lengths):
+    """(code-length symbol, extra) pairs for a list of code lengths: zero runs
+    of 3-10 as symbol 17, of 11-138 as symbol 18, other lengths literally."""
+    seq, i = [], 0
+    while i < len(lengths):
+        if lengths[i] == 0:
+            j = i
+            while j < len(lengths) and lengths[j] == 0 and j - i < 138:
+                j += 1
+            run = j - i
+            if run >= 11:
+                seq.append((18, run - 11))
+                i = j
+                continue
+            if run >= 3:
+                seq.append((17, run - 3))
+                i = j
+                continue
+        seq.append((lengths[i], 0))
+        i += 1
+    return seq
+
+
+def put_header(w, lit_lens, dist_lens, bfinal=0):
+    """Append a dynamic block header (complete precode) to BitWriter w."""
+    hdr, _ = dynamic_header(PRECODE_19, length_seq(list(lit_lens) + list(dist_lens)), len(lit_lens), len(dist_lens),
+                            bfinal=bfinal)
+    w.bits += hdr.bits
+
+
+# "Byte code": literals 0..254 in 8 bits (code = the byte value), literal 255
+# and end-of-block in 9 bits (111111110, 111111111), one 1-bit distance code.
+# Both codes pass the strict parse (the literal code is complete, a one-symbol
+# distance code is allowed), and any bit string without nine ones in a row can
+# be written as a run of its literals.
+BYTE_LIT = [8] * 255 + [9, 9]
+ONE_DIST = [1]
+BYTE_CODES = canonical_codes(BYTE_LIT)
+
+
+def put_literal(w, b, out):
+    w.put_code(BYTE_CODES[b], BYTE_LIT[b])
+    out.append(b)
+
+
+def put_eob(w):
+    w.put_code(BYTE_CODES[256], 9)
+
+
+def put_bits_as_literals(w, bits, out):
+    """Write `bits` (stream order) into a byte-code block as literals: each
+    group of 8 is the code of the literal of that value, a group of eight
+    ones followed by a zero is literal 255.  Zeros complete the last group."""
+    bits = list(bits)
+    i = 0
+    while i < len(bits):
+        g = (bits[i:i + 8] + [0] * 8)[:8]
+        b = sum(bit << (7 - k) for k, bit in enumerate(g))
+        if b != 255:
+            put_literal(w, b, out)
+            i += 8
+        else:
+            assert bits[i:i + 9] == [1] * 8 + [0], "nine ones in a row cannot be written"
+            put_literal(w, 255, out)
+            i += 9
+
+
+def put_filler(w, nbits, out, seed):
+    """Exactly nbits (>= 63) of byte-code literals: printable bytes in 8 bits,
+    with nbits mod 8 literals 255 (9 bits each) first."""
+    assert nbits >= 63
+    nine = nbits % 8
+    eight = (nbits - 9 * nine) // 8
+    for _ in range(nine):
+        put_literal(w, 255, out)
+    for b in printable(eight, seed):
+        put_literal(w, b, out)
+
+
+def put_literals(w, data, out):
+    for b in data:
+        put_literal(w, b, out)
+
+
+# The code of the trial-decode candidates: literal 'A' = 0, end-of-block = 10,
+# length 3 (symbol 257) = 11, and one distance code (0 = distance 1).  The bit
+# 1 after a length symbol has no distance code: "no matching code".
+TRIAL_LIT = [0] * 258
+TRIAL_LIT[65], TRIAL_LIT[256], TRIAL_LIT[257] = 1, 2, 2
+
+
+def trial_candidate_bits(k, last="bad"):
+    """A non-final dynamic header with the trial code, then k - 1 valid
+    symbols (literals, every third a length-3 match at distance 1), then at
+    symbol k either an undecodable distance ("bad") or end-of-block ("eob")."""
+    w = BitWriter()
+    put_header(w, TRIAL_LIT, ONE_DIST)
+    for i in range(1, k):
+        if i % 3 == 0:
+            w.put_code(0b11, 2)  # length 3
+            w.put_code(0b0, 1)   # distance 1
+        else:
+            w.put_code(0b0, 1)   # literal 'A'
+    if last == "bad":
+        w.put_code(0b11, 2)
+        w.put_code(0b1, 1)
+    else:
+        w.put_code(0b10, 2)
+    return w.bits
+
+
+def gzip_member(body_bits, payload):
+    w = BitWriter()
+    w.bits = list(body_bits)
+    return HEADER + w.tobytes() + trailer(payload)
+
+
+CHUNK_TARGET_BITS = 65536 * 8  # the first chunk target of a 64 KiB chunk size
+
+
+def finder_trap(kind, seed=11):
+    """A gzip member built around its first chunk target T (bit 524288 of the
+    deflate body, chunk size 64 KiB).  Block 1 is a byte-code block; what sits
+    at T depends on `kind`:
+
+      "trial64" / "trial65": a false candidate written as block-1 literals --
+          a dynamic header whose trial symbol 64 / 65 cannot be decoded.  The
+          reference's 64-symbol trial rejects the first and accepts the
+          second (which then fails as a chunk start: a speculation failure).
+      "skip": eleven one bits (a literal ending in 11, then block 1's
+          end-of-block), then block 2's header at T + 11.  The 13-bit window
+          at T has no candidate at any shift, so the NEXT_LUT chain jumps to
+          T + 13 and never tests T + 11; the reference's find is block 3.
+
+    Returns (file, payload, info) with info = dict(B2, B3, T, block_out=[...])
+    (block starts as body bit offsets, the output length of each block)."""
+    T = CHUNK_TARGET_BITS
+    w = BitWriter()
+    outs = []
+    out = []
+    put_header(w, BYTE_LIT, ONE_DIST)
+    if kind in ("trial64", "trial65"):
+        put_filler(w, T - len(w.bits), out, seed)
+        assert len(w.bits) == T
+        put_bits_as_literals(w, trial_candidate_bits(64 if kind == "trial64" else 65), out)
+        put_literals(w, printable(2000, seed + 1), out)
+        put_eob(w)
+    else:
+        put_filler(w, T - 6 - len(w.bits), out, seed)
+        put_literal(w, 35, out)  # code 00100011: ends in two ones at T, T + 1
+        put_eob(w)               # nine ones: T + 2 .. T + 10
+        assert len(w.bits) == T + 11
+    outs.append(out)
+    B2 = len(w.bits)
+    out = []
+    put_header(w, BYTE_LIT, ONE_DIST)
+    put_literals(w, printable(3000, seed + 2), out)
+    put_eob(w)
+    outs.append(out)
+    B3 = len(w.bits)
+    out = []
+    put_header(w, BYTE_LIT, ONE_DIST)
+    put_literals(w, printable(1500, seed + 3), out)
+    put_eob(w)
+    outs.append(out)
+    out = []
+    put_header(w, BYTE_LIT, ONE_DIST, bfinal=1)
+    put_literals(w, printable(700, seed + 4), out)
+    put_eob(w)
+    outs.append(out)
+    payload = b"".join(bytes(o) for o in outs)
+    return gzip_member(w.bits, payload), payload, dict(T=T, B2=B2, B3=B3, block_out=[len(o) for o in outs])
+
+
+def next_lut_brute(w):
+    """NEXT_LUT[w] from its definition (block_finder.rs:84-101): the smallest
+    shift j in [0, 13) whose zero-padded window passes the 13-bit predicate."""
+    def cand(b):
+        return (b & 1) == 0 and ((b >> 1) & 3) == 2 and ((b >> 3) & 31) <= 29 and ((b >> 8) & 31) <= 29
+
+    return next((j for j in range(13) if cand(w >> j)), 13)
+
+
+_NEXT_LUT = []
+
+
+def chain_positions(body, start, stop):
+    """Positions the reference's skip chain visits from `start` (below `stop`),
+    with the candidates (jump 0) marked: [(pos, is_candidate)].  A candidate
+    that does not verify is left by one bit."""
+    if not _NEXT_LUT:
+        _NEXT_LUT.extend(next_lut_brute(w) for w in range(1 << 13))
+    padded = bytes(body) + b"\0\0\0"
+    nbits = len(body) * 8
+    out, p = [], start
+    while p < stop and p + 13 <= nbits:
+        q = p >> 3
+        s = _NEXT_LUT[(int.from_bytes(padded[q:q + 3], "little") >> (p & 7)) & 0x1FFF]
+        out.append((p, s == 0))
+        p += s if s else 1
+    return out
+
+
+def bgzf_stored_member(data):
+    """One BGZF member (BC extra field, BSIZE) holding `data` in a stored block:
+    31 + len(data) bytes."""
+    assert len(data) <= 65535
+    size = 18 + 5 + len(data) + 8
+    head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff" + struct.pack("<H", 6) + b"BC" + struct.pack("<HH", 2, size - 1)
+    block = b"\x01" + struct.pack("<HH", len(data), len(data) ^ 0xFFFF) + data
+    return head + block + trailer(data)
+
+
+def synced_member_starts(pieces, level=6):
+    """As synced_member, also returning the byte offset (in the deflate body)
+    where each piece's data block starts: zlib writes each piece as one block
+    right after the previous sync flush, which ends on a byte boundary."""
+    co = zlib.compressobj(level, zlib.DEFLATED, -15)
+    body, starts, n = [], [], 0
+    for p in pieces:
+        assert len(p) < 16000
+        starts.append(n)
+        a = co.compress(p) + co.flush(zlib.Z_SYNC_FLUSH)
+        body.append(a)
+        n += len(a)
+    body.append(co.flush(zlib.Z_FINISH))
+    payload = b"".join(pieces)
+    return b"".join(body), payload, starts

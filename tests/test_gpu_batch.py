@@ -209,3 +209,51 @@ def test_server_admission_waits_for_release(cases):
             assert srv.poll(4, timeout_ms=200) == []  # held output blocks the next admission
             srv.release(c.id)
             seen += 1
+
+
+def test_server_release_before_poll(cases):
+    """A completion released before it was polled is dropped, not returned."""
+    import time
+
+    import rusty_rapidgzip_amd as rrg
+
+    cfg, files, alone = cases
+    with rrg.Server(0, cfg) as srv:
+        rid = srv.submit(files[0][1])
+        deadline = time.time() + 60
+        while not srv.chunk_ends(rid) and time.time() < deadline:
+            time.sleep(0.01)
+        assert srv.chunk_ends(rid), "request did not complete"
+        srv.release(rid)
+        assert srv.poll(4, timeout_ms=100) == []
+        rid2 = srv.submit(files[1][1])
+        got = srv.poll(4, timeout_ms=60000)
+        assert [c.id for c in got] == [rid2]
+        c = got[0]
+        _check(files[1][0], alone[1], _split(srv.result(rid2, c.out_len), srv.chunk_ends(rid2)), c.stats, c.error)
+        srv.release(rid2)
+
+
+def test_server_recovers_from_allocation_failure(cases):
+    """A batch whose device allocation fails reports an error for its own
+    requests only; later requests still decode to their solo results."""
+    import rusty_rapidgzip_amd as rrg
+
+    cfg, files, alone = cases
+    with rrg.Server(0, cfg) as srv:
+        rrg.debug_fail_alloc(3)  # an allocation of the first batch's decode fails
+        try:
+            rid = srv.submit(files[0][1])
+            got = srv.poll(4, timeout_ms=60000)
+        finally:
+            rrg.debug_fail_alloc(0)
+        assert [c.id for c in got] == [rid]
+        assert isinstance(got[0].error, rrg.DeviceError), got[0].error
+        srv.release(rid)
+        for i in (0, 3, 7):
+            r = srv.submit(files[i][1])
+            c = srv.poll(4, timeout_ms=60000)[0]
+            assert c.id == r
+            _check(files[i][0], alone[i], _split(srv.result(r, c.out_len) if c.out_len else b"",
+                                                 srv.chunk_ends(r)), c.stats, c.error)
+            srv.release(r)
