@@ -158,6 +158,22 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
 
 void rrg_gpu_last_profile(void* ctx, rrg_profile* out);
 
+/* No reference counterpart: rrg_gpu_decode_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call allocates nothing: every device
+ * buffer of the decode comes from one 256 MiB device array of the library,
+ * held per device from its first use until the process ends, and the output
+ * goes to the caller's d_out (256-byte aligned, out_cap >= decompressed size
+ * + 64 bytes).  Inputs of at most 8 MiB (compressed); a larger input, an
+ * output buffer too small or a decode that needs more workspace is RRG_E_ARG.
+ * The array is shared by every call on a device and keeps the last call's
+ * contents: calls that overlap on one device race.  Results are those of
+ * rrg_gpu_decode_device. */
+int rrg_gpu_decode_device_noalloc(void* ctx, const void* d_file, size_t len, const uint8_t* h_file,
+                                  const rrg_cfg* cfg, uint8_t* d_out, size_t out_cap, uint64_t* out_len,
+                                  uint64_t* chunk_ends, size_t chunk_cap, size_t* n_chunks, rrg_stats* stats,
+                                  rrg_error* err);
+
 /* ---- batched form ----------------------------------------------------- */
 
 /* One file of a batch.  Inputs: d_file (device memory) or, when NULL, h_file

@@ -149,6 +149,7 @@ EXPORTED_SYMBOLS = (
     "rrg_server_pending",
     "rrg_server_destroy",
     "rrg_gpu_debug_fail_alloc",
+    "rrg_gpu_decode_device_noalloc",
 )
 
 
@@ -171,6 +172,10 @@ def _lib() -> ctypes.CDLL:
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(_Cfg),
             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
             ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
+        lib.rrg_gpu_decode_device_noalloc.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(_Cfg), ctypes.c_void_p,
+            ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+            ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
         lib.rrg_gpu_last_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(Profile)]
         lib.rrg_gpu_decode_fd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_Cfg), _CB, ctypes.c_void_p,
                                           ctypes.POINTER(_Stats), ctypes.POINTER(_Err)]
@@ -448,6 +453,26 @@ class Decoder:
             _raise(rc, er)
         stats = DecodeStats(st.compressed_bytes, st.uncompressed_bytes, st.chunks_decoded, st.speculation_failures)
         return d_out.value or 0, out_len.value, [ends[i] for i in range(min(n.value, max_chunks))], stats
+
+    def decode_device_noalloc(self, d_ptr: int, length: int, d_out: int, out_cap: int, host: bytes | None = None,
+                              config: Config | None = None, max_chunks: int = 1 << 16):
+        """Device-resident decode that allocates no device memory
+        (rrg_gpu_decode_device_noalloc): the output goes to the caller's
+        buffer d_out.  Returns (out_len, chunk_ends, stats)."""
+        config = config or Config()
+        ends = (ctypes.c_uint64 * max_chunks)()
+        out_len = ctypes.c_uint64()
+        n = ctypes.c_size_t()
+        st, er = _Stats(), _Err()
+        hptr, _keep = _host_ptr(host)
+        rc = self._lib.rrg_gpu_decode_device_noalloc(self._ctx, ctypes.c_void_p(d_ptr), length, hptr,
+                                                     ctypes.byref(config._c()), ctypes.c_void_p(d_out), out_cap,
+                                                     ctypes.byref(out_len), ends, max_chunks, ctypes.byref(n),
+                                                     ctypes.byref(st), ctypes.byref(er))
+        if rc != RRG_OK:
+            _raise(rc, er)
+        stats = DecodeStats(st.compressed_bytes, st.uncompressed_bytes, st.chunks_decoded, st.speculation_failures)
+        return out_len.value, [ends[i] for i in range(min(n.value, max_chunks))], stats
 
     def autotune(self, d_ptr: int, length: int, host: bytes | None = None, config: Config | None = None,
                  reps: int = 2) -> tuple[Config, float]:

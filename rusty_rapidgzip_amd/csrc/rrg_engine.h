@@ -23,6 +23,7 @@ struct Arena {
     uint8_t* base = nullptr;
     size_t cap = 0, used = 0, high = 0;
     bool exhausted = false;
+    size_t fixed_short = 0;  // bytes a caller-owned (fixed) buffer was asked to hold beyond its capacity
     void* take(size_t n) {
         const size_t a = (used + 255) & ~(size_t)255;
         if (a > cap || n > cap - a) {
@@ -45,7 +46,10 @@ struct DevBuf {
     Arena* arena = nullptr;  // carve from this arena instead of allocating
     bool fixed = false;      // caller-owned memory of capacity cap: never reallocated
     hipError_t take(void** q, size_t n) {
-        if (fixed) return hipErrorOutOfMemory;
+        if (fixed) {
+            if (arena) arena->fixed_short = n;
+            return hipErrorOutOfMemory;
+        }
         if (arena) {
             *q = arena->take(n);
             return *q ? hipSuccess : hipErrorOutOfMemory;
@@ -234,6 +238,11 @@ void begin_profile(Ctx& c, uint64_t len);
 void print_profile(const Ctx& c);
 // Decode every job (inputs already in c.in[0, c.in_used)) into `out`.
 int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out, rrg_error* err);
+// The device-resident decode of rrg_gpu_decode_device into c.out (the file at
+// d_file in device memory); the out-parameters as there, minus d_out.
+int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,
+                       uint64_t* out_len, uint64_t* chunk_ends, size_t chunk_cap, size_t* n_chunks, rrg_stats* stats,
+                       rrg_error* err);
 // Create the context's stream and events on `device`; false on failure.
 bool init_ctx(Ctx& c, int device);
 

@@ -1468,6 +1468,57 @@ void print_profile(const Ctx& c) {
                  (unsigned long long)p.n_segments);
 }
 
+int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,
+                       uint64_t* out_len, uint64_t* chunk_ends, size_t chunk_cap, size_t* n_chunks, rrg_stats* stats,
+                       rrg_error* err) {
+    HIPCHK(hipSetDevice(c.device));
+    rrg_cfg cfg;
+    if (cfg_in) cfg = *cfg_in;
+    else rrg_cfg_default(&cfg);
+    const double t0 = now_ms();
+    begin_profile(c, len);
+    c.verbose = cfg.verbose != 0;
+    if (len == 0) {
+        set_err(err, RRG_E_IO, "empty input");
+        return RRG_E_IO;
+    }
+    HIPCHK(c.in.ensure(len + 64));
+    HIPCHK(hipMemcpyAsync(c.in.p, d_file, len, hipMemcpyDeviceToDevice, c.s));
+    HIPCHK(hipMemsetAsync(c.in.as<uint8_t>() + len, 0, 64, c.s));
+    c.in_used = len + 64;
+    std::vector<uint8_t> hcopy;
+    const uint8_t* h = h_file;
+    if (!h) {
+        hcopy.resize(len);
+        HIPCHK(hipMemcpyAsync(hcopy.data(), d_file, len, hipMemcpyDeviceToHost, c.s));
+        HIPCHK(hipStreamSynchronize(c.s));
+        h = hcopy.data();
+    }
+    Decoded dec;
+    int rc = decode_loaded(c, h, len, cfg, dec, err);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c.s));
+    const uint64_t total = dec.chunk_ends.empty() ? 0 : dec.chunk_ends.back();
+    c.prof.uncompressed_bytes = total;
+    c.prof.ms_total = now_ms() - t0;
+    if (cfg.verbose) print_profile(c);
+    if (out_len) *out_len = total;
+    if (n_chunks) *n_chunks = dec.chunk_ends.size();
+    if (chunk_ends)
+        for (size_t i = 0; i < dec.chunk_ends.size() && i < chunk_cap; ++i) chunk_ends[i] = dec.chunk_ends[i];
+    if (stats) {
+        stats->compressed_bytes = len;
+        stats->uncompressed_bytes = total;
+        stats->chunks_decoded = dec.chunk_ends.size();
+        stats->speculation_failures = dec.spec_failures;
+    }
+    if (dec.failed) {
+        if (err) *err = dec.error;
+        return dec.error.kind ? dec.error.kind : RRG_E_INVALID;
+    }
+    return RRG_OK;
+}
+
 bool init_ctx(Ctx& c, int device) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return false;
@@ -1537,53 +1588,10 @@ int rrg_gpu_decode_device(void* ctx, const void* d_file, size_t len, const uint8
         return RRG_E_ARG;
     }
     Ctx& c = *static_cast<Ctx*>(ctx);
-    HIPCHK(hipSetDevice(c.device));
-    rrg_cfg cfg;
-    if (cfg_in) cfg = *cfg_in;
-    else rrg_cfg_default(&cfg);
-    const double t0 = now_ms();
-    begin_profile(c, len);
-    c.verbose = cfg.verbose != 0;
-    if (len == 0) {
-        set_err(err, RRG_E_IO, "empty input");
-        return RRG_E_IO;
-    }
-    HIPCHK(c.in.ensure(len + 64));
-    HIPCHK(hipMemcpyAsync(c.in.p, d_file, len, hipMemcpyDeviceToDevice, c.s));
-    HIPCHK(hipMemsetAsync(c.in.as<uint8_t>() + len, 0, 64, c.s));
-    c.in_used = len + 64;
-    std::vector<uint8_t> hcopy;
-    const uint8_t* h = h_file;
-    if (!h) {
-        hcopy.resize(len);
-        HIPCHK(hipMemcpyAsync(hcopy.data(), d_file, len, hipMemcpyDeviceToHost, c.s));
-        HIPCHK(hipStreamSynchronize(c.s));
-        h = hcopy.data();
-    }
-    Decoded dec;
-    int rc = decode_loaded(c, h, len, cfg, dec, err);
-    if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(c.s));
-    const uint64_t total = dec.chunk_ends.empty() ? 0 : dec.chunk_ends.back();
-    c.prof.uncompressed_bytes = total;
-    c.prof.ms_total = now_ms() - t0;
-    if (cfg.verbose) print_profile(c);
+    const int rc = decode_device_impl(c, d_file, len, h_file, cfg_in, out_len, chunk_ends, chunk_cap, n_chunks,
+                                      stats, err);
     if (d_out) *d_out = c.out.as<uint8_t>();
-    if (out_len) *out_len = total;
-    if (n_chunks) *n_chunks = dec.chunk_ends.size();
-    if (chunk_ends)
-        for (size_t i = 0; i < dec.chunk_ends.size() && i < chunk_cap; ++i) chunk_ends[i] = dec.chunk_ends[i];
-    if (stats) {
-        stats->compressed_bytes = len;
-        stats->uncompressed_bytes = total;
-        stats->chunks_decoded = dec.chunk_ends.size();
-        stats->speculation_failures = dec.spec_failures;
-    }
-    if (dec.failed) {
-        if (err) *err = dec.error;
-        return dec.error.kind ? dec.error.kind : RRG_E_INVALID;
-    }
-    return RRG_OK;
+    return rc;
 }
 
 int rrg_gpu_autotune(void* ctx, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* base, int reps,
