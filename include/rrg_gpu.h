@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define RRG_GPU_ABI_VERSION 2
+#define RRG_GPU_ABI_VERSION 3
 
 /* Error kinds; values match the variants of the reference error enums. */
 enum rrg_error_kind {
@@ -96,7 +96,7 @@ typedef int (*rrg_chunk_cb)(void* user, const uint8_t* data, size_t len);
 typedef struct {
     double ms_total;      /* host wall clock of the whole call */
     double ms_find;       /* block finder kernel */
-    double ms_inflate;    /* speculative inflate kernel(s), retries included */
+    double ms_inflate;    /* inflate pass 1 (Huffman decode to tokens), retries included */
     double ms_tail;       /* tail-chain kernels */
     double ms_resolve;    /* resolve + narrow kernel */
     double ms_crc;        /* CRC kernel */
@@ -106,7 +106,9 @@ typedef struct {
     uint64_t n_tail_rounds;
     uint64_t compressed_bytes;
     uint64_t uncompressed_bytes;
-    uint64_t n_grown;         /* output regions grown after an overflow */
+    uint64_t n_grown;         /* token regions grown after an overflow */
+    double ms_expand;         /* inflate pass 2 (tokens to cells / bytes) */
+    uint64_t n_tokens;        /* tokens of the segments expanded */
 } rrg_profile;
 
 int rrg_gpu_abi_version(void);

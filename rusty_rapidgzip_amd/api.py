@@ -80,6 +80,8 @@ class Profile(ctypes.Structure):
         ("compressed_bytes", ctypes.c_uint64),
         ("uncompressed_bytes", ctypes.c_uint64),
         ("n_grown", ctypes.c_uint64),
+        ("ms_expand", ctypes.c_double),
+        ("n_tokens", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

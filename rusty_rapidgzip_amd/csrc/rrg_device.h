@@ -13,8 +13,8 @@ namespace rrg {
 struct SegDesc {
     uint64_t start_bit;
     uint64_t hint_bit;
-    uint64_t cells_base;  // first u16 cell of this segment's output region
-    uint64_t cells_cap;   // capacity in cells
+    uint64_t tok_base;    // first token of this segment's token region (a multiple of 4)
+    uint64_t tok_cap;     // capacity in tokens (a multiple of 4)
     uint64_t end_byte;    // end of the segment's file in the input buffer (bits past it are not read)
     uint32_t flags;       // SEG_SPEC | SEG_SINGLE_MEMBER | SEG_BLOCK_ENDS
     uint32_t mb_base;     // first MemberRec slot
@@ -32,13 +32,36 @@ enum : uint32_t {
 constexpr uint64_t MB_BLOCK_END = ~0ull;
 
 struct SegResult {
-    uint64_t n_cells;   // cells produced
+    uint64_t n_cells;   // output cells the tokens stand for
     uint64_t end_bit;   // actual_end_bit (pipeline.rs:1176-1182)
     uint32_t status;    // ST_*
     uint32_t detail;    // DET_* for ST_INVALID
     uint32_t final_block;
     uint32_t n_mb;      // member boundaries recorded
+    uint32_t n_tok;     // tokens produced (with ST_OVERFLOW: tokens needed)
+    uint32_t pad;
 };
+
+// Tokens (u32) of the first decode pass; bits 31..30 give the type.
+//   literals  00: count (1..3) in bits 25..24, the bytes in bits 7..0, 15..8, 23..16
+//   match     01: length - 3 in bits 23..16, distance - 1 in bits 14..0
+//   stored    10: byte count (1..65535) in bits 15..0; the next token is its
+//   offset    11: input byte offset of the stored bytes from the segment's
+//                 first input byte, in bits 29..0 (no output of its own)
+enum : uint32_t { TOK_LIT = 0u, TOK_MATCH = 1u, TOK_STORED = 2u, TOK_OFFSET = 3u };
+
+// One segment of the expansion pass: its tokens become n_cells u16 cells at
+// cells[dst] (or, with EXP_U8, bytes at out[dst]: members that cannot hold
+// markers).
+struct ExpandItem {
+    uint64_t tok_base;
+    uint64_t dst;
+    uint64_t in_byte;  // the segment's first input byte (stored-token offsets are relative to it)
+    uint64_t n_cells;
+    uint32_t n_tok;
+    uint32_t flags;    // EXP_U8
+};
+enum : uint32_t { EXP_U8 = 1u };
 
 // Member boundary inside a segment (pipeline.rs:417-427 MemberBoundary), or,
 // with SEG_BLOCK_ENDS, a non-final block end (trailer_byte == MB_BLOCK_END).
@@ -57,8 +80,9 @@ enum : uint32_t {
     ST_METHOD = 4,       // GzipError::UnsupportedMethod
     ST_RESERVED = 5,     // GzipError::ReservedFlags
     ST_TRUNCATED = 6,    // GzipError::Truncated
-    ST_OVERFLOW = 11,    // cell capacity exceeded (host retries bigger)
+    ST_OVERFLOW = 11,    // token capacity exceeded (host retries bigger)
     ST_MB_OVERFLOW = 12, // member-record capacity exceeded (host retries bigger)
+    ST_TOO_BIG = 13,     // a segment's output reaches 2^32 cells, or a stored block lies 1 GiB past its start
 };
 
 enum : uint32_t {

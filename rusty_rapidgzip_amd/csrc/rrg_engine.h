@@ -107,8 +107,8 @@ struct Ctx {
     hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
     uint64_t max_lanes = 0;  // cap on inflate lanes in flight (0 = as many as fit)
-    DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, cells, sorted, chain, tz, tiles, out, poff, plen,
-        pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
+    DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, toks, xitems, cells, sorted, chain, tz,
+        tiles, out, poff, plen, pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
     uint8_t* pin = nullptr;
@@ -119,9 +119,9 @@ struct Ctx {
     hipEvent_t evc[2] = {nullptr, nullptr};  // staging copies of the chunk emission
     // every device buffer of the context
     std::vector<DevBuf*> bufs() {
-        return {&in,    &targets, &limits, &scan_ends, &found, &segs, &res,  &mrec, &work, &counter, &cells,
-                &sorted, &chain,   &tz,     &tiles, &out,  &poff, &plen, &pcrc, &err,     &reach, &first,
-                &stops,  &exits,   &recs,   &legtg, &lfound, &lnext, &exact_idx};
+        return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,
+                &toks,  &xitems,  &cells,  &sorted,    &chain, &tz,    &tiles, &out,  &poff, &plen,
+                &pcrc,  &err,     &reach,  &first,     &stops, &exits, &recs,  &legtg, &lfound, &lnext, &exact_idx};
     }
     ~Ctx() {
         for (DevBuf* b : bufs()) b->release();

@@ -228,23 +228,23 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     return G;
 }
 
-// Output cells of a speculative segment spanning `span` input bytes: room
-// for an expansion of 4, plus the rest of a block of up to 32 KiB of input.
-// A segment decodes on to the first block boundary at or after its hint, so
-// when the next segment's start is a false find (no block starts there) its
-// decode runs into the next segment's range; that room keeps it from
-// overflowing, and the false start then costs one re-decode of the next
-// segment only.  Beyond it the kernel counts the cells needed and the segment
-// is regrown.
-uint64_t seg_cells(uint64_t span) { return ((span + 32768) * 4 + 16384 + 7) & ~7ull; }
+// Token slots of a speculative segment spanning `span` input bytes: 1.5
+// tokens per input byte (gzip -6 text and FASTQ need 0.6-0.8), plus room for
+// the rest of a block of up to 32 KiB of input.  A segment decodes on to the
+// first block boundary at or after its hint, so when the next segment's start
+// is a false find (no block starts there) its decode runs into the next
+// segment's range; that room keeps it from overflowing, and the false start
+// then costs one re-decode of the next segment only.  Beyond it the kernel
+// counts the tokens needed and the segment is regrown.  A multiple of 4.
+uint64_t tok_slots(uint64_t span) { return ((span * 3) / 2 + 16384 + 3) & ~3ull; }
 
 // Device bytes a decode of `len` compressed bytes holds at its peak: the
-// input, the u16 cell regions of its segments (seg_cells over segments of at
-// least 24 KiB) and `out` bytes of output.
+// input, the token regions of its segments (tok_slots over segments of at
+// least 24 KiB), the u16 cells of the output and `out` bytes of output.
 uint64_t engine_footprint(uint64_t len, uint64_t out) {
     const uint64_t n_seg = len / (24 * 1024) + 1;
-    const uint64_t cells = 4 * len + n_seg * (4 * 32768 + 16384 + 8);
-    return len + 64 + 2 * cells + out;
+    const uint64_t toks = (len * 3) / 2 + n_seg * (16384 + 4);
+    return len + 64 + 4 * toks + 2 * (out + 64 * n_seg) + out;
 }
 
 // Member-record slots of a segment spanning `span` input bytes: a few member
@@ -254,33 +254,34 @@ uint32_t mb_slots(uint64_t span, bool block_ends) {
     return block_ends ? (uint32_t)std::min<uint64_t>(8 + span / 4096, 1u << 20) : 4u;
 }
 
-// Give a segment a larger output region (cells or member records) after it
-// overflowed; the old region is abandoned.  A cell overflow reports the cells
-// the whole segment needs (the kernel counts on without storing).
-void grow_segment(SegDesc& d, uint32_t status, uint64_t& cells_used, uint64_t& mb_used, uint64_t needed) {
+// Give a segment a larger token region (or member-record region) after it
+// overflowed; the old region is abandoned.  A token overflow reports the
+// tokens the whole segment needs (the kernel counts on without storing).
+void grow_segment(SegDesc& d, uint32_t status, uint64_t& tok_used, uint64_t& mb_used, uint64_t needed) {
     if (status == ST_MB_OVERFLOW) {
         d.mb_base = (uint32_t)mb_used;
         d.mb_cap = d.mb_cap * 16 + 16;
         mb_used += d.mb_cap;
     } else {
-        const uint64_t cap = (std::max<uint64_t>(needed + 64, 2 * d.cells_cap) + 7) & ~7ull;
-        d.cells_base = (cells_used + 7) & ~7ull;
-        d.cells_cap = cap;
-        cells_used = d.cells_base + cap;
+        const uint64_t cap = (std::max<uint64_t>(needed + 64, 2 * d.tok_cap) + 3) & ~3ull;
+        d.tok_base = (tok_used + 3) & ~3ull;
+        d.tok_cap = std::min<uint64_t>(cap, 0xFFFFFFF0ull);
+        tok_used = d.tok_base + d.tok_cap;
     }
 }
 
 inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
 
-// One launch of the inflate kernel over the segment indices in `work`; results
+// One launch of inflate pass 1 over the segment indices in `work`; results
 // for every segment are copied back into `res`.
 int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vector<SegResult>& res,
-                uint64_t cells_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
+                uint64_t tok_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
     HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * inflate_scratch_u16_per_lane() *
                            sizeof(uint16_t)));
-    HIPCHK(c.cells.grow_keep((cells_used + 64) * sizeof(uint16_t), c.cells.cap, c.s));
+    // tokens of segments decoded by earlier launches stay where they are
+    HIPCHK(c.toks.grow_keep((tok_used + 64) * sizeof(uint32_t), c.toks.cap, c.s));
     HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
     HIPCHK(c.segs.ensure(S * sizeof(SegDesc)));
     // results of segments decoded by earlier launches of this decode stay in
@@ -294,9 +295,9 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     const double before = c.prof.ms_inflate;
     {
         StageTimer t(c, &c.prof.ms_inflate);
-        HIPCHK(launch_inflate(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(),
-                              c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
-                              c.counter.as<uint32_t>(), c.cells.as<uint16_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
+        HIPCHK(launch_tokens(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(), c.mrec.as<MemberRec>(),
+                             c.work.as<uint32_t>(), (uint32_t)work.size(), c.counter.as<uint32_t>(),
+                             c.toks.as<uint32_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
         t.stop();
     }
     c.prof.n_inflate_launches++;
@@ -307,20 +308,42 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
         // the launch's largest outputs and its status counts
         std::vector<std::pair<uint64_t, uint32_t>> big;
         size_t n_over = 0, n_err = 0;
+        uint64_t toks = 0, bytes = 0;
         for (uint32_t j : work) {
             big.push_back({res[j].n_cells, j});
             n_over += overflowed(res[j].status);
             n_err += res[j].status != ST_OK && !overflowed(res[j].status);
+            toks += res[j].n_tok;
+            bytes += (res[j].end_bit - desc[j].start_bit) / 8;
         }
         std::sort(big.rbegin(), big.rend());
-        std::fprintf(stderr, "[rrg-gpu] inflate launch %llu: %zu segments in %.3f ms (%zu out of room, %zu failed); "
-                     "largest outputs:", (unsigned long long)c.prof.n_inflate_launches, work.size(),
-                     c.prof.ms_inflate - before, n_over, n_err);
+        std::fprintf(stderr, "[rrg-gpu] inflate launch %llu: %zu segments in %.3f ms (%zu out of room, %zu failed, "
+                     "%.3f tokens per input byte); largest outputs:", (unsigned long long)c.prof.n_inflate_launches,
+                     work.size(), c.prof.ms_inflate - before, n_over, n_err, bytes ? (double)toks / (double)bytes : 0.0);
         for (size_t k = 0; k < big.size() && k < 4; ++k)
             std::fprintf(stderr, " %llu cells from %llu B", (unsigned long long)big[k].first,
                          (unsigned long long)((res[big[k].second].end_bit - desc[big[k].second].start_bit) / 8));
         std::fprintf(stderr, "\n");
     }
+    return RRG_OK;
+}
+
+// Inflate pass 2 over `items` (tokens to cells, or to bytes of `out`), the
+// largest first so that no long expansion starts last.
+int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* err) {
+    if (items.empty()) return RRG_OK;
+    std::sort(items.begin(), items.end(),
+              [](const ExpandItem& a, const ExpandItem& b) { return a.n_cells > b.n_cells; });
+    HIPCHK(c.xitems.ensure(items.size() * sizeof(ExpandItem)));
+    HIPCHK(c.counter.ensure(16));
+    HIPCHK(hipMemcpyAsync(c.xitems.p, items.data(), items.size() * sizeof(ExpandItem), hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
+    StageTimer t(c, &c.prof.ms_expand);
+    HIPCHK(launch_expand(c.toks.as<uint32_t>(), c.xitems.as<ExpandItem>(), (uint32_t)items.size(),
+                         c.counter.as<uint32_t>(), c.in.as<uint8_t>(), c.cells.as<uint16_t>(), out.as<uint8_t>(),
+                         c.n_cu, c.s));
+    t.stop();
+    for (const ExpandItem& it : items) c.prof.n_tokens += it.n_tok;
     return RRG_OK;
 }
 
@@ -536,7 +559,7 @@ struct Redo {
     size_t job, pos;
     uint64_t start;
     uint32_t status;  // ST_OK: false start; ST_OVERFLOW / ST_MB_OVERFLOW: region too small
-    uint64_t needed;  // cells the segment needs (ST_OVERFLOW)
+    uint64_t needed;  // tokens the segment needs (ST_OVERFLOW)
     uint64_t end_bit = UINT64_MAX;  // where its decode ended (ST_OVERFLOW; the start may be unverified)
 };
 
@@ -569,7 +592,7 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
             break;
         }
         const bool verified = redo.size() == before;  // the chain before it holds no re-decode
-        redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK, r.n_cells});
+        redo.push_back(Redo{job, k, expected, desc[j].start_bit == expected ? r.status : (uint32_t)ST_OK, r.n_tok});
         if (desc[j].start_bit == expected && r.status == ST_OVERFLOW) redo.back().end_bit = r.end_bit;
         if (verified && desc[j].start_bit == expected && r.status == ST_OVERFLOW) {
             // out of cells at a verified start, the lane decoded on without
@@ -798,7 +821,7 @@ void finish_serial(Job& J, const std::vector<SegDesc>& desc, const std::vector<S
 // inside a member decodes such a back-reference into markers; only segments
 // that start less than 32 KiB after their member's start can hold one.
 int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc>& desc,
-                 const std::vector<SegResult>& res, const std::vector<MemberRec>& recs,
+                 const std::vector<SegResult>& res, const std::vector<MemberRec>& recs, const std::vector<ChainSeg>& cs,
                  std::vector<std::vector<uint64_t>>& bad, rrg_error* err) {
     constexpr uint64_t W = 32768;
     std::vector<ReachScan> items;
@@ -819,7 +842,7 @@ int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc
                 check = !J.seg_is_chunk[k] && mstart > cstart;  // not the chunk's first member
             }
             if (check && (desc[j].flags & SEG_SPEC) && o - mstart < W && res[j].n_cells) {
-                items.push_back(ReachScan{desc[j].cells_base, (uint32_t)res[j].n_cells, (uint32_t)(o - mstart)});
+                items.push_back(ReachScan{cs[J.c_lo + nb + k].cells_base, (uint32_t)res[j].n_cells, (uint32_t)(o - mstart)});
                 where.push_back({ji, k});
             }
             for (uint32_t q = 0; q < res[j].n_mb; ++q) {
@@ -1008,7 +1031,7 @@ int finish_open(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<
 // enough input, is cut at the blocks the finder reports inside it (one finder
 // launch for all requests); the pieces join the job's chain after it.
 int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vector<SegDesc>& desc,
-               const std::vector<SegResult>& res, uint64_t& cells_used, uint64_t& mb_used,
+               const std::vector<SegResult>& res, uint64_t& tok_used, uint64_t& mb_used,
                std::vector<uint32_t>& again, rrg_error* err) {
     if (redo.empty()) return RRG_OK;
     constexpr uint64_t kPieceBits = 8 * 2048;  // at least 2 KiB of input per piece
@@ -1033,10 +1056,10 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
     std::vector<uint64_t> found;
     const int frc = run_finder(c, targets, limits, scan_ends, found, err);
     if (frc) return frc;
-    // cells granted beyond the default sizes in this round (a start that is
-    // not verified yet may be a false one, whose counts mean nothing)
+    // token slots granted beyond the default sizes in this round (a start
+    // that is not verified yet may be a false one, whose counts mean nothing)
     uint64_t guessed = 0;
-    constexpr uint64_t kGuessBudget = 1ull << 31;
+    constexpr uint64_t kGuessBudget = 1ull << 30;
     // later positions first, so earlier insertions do not move them
     std::vector<size_t> idx(redo.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
@@ -1054,30 +1077,31 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         // start in [hint, end_bit): cuts stay below the hint, and the last
         // piece's region covers the decode up to end_bit
         const uint64_t last_end = R.end_bit != UINT64_MAX && R.end_bit > end ? R.end_bit : end;
-        // expected output cells per input byte in this range: what the
-        // segment itself counted when it ran out of room, else what its chain
-        // predecessor produced (pieces of a high-ratio stretch would
-        // otherwise overflow again)
+        // expected tokens per input byte in this range: what the segment
+        // itself counted when it ran out of room, else what its chain
+        // predecessor produced (pieces of a dense stretch would otherwise
+        // overflow again)
         double ratio = 0;
         if (R.status == ST_OVERFLOW && R.end_bit != UINT64_MAX && R.end_bit > R.start + 8 * 1024) {
             ratio = (double)R.needed / (double)((R.end_bit - R.start) / 8);
         } else if (R.pos > 0) {
             const uint32_t q = J.order[R.pos - 1];
             if (res[q].status == ST_OK && res[q].end_bit == R.start && res[q].end_bit > desc[q].start_bit + 8 * 1024)
-                ratio = (double)res[q].n_cells / (double)((res[q].end_bit - desc[q].start_bit) / 8);
+                ratio = (double)res[q].n_tok / (double)((res[q].end_bit - desc[q].start_bit) / 8);
         }
-        // at most DEFLATE's largest expansion; the region of one piece at most
-        // kMaxGuess cells (beyond it the kernel counts and the piece is regrown)
+        // at most 4 tokens per input byte (a token takes at least 2 bits, a
+        // literal triple at least 3); one piece's region at most kMaxGuess
+        // tokens (beyond it the kernel counts and the piece is regrown)
         constexpr uint64_t kMaxGuess = 1ull << 26;
-        ratio = std::min(ratio, 1032.0);
+        ratio = std::min(ratio, 4.0);
         auto guess = [&](uint64_t span) -> uint64_t {
-            const uint64_t g = std::min<uint64_t>(kMaxGuess, ((uint64_t)((double)span * ratio * 1.25) + 65536 + 7) & ~7ull);
+            const uint64_t g = std::min<uint64_t>(kMaxGuess, ((uint64_t)((double)span * ratio * 1.25) + 16384 + 3) & ~3ull);
             if (guessed + g > kGuessBudget) return 0;
             guessed += g;
             return g;
         };
         if (overflowed(R.status)) {
-            grow_segment(d, R.status, cells_used, mb_used, R.needed);
+            grow_segment(d, R.status, tok_used, mb_used, R.needed);
             ++c.prof.n_grown;
         }
         d.start_bit = R.start;
@@ -1088,11 +1112,11 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
         {
             // the segment's own region, for its (new) range
             const uint64_t span0 = ((cuts.empty() ? last_end : cuts[0]) - R.start) / 8;
-            const uint64_t want = guess(span0);
-            if (ratio > 0 && !overflowed(R.status) && d.cells_cap < want) {
-                d.cells_base = (cells_used + 7) & ~7ull;
-                d.cells_cap = want;
-                cells_used = d.cells_base + want;
+            const uint64_t want = std::max<uint64_t>(guess(span0), ratio > 0 ? 0 : tok_slots(span0));
+            if (!overflowed(R.status) && d.tok_cap < want) {
+                d.tok_base = (tok_used + 3) & ~3ull;
+                d.tok_cap = want;
+                tok_used = d.tok_base + want;
             }
         }
         again.push_back(j);
@@ -1105,9 +1129,9 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
             p.hint_bit = q + 1 < cuts.size() ? cuts[q + 1] : hint;
             p.end_byte = d.end_byte;
             const uint64_t span = ((q + 1 < cuts.size() ? cuts[q + 1] : last_end) - cuts[q]) / 8;
-            p.cells_base = cells_used;
-            p.cells_cap = std::max<uint64_t>(seg_cells(span), ratio > 0 ? guess(span) : 0);
-            cells_used += p.cells_cap;
+            p.tok_base = (tok_used + 3) & ~3ull;
+            p.tok_cap = std::max<uint64_t>(tok_slots(span), ratio > 0 ? guess(span) : 0);
+            tok_used = p.tok_base + p.tok_cap;
             p.flags = SEG_SPEC | (d.flags & SEG_BLOCK_ENDS);
             p.mb_base = (uint32_t)mb_used;
             p.mb_cap = mb_slots(span, p.flags & SEG_BLOCK_ENDS);
@@ -1169,7 +1193,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     if (rc) return rc;
     // 3. segment descriptors
     std::vector<SegDesc> desc;
-    uint64_t cells_used = 0, mb_used = 0;
+    uint64_t tok_used = 0, mb_used = 0;
     for (Job& J : jobs) {
         // BGZF members: one single-member segment each, output sized from ISIZE
         J.bg_lo = desc.size();
@@ -1192,9 +1216,11 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
                 d.hint_bit = UINT64_MAX;
                 d.end_byte = J.in_off + J.len;
             }
-            d.cells_base = cells_used;
-            d.cells_cap = ((std::min<uint64_t>(isz, 1u << 22) + 64) + 7) & ~7ull;
-            cells_used += d.cells_cap;
+            // a member's tokens stand for at least one byte each and take at
+            // least two bits each: at most ISIZE and 4 per input byte
+            d.tok_base = tok_used;
+            d.tok_cap = (std::min<uint64_t>(isz, 4 * sz) + 64 + 3) & ~3ull;
+            tok_used += d.tok_cap;
             d.flags = SEG_SINGLE_MEMBER;
             d.mb_base = (uint32_t)mb_used;
             d.mb_cap = 1;
@@ -1234,9 +1260,9 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             d.hint_bit = j + 1 < S ? M[j + 1] : UINT64_MAX;
             d.end_byte = J.in_off + J.len;
             const uint64_t span = ((j + 1 < S ? M[j + 1] : J.end_bits()) - M[j]) / 8;
-            d.cells_base = cells_used;
-            d.cells_cap = seg_cells(span);
-            cells_used += d.cells_cap;
+            d.tok_base = tok_used;
+            d.tok_cap = tok_slots(span);
+            tok_used += d.tok_cap;
             d.flags = (j ? SEG_SPEC : 0u) | (J.serial ? SEG_BLOCK_ENDS : 0u);
             d.mb_base = (uint32_t)mb_used;
             d.mb_cap = mb_slots(span, J.serial);
@@ -1253,7 +1279,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<SegResult> res;
     std::vector<uint32_t> all(S);
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
-    rc = run_inflate(c, G, desc, res, cells_used, mb_used, all, err);
+    rc = run_inflate(c, G, desc, res, tok_used, mb_used, all, err);
     if (rc) return rc;
     for (Job& J : jobs) {
         // a capacity overflow is not a decode failure (the region is grown below)
@@ -1272,7 +1298,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             if (round < 6)
                 for (size_t j = J.bg_lo; j < J.bg_hi; ++j)
                     if (overflowed(res[j].status)) {
-                        grow_segment(desc[j], res[j].status, cells_used, mb_used, res[j].n_cells);
+                        grow_segment(desc[j], res[j].status, tok_used, mb_used, res[j].n_tok);
                         ++c.prof.n_grown;
                         again.push_back((uint32_t)j);
                     }
@@ -1289,20 +1315,28 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             std::fprintf(stderr, "[rrg-gpu] settle round %d: %zu false starts, %zu out of room, %zu regrown alone\n",
                          round, n_false, n_over, again.size());
         }
-        rc = split_redo(c, jobs, redo, desc, res, cells_used, mb_used, again, err);
+        rc = split_redo(c, jobs, redo, desc, res, tok_used, mb_used, again, err);
         if (rc) return rc;
         c.prof.n_redecoded += again.size();
-        rc = run_inflate(c, G, desc, res, cells_used, mb_used, again, err);
+        rc = run_inflate(c, G, desc, res, tok_used, mb_used, again, err);
         if (rc) return rc;
     }
     c.prof.n_segments += desc.size();
     // the serial paths also send output of the failing segment (the blocks
     // before the failure, or a member whose trailer is cut off)
     for (Job& J : jobs) J.n_out = J.n_chain + (J.serial && J.fatal >= 0 ? 1 : 0);
-    // 6. output layout (each file at a 256-byte aligned offset) and the chain
+    // 6. output layout (each file at a 256-byte aligned offset), the chain and
+    // the expansion.  Chain segments get exact cell regions in chain order
+    // (64-cell aligned: the expansion stores whole lines); BGZF members hold
+    // no markers and expand straight into the output bytes.
     std::vector<ChainSeg> cs;
     std::vector<uint64_t> cs_out;
-    uint64_t out_total = 0;
+    std::vector<ExpandItem> items;
+    uint64_t out_total = 0, cells_total = 0;
+    // the tokens of a segment that overflowed its region are not all stored
+    auto expandable = [&](size_t j) -> uint32_t {
+        return overflowed(res[j].status) ? 0u : (uint32_t)std::min<uint64_t>(res[j].n_tok, desc[j].tok_cap);
+    };
     for (Job& J : jobs) {
         J.dec.out_base = out_total;
         J.c_lo = cs.size();
@@ -1310,16 +1344,23 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         uint64_t o = 0;
         for (size_t j = J.bg_lo; j < J.bg_hi; ++j) {
             J.off.push_back(o);
-            cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, 0u, 1u});
+            cs.push_back(ChainSeg{0, 0, 0, 0u, 1u});
             cs_out.push_back(out_total + o);
+            if (res[j].n_cells)
+                items.push_back(ExpandItem{desc[j].tok_base, out_total + o, desc[j].start_bit >> 3, res[j].n_cells,
+                                           expandable(j), EXP_U8});
             o += res[j].n_cells;
         }
         for (size_t k = 0; k < J.n_out; ++k) {
             const size_t j = J.order[k];
             J.off.push_back(o);
-            cs.push_back(ChainSeg{desc[j].cells_base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
-                                  k == 0 ? 1u : 0u});
+            const uint64_t base = cells_total;
+            cells_total = (cells_total + res[j].n_cells + 63) & ~63ull;
+            cs.push_back(ChainSeg{base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u, k == 0 ? 1u : 0u});
             cs_out.push_back(out_total + o);
+            if (res[j].n_cells)
+                items.push_back(ExpandItem{desc[j].tok_base, base, desc[j].start_bit >> 3, res[j].n_cells,
+                                           expandable(j), 0u});
             o += res[j].n_cells;
         }
         J.off.push_back(o);
@@ -1327,6 +1368,9 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         out_total = (out_total + o + 255) & ~255ull;
     }
     HIPCHK(out.ensure(out_total + 64));
+    HIPCHK(c.cells.ensure((cells_total + 64) * sizeof(uint16_t)));
+    rc = run_expand(c, items, out, err);
+    if (rc) return rc;
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
         HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
@@ -1335,7 +1379,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // back-references before their member's start (read before the tail pass
     // rewrites markers)
     std::vector<std::vector<uint64_t>> bad(jobs.size());
-    rc = member_reach(c, jobs, desc, res, recs, bad, err);
+    rc = member_reach(c, jobs, desc, res, recs, cs, bad, err);
     if (rc) return rc;
     std::vector<uint32_t> flags;
     rc = resolve_chain(c, cs, cs_out, out, flags, err);
@@ -1461,11 +1505,11 @@ void print_profile(const Ctx& c) {
     const rrg_profile& p = c.prof;
     std::fprintf(stderr,
                  "[rrg-gpu] %llu -> %llu bytes in %.3f ms | find %.3f inflate %.3f (x%llu, %llu re-decoded, %llu "
-                 "grown) tail %.3f (%llu blocks) resolve %.3f crc %.3f | %llu segments\n",
+                 "grown) expand %.3f (%llu tokens) tail %.3f (%llu blocks) resolve %.3f crc %.3f | %llu segments\n",
                  (unsigned long long)p.compressed_bytes, (unsigned long long)p.uncompressed_bytes, p.ms_total,
                  p.ms_find, p.ms_inflate, (unsigned long long)p.n_inflate_launches, (unsigned long long)p.n_redecoded,
-                 (unsigned long long)p.n_grown, p.ms_tail, (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc,
-                 (unsigned long long)p.n_segments);
+                 (unsigned long long)p.n_grown, p.ms_expand, (unsigned long long)p.n_tokens, p.ms_tail,
+                 (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc, (unsigned long long)p.n_segments);
 }
 
 int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,

@@ -2,7 +2,8 @@
 //
 // Stages (reference counterparts, paths relative to crates/rusty-rapidgzip/src/):
 //   k_find_blocks   block finder          deflate/block_finder.rs:57-301
-//   k_inflate       speculative decode    pipeline.rs:1080-1185, deflate/fast_inflate.rs:1794-2234
+//   k_tokens        speculative decode    pipeline.rs:1080-1185, deflate/fast_inflate.rs:1794-2234
+//   k_expand        LZ77 copies / markers deflate/fast_inflate.rs:1973-2004, 2209-2222
 //   k_tail_*        tail-window chain     pipeline.rs:742-856, 1043-1078
 //   k_resolve       marker resolution     deflate/speculative.rs:168-213
 //   k_crc_pieces    CRC-32 pieces         pipeline.rs:706-733 (crc32fast update)
@@ -975,8 +976,13 @@ hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const L
 }
 
 // ===========================================================================
-// Speculative inflate: one lane decodes one segment.  Persistent grid; lanes
-// pull segment indices from a device work queue as they finish.
+// Speculative inflate, pass 1 (k_tokens): one lane decodes one segment's
+// Huffman symbols into tokens.  Persistent grid; lanes pull segment indices
+// from a device work queue as they finish.  No output bytes are touched here:
+// a lane's only global loads are its own input words, which it requests about
+// 200 bits ahead of use, so a lane waits on LDS table lookups, not on memory.
+// The copies happen in pass 2 (k_expand), cooperatively and only for the
+// segments that the host's verification chain accepts.
 //
 // Per-lane LDS: literal/length LUT (2^LB u16), distance LUT (128 u16, also the
 // 7-bit precode LUT while a header is read), code lengths (320 B), counts and
@@ -985,9 +991,9 @@ hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const L
 // the canonical slow path, which reads the length-sorted symbols that the
 // builder stored in this lane's global scratch.
 //
-// Cells (u16) go straight to HBM.  A back-reference that reaches before the
-// segment's first byte (first member only) emits marker cells
-// 0x8000 | prefix_offset (fast_inflate.rs:2209-2222); later members must stay
+// A back-reference that reaches before the segment's first byte (first member
+// only) is kept as a match; pass 2 turns its cells into markers
+// 0x8000 | prefix_offset (fast_inflate.rs:2209-2222).  Later members must stay
 // inside themselves (libdeflate_ffi.rs:64-123).
 // ===========================================================================
 constexpr int DIST_LB = 7;
@@ -1111,14 +1117,17 @@ __device__ int parse_gz_header(const uint64_t* __restrict__ in64, uint64_t len, 
     return ST_OK;
 }
 
-enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_STORED, S_BLOCK_END, S_EXIT };
+enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_BLOCK_END, S_EXIT };
+
+// Largest cell count a segment may reach (its cell positions are 32-bit).
+constexpr uint32_t kMaxSegCells = 0xF0000000u;
 
 template <int LB>
-// 4 waves per SIMD (<= 128 VGPRs): the few spills fall in block-header code
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_inflate(const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
-                                                MemberRec* __restrict__ mrec, const uint32_t* __restrict__ work,
-                                                uint32_t n_work, uint32_t* __restrict__ counter,
-                                                uint16_t* __restrict__ cells, uint16_t* __restrict__ sorted_scratch) {
+// 3 waves per SIMD (<= 168 VGPRs): the per-lane LDS tables allow about as many
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_tokens(
+    const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs, SegResult* __restrict__ res,
+    MemberRec* __restrict__ mrec, const uint32_t* __restrict__ work, uint32_t n_work, uint32_t* __restrict__ counter,
+    uint32_t* __restrict__ toks, uint16_t* __restrict__ sorted_scratch) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int LMASK = (1 << LB) - 1;
     const int tid = (int)threadIdx.x;
@@ -1138,26 +1147,68 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     int state = S_FETCH;
     uint32_t seg = 0;
-    uint64_t pos = 0, hint = 0, bitbuf = 0;
+    uint64_t pos = 0, hint = 0, bitbuf = 0, seg_byte = 0;
     uint64_t nbits = 0;  // end of the segment's file in the input buffer (bits)
     uint32_t bitcnt = 0;
-    // output region of the segment (a region holds < 2^32 cells)
-    uint16_t* o = cells;
-    uint32_t out_cap = 0, out_n = 0, member_base = 0;
-    uint32_t spec = 0, mflags = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, stored_left = 0;
-    // counting: the output region is full; decode on without storing so the
+    // input words: A0..A3 = in64[wb .. wb+3] (settled), B0..B3 = in64[wb+4 ..
+    // wb+7] (B1..B3 still in flight after a rotation); a refill reads two
+    // consecutive A words, so the words loaded at a rotation are first used
+    // ~192 bits of input later
+    uint64_t wb = 0, A0 = 0, A1 = 0, A2 = 0, A3 = 0, B0 = 0, B1 = 0, B2 = 0, B3 = 0;
+    uint32_t* tk = toks;
+    uint32_t tok_cap = 0, n_tok = 0;
+    uint32_t out_n = 0, member_base = 0;
+    // up to three literals wait in lit_acc; up to three tokens wait in q1..q3
+    // (q0 is the oldest) and are stored four at a time as one 16-byte write
+    uint32_t lit_acc = 0, lit_n = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0, nq = 0;
+    uint32_t spec = 0, mflags = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0;
+    // counting: the token region is full; decode on without storing so the
     // segment reports the size it needs (one regrow instead of repeated ones)
     uint32_t counting = 0;
     uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
 
-// 64 bits at pos (the input buffer is padded, so words up to 8 past the end
-// exist).  The asm use completes the loads inside the refill, so that code
-// merging with a refill-free path does not inherit a pending-load wait.
-#define RRG_REFILL()                      \
-    do {                                  \
-        bitbuf = bits64_at(in64, pos);    \
-        asm volatile("" : "+v"(bitbuf));  \
-        bitcnt = 64;                      \
+#define RRG_LOAD_ALL(q)                \
+    do {                               \
+        wb = (q);                      \
+        A0 = in64[wb];                 \
+        A1 = in64[wb + 1];             \
+        A2 = in64[wb + 2];             \
+        A3 = in64[wb + 3];             \
+        B0 = in64[wb + 4];             \
+        B1 = in64[wb + 5];             \
+        B2 = in64[wb + 6];             \
+        B3 = in64[wb + 7];             \
+    } while (0)
+// 64 bits at pos from the word cache (the input buffer is padded, so words
+// up to 8 past the end exist).  Past the cached words: rotate by three (the
+// B words were loaded a rotation ago), or reload at a jump.
+#define RRG_REFILL()                                                                   \
+    do {                                                                               \
+        const uint64_t _q = pos >> 6;                                                  \
+        uint64_t _d = _q - wb;                                                         \
+        if (_d >= 3) {                                                                 \
+            if (_d < 6) {                                                              \
+                asm volatile("" : "+v"(B0), "+v"(B1), "+v"(B2));                       \
+                A0 = A3;                                                               \
+                A1 = B0;                                                               \
+                A2 = B1;                                                               \
+                A3 = B2;                                                               \
+                B0 = B3;                                                               \
+                wb += 3;                                                               \
+                B1 = in64[wb + 5];                                                     \
+                B2 = in64[wb + 6];                                                     \
+                B3 = in64[wb + 7];                                                     \
+                _d -= 3;                                                               \
+            } else {                                                                   \
+                RRG_LOAD_ALL(_q);                                                      \
+                asm volatile("" : "+v"(A0), "+v"(A1), "+v"(A2), "+v"(A3));             \
+                _d = 0;                                                                \
+            }                                                                          \
+        }                                                                              \
+        const uint64_t _lo = _d == 0 ? A0 : (_d == 1 ? A1 : A2);                       \
+        const uint64_t _hi = _d == 0 ? A1 : (_d == 1 ? A2 : A3);                       \
+        bitbuf = funnel_lo(_lo, _hi, (uint32_t)(pos & 63));                            \
+        bitcnt = 64;                                                                   \
     } while (0)
 #define RRG_CONSUME(n)           \
     do {                         \
@@ -1182,6 +1233,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         (dst) = _k ? (uint32_t)(bitbuf & ((1ull << _k) - 1)) : 0u; \
         RRG_CONSUME(_k);                                            \
     } while (0)
+// one token into the queue; every fourth completes an aligned 16-byte group
+#define RRG_EMIT(t)                                                                   \
+    do {                                                                              \
+        if (n_tok >= tok_cap) counting = 1;                                           \
+        if (!counting) {                                                              \
+            q0 = q1;                                                                  \
+            q1 = q2;                                                                  \
+            q2 = q3;                                                                  \
+            q3 = (t);                                                                 \
+            if (++nq == 4) {                                                          \
+                *reinterpret_cast<uint4*>(tk + n_tok - 3) = make_uint4(q0, q1, q2, q3); \
+                nq = 0;                                                               \
+            }                                                                         \
+        }                                                                             \
+        ++n_tok;                                                                      \
+    } while (0)
+#define RRG_FLUSH_LITS()                                             \
+    do {                                                             \
+        if (lit_n) {                                                 \
+            RRG_EMIT((TOK_LIT << 30) | (lit_n << 24) | lit_acc);     \
+            lit_n = 0;                                               \
+            lit_acc = 0;                                             \
+        }                                                            \
+    } while (0)
 
     while (true) {
         if (state == S_FETCH) {
@@ -1192,11 +1267,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 seg = work[w];
                 const SegDesc d = segs[seg];
                 pos = d.start_bit;
+                seg_byte = d.start_bit >> 3;
                 hint = d.hint_bit;
                 counting = 0;
                 nbits = d.end_byte * 8;
-                o = cells + d.cells_base;
-                out_cap = (uint32_t)d.cells_cap;
+                tk = toks + d.tok_base;
+                tok_cap = (uint32_t)d.tok_cap;
+                n_tok = 0;
+                nq = 0;
+                lit_n = 0;
+                lit_acc = 0;
                 out_n = 0;
                 member_base = 0;
                 spec = d.flags & SEG_SPEC;
@@ -1212,6 +1292,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     status = ST_EOF;
                     goto finish;
                 }
+                RRG_LOAD_ALL(pos >> 6);
                 RRG_REFILL();
                 state = S_BLOCK_START;
             }
@@ -1219,12 +1300,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (__ballot(state != S_EXIT) == 0) break;
 
         if (state == S_SYM) {
-            // ---- Huffman symbols (fast_inflate.rs:2065-2229) ----
-            // Global stores and loads share one in-order counter (vmcnt), so a
-            // wait for any load also waits for every earlier store.  The loop
-            // is arranged so that a match waits once: the bit-buffer refill
-            // loads are issued together with the copy's source loads, and both
-            // complete before the copy's stores are issued.
+            // ---- Huffman symbols (fast_inflate.rs:2065-2229), as tokens ----
             for (int it = 0; it < 64; ++it) {
                 // 48 bits cover a whole match: litlen <= 15 + 5 extra, dist <= 15 + 13 extra
                 if (bitcnt < 48) RRG_REFILL();
@@ -1237,8 +1313,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (pos + l > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                 RRG_CONSUME(l);
                 if (sym < 256) {
-                    if (out_n >= out_cap) counting = 1;
-                    if (!counting) o[out_n] = (uint16_t)sym;
+                    lit_acc |= sym << (8 * lit_n);
+                    if (++lit_n == 3) {
+                        RRG_EMIT((TOK_LIT << 30) | (3u << 24) | lit_acc);
+                        lit_n = 0;
+                        lit_acc = 0;
+                    }
                     ++out_n;
                     continue;
                 }
@@ -1250,7 +1330,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const uint32_t li = sym - 257;
                 const uint32_t lx = len_extra(li);
                 if (pos + lx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));
+                const uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));
                 RRG_CONSUME(lx);
                 uint32_t de = distlut[bitbuf & ((1u << DIST_LB) - 1)];
                 uint32_t dl = de & 7u, dsym = de >> 3;
@@ -1265,111 +1345,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (pos + dx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                 const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));
                 RRG_CONSUME(dx);
-                // copies store whole groups of 8 cells (up to 7 past the match end,
-                // overwritten by what follows): keep them inside the region
-                if ((uint64_t)out_n + length + 8 > out_cap) counting = 1;
-                const uint32_t cur = out_n - member_base;
-                if ((uint64_t)distance > cur && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
-                if (counting) {
-                    out_n += length;
-                    RRG_REFILL();
-                    continue;
-                }
-                if ((uint64_t)distance > cur) {
-                    const uint32_t gap = (uint32_t)((uint64_t)distance - cur);
-                    const uint32_t pre = gap < length ? gap : length;
-                    for (uint32_t k = 0; k < pre; ++k) o[out_n + k] = (uint16_t)(0x8000u | (gap - k - 1));
-                    out_n += pre;
-                    length -= pre;
-                }
-                // refill loads now, consumed after the first group's source loads
-                const uint64_t rq = pos >> 6;
-                const uint64_t r0 = in64[rq], r1 = in64[rq + 1];
-                if (length && (distance == 1 || distance == 2 || distance == 4)) {
-                    // period 1, 2 or 4: load the pattern once, then store it --
-                    // single cells up to an 8-cell boundary, then whole aligned
-                    // 16-byte groups (the last may run up to 7 cells past the end)
-                    const uint32_t src = out_n - distance;
-                    const uint32_t dm = distance - 1;
-                    uint32_t c0 = o[src], c1 = o[src + (1 & dm)], c2 = o[src + (2 & dm)], c3 = o[src + (3 & dm)];
-                    bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
-                    bitcnt = 64;
-                    asm volatile("" : "+v"(bitbuf), "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
-                    uint32_t k = 0;
-                    while (k < length && ((out_n + k) & 7)) {
-                        const uint32_t j = k & 3;
-                        o[out_n + k] = (uint16_t)(j == 0 ? c0 : j == 1 ? c1 : j == 2 ? c2 : c3);
-                        ++k;
-                    }
-                    if (k < length) {
-                        const uint32_t ph = k & 3;  // pattern phase at the first aligned cell
-                        const uint32_t q0 = ph == 0 ? c0 : ph == 1 ? c1 : ph == 2 ? c2 : c3;
-                        const uint32_t q1 = ph == 0 ? c1 : ph == 1 ? c2 : ph == 2 ? c3 : c0;
-                        const uint32_t q2 = ph == 0 ? c2 : ph == 1 ? c3 : ph == 2 ? c0 : c1;
-                        const uint32_t q3 = ph == 0 ? c3 : ph == 1 ? c0 : ph == 2 ? c1 : c2;
-                        const uint4 g = make_uint4(q0 | (q1 << 16), q2 | (q3 << 16), q0 | (q1 << 16), q2 | (q3 << 16));
-                        uint4* gp = reinterpret_cast<uint4*>(o + out_n + k);
-                        for (; k < length; k += 8) *gp++ = g;
-                    }
-                    out_n += length;
-                } else if (length) {
-                    // o[out_n + t] = o[src + (t mod distance)]: every load reads a
-                    // cell written before this match, so the loads of a batch
-                    // never wait for the match's own stores.  A wait for loads
-                    // also waits for every earlier store (one in-order counter),
-                    // so after the first group of 8 (issued with the refill) a
-                    // long match goes on in batches of 32 cells: one wait per
-                    // batch instead of one per group.
-                    const uint32_t src = out_n - distance;
-                    uint32_t jm = 0;
-                    {
-                        uint16_t v[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            v[i] = o[src + jm];
-                            jm = (jm + 1 == distance) ? 0u : jm + 1;
-                        }
-                        bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
-                        bitcnt = 64;
-                        // complete the refill here, before the stores (and so
-                        // not as a full wait at the top of the next symbol)
-                        asm volatile("" : "+v"(bitbuf));
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) o[out_n + i] = v[i];
-                    }
-                    if (length <= 24) {
-                        // short matches (the common case): plain groups of 8
-                        for (uint32_t k = 8; k < length; k += 8) {
-                            uint16_t v[8];
-#pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                v[i] = o[src + jm];
-                                jm = (jm + 1 == distance) ? 0u : jm + 1;
-                            }
-#pragma unroll
-                            for (int i = 0; i < 8; ++i) o[out_n + k + i] = v[i];
-                        }
-                    } else
-                    for (uint32_t k = 8; k < length; k += 32) {
-                        uint16_t v[32];
-#pragma unroll
-                        for (int i = 0; i < 32; ++i) {
-                            v[i] = o[src + jm];
-                            jm = (jm + 1 == distance) ? 0u : jm + 1;
-                        }
-#pragma unroll
-                        for (int g = 0; g < 4; ++g)
-                            if (k + 8u * (uint32_t)g < length) {
-#pragma unroll
-                                for (int i = 0; i < 8; ++i) o[out_n + k + 8 * g + i] = v[8 * g + i];
-                            }
-                    }
-                    out_n += length;
-                } else {
-                    bitbuf = funnel_lo(r0, r1, (uint32_t)(pos & 63));
-                    bitcnt = 64;
-                    asm volatile("" : "+v"(bitbuf));
-                }
+                // a back-reference before the segment's first byte becomes
+                // markers in the first member of a speculative segment
+                // (fast_inflate.rs:2209-2222); elsewhere it is an error
+                if (distance > out_n - member_base && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
+                RRG_FLUSH_LITS();
+                RRG_EMIT((TOK_MATCH << 30) | ((length - 3) << 16) | (distance - 1));
+                out_n += length;
+                if (out_n >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);
             }
         } else if (state == S_BLOCK_START) {
             // ---- block header (fast_inflate.rs:1914-1935, inflate.rs:74-145) ----
@@ -1382,6 +1365,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             bfinal = v;
             RRG_GETBITS(bt, 2);
             if (bt == 0) {
+                // stored block (fast_inflate.rs:1937-1963): one token for its bytes
                 pos = (pos + 7) & ~7ull;
                 RRG_REFILL();
                 uint32_t ln, nln;
@@ -1389,9 +1373,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 RRG_GETBITS(nln, 16);
                 if ((ln ^ nln) != 0xffffu) RRG_FAIL(ST_INVALID, DET_STORED_NLEN);
                 if (pos + 8ull * ln > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                if ((uint64_t)out_n + ln > out_cap) counting = 1;
-                stored_left = ln;
-                state = S_STORED;
+                if (ln) {
+                    const uint64_t off = (pos >> 3) - seg_byte;
+                    if (off >= (1ull << 30) || out_n + ln >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);
+                    RRG_FLUSH_LITS();
+                    RRG_EMIT((TOK_STORED << 30) | ln);
+                    RRG_EMIT((TOK_OFFSET << 30) | (uint32_t)off);
+                    out_n += ln;
+                    pos += 8ull * ln;
+                }
+                RRG_REFILL();
+                state = S_BLOCK_END;
             } else if (bt == 1) {
                 for (int s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
                 build_table(lens, 288, litlut, LB, cntL, tnext, toffs, sortedL);
@@ -1467,37 +1459,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             } else {
                 RRG_FAIL(ST_INVALID, DET_RESERVED_BTYPE);
             }
-        } else if (state == S_STORED) {
-            // ---- stored block, 32 bytes per step (fast_inflate.rs:1937-1963) ----
-            // (five words loaded together, one wait per 32 bytes; the block
-            // lies inside the input, so the words read exist)
-            for (int it = 0; it < 16 && stored_left; ++it) {
-                const uint32_t n = stored_left < 32 ? stored_left : 32u;
-                const uint64_t q = pos >> 6;
-                const uint32_t sh = (uint32_t)(pos & 63);
-                uint64_t w[5];
-#pragma unroll
-                for (int i = 0; i < 5; ++i) w[i] = in64[q + i];
-                if (!counting) {
-                    uint16_t* ob = o + out_n;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const uint64_t v = funnel_lo(w[i], w[i + 1], sh);
-#pragma unroll
-                        for (int b = 0; b < 8; ++b)
-                            if ((uint32_t)(8 * i + b) < n) ob[8 * i + b] = (uint16_t)((v >> (8 * b)) & 0xffu);
-                    }
-                } else {
-                    asm volatile("" : "+v"(w[4]));
-                }
-                out_n += n;
-                pos += 8ull * n;
-                stored_left -= n;
-            }
-            if (!stored_left) {
-                RRG_REFILL();
-                state = S_BLOCK_END;
-            }
         } else if (state == S_BLOCK_END) {
             if (!bfinal) {
                 if (mflags & SEG_BLOCK_ENDS) {
@@ -1549,8 +1510,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         continue;
     finish : {
-        // n_cells = the cells it needs (to the failure, when block ends are
-        // recorded: the cells before a failure are part of that output)
+        // the literals and tokens still waiting belong to the output before the
+        // segment's end (or its failure, whose output the single-worker paths send)
+        RRG_FLUSH_LITS();
+        if (!counting) {
+            const uint32_t b = n_tok - nq;  // first waiting token
+            if (nq > 0) tk[b] = nq == 1 ? q3 : nq == 2 ? q2 : q1;
+            if (nq > 1) tk[b + 1] = nq == 2 ? q3 : q2;
+            if (nq > 2) tk[b + 2] = q3;
+        }
+        // n_tok = the tokens it needs (to the failure, when block ends are
+        // recorded: the output before a failure is part of that decode)
         if (counting && (status == ST_OK || (mflags & SEG_BLOCK_ENDS))) status = ST_OVERFLOW;
         SegResult r;
         r.n_cells = out_n;
@@ -1559,13 +1529,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         r.detail = detail;
         r.final_block = final_block;
         r.n_mb = n_mb;
+        r.n_tok = n_tok;
+        r.pad = 0;
         res[seg] = r;
     }
     }
+#undef RRG_LOAD_ALL
 #undef RRG_REFILL
 #undef RRG_CONSUME
 #undef RRG_FAIL
 #undef RRG_GETBITS
+#undef RRG_EMIT
+#undef RRG_FLUSH_LITS
 }
 
 int inflate_blocks_per_cu(int lut_bits, int lanes) {
@@ -1573,32 +1548,169 @@ int inflate_blocks_per_cu(int lut_bits, int lanes) {
     int nb = 0;
     hipError_t e;
     switch (lut_bits) {
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<8>, lanes, shm); break;
-        case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<10>, lanes, shm); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inflate<9>, lanes, shm); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens<8>, lanes, shm); break;
+        case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens<10>, lanes, shm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens<9>, lanes, shm); break;
     }
     return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
-                          const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
-                          uint16_t* d_sorted, const InflateGeom& g, hipStream_t s) {
+hipError_t launch_tokens(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                         const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint32_t* d_toks,
+                         uint16_t* d_sorted, const InflateGeom& g, hipStream_t s) {
     if (n_work == 0) return hipSuccess;
     const size_t shm = (size_t)g.lanes_per_block * inflate_lds_per_lane(g.lut_bits);
     switch (g.lut_bits) {
         case 8:
-            hipLaunchKernelGGL(k_inflate<8>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
-                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_tokens<8>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_toks, d_sorted);
             break;
         case 10:
-            hipLaunchKernelGGL(k_inflate<10>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
-                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_tokens<10>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_toks, d_sorted);
             break;
         default:
-            hipLaunchKernelGGL(k_inflate<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
-                               d_mrec, d_work, n_work, d_counter, d_cells, d_sorted);
+            hipLaunchKernelGGL(k_tokens<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
+                               d_mrec, d_work, n_work, d_counter, d_toks, d_sorted);
             break;
     }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Expansion: one wavefront per segment turns its tokens into cells (or, for
+// members that cannot hold markers, bytes), 64 tokens at a time.  A wave
+// prefix sum over the tokens' lengths gives their first output positions;
+// then each 64-cell window of the batch is produced by the 64 lanes, one
+// cell each, and stored as one coalesced line.  A match cell copies the cell
+// d before it: by periodicity, the cell at (token start - d + (offset mod d)),
+// which lies before the token.  If that cell is in the window too, the lane
+// follows it to its own token (register shuffles; the chain ends, since every
+// step moves to an earlier token); else the cell was stored before (an earlier
+// window) and is loaded, or it lies before the segment's output and becomes
+// the marker 0x8000 | (offset before the segment) of fast_inflate.rs:2209-2222.
+// Literals come from the token, stored bytes from the input.
+// ===========================================================================
+constexpr int EXPAND_THREADS = 256;
+
+__device__ __forceinline__ uint32_t tok_len(uint32_t w) {
+    const uint32_t ty = w >> 30;
+    return ty == TOK_LIT ? (w >> 24) & 3u : ty == TOK_MATCH ? ((w >> 16) & 0xffu) + 3u : ty == TOK_STORED ? (w & 0xffffu) : 0u;
+}
+
+__global__ __launch_bounds__(EXPAND_THREADS) void k_expand(const uint32_t* __restrict__ toks,
+                                                         const ExpandItem* __restrict__ items,
+                                                         uint32_t n_items, uint32_t* __restrict__ counter,
+                                                         const uint8_t* __restrict__ in8, uint16_t* __restrict__ cells,
+                                                         uint8_t* __restrict__ out) {
+    const int lane = (int)(threadIdx.x & 63);
+    while (true) {
+        uint32_t wi = 0;
+        if (lane == 0) wi = atomicAdd(counter, 1u);
+        wi = uni32((uint32_t)__shfl((int)wi, 0, 64));
+        if (wi >= n_items) break;
+        const uint64_t tok_base = uni64(items[wi].tok_base);
+        const uint64_t dst = uni64(items[wi].dst);
+        const uint64_t in_byte = uni64(items[wi].in_byte);
+        const uint32_t n_cells = uni32((uint32_t)items[wi].n_cells);
+        const uint32_t n_tok = uni32(items[wi].n_tok);
+        const bool u8 = (uni32(items[wi].flags) & EXP_U8) != 0;
+        const uint32_t* tk = toks + tok_base;
+        uint16_t* c16 = cells + dst;
+        uint8_t* o8 = out + dst;
+        const uint8_t* src = in8 + in_byte;
+        uint32_t p0 = 0;  // first cell of the batch
+        for (uint32_t t0 = 0; t0 < n_tok; t0 += 64) {
+            const uint32_t i = t0 + (uint32_t)lane;
+            const uint32_t w = i < n_tok ? tk[i] : (TOK_OFFSET << 30);
+            const uint32_t L = tok_len(w);
+            uint32_t incl = L;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 64);
+                if (lane >= d) incl += o;
+            }
+            const uint32_t total = uni32((uint32_t)__shfl((int)incl, 63, 64));
+            const uint32_t S = p0 + incl - L;  // this token's first cell
+            // a stored token's offset word is the next token
+            uint32_t aux = (uint32_t)__shfl((int)w, lane < 63 ? lane + 1 : 63, 64);
+            if ((w >> 30) == TOK_STORED && lane == 63 && i + 1 < n_tok) aux = tk[i + 1];
+            const uint32_t pe = min(p0 + total, n_cells);
+            uint32_t ka = 0;  // the token owning the window's first cell
+            for (uint32_t c0 = p0 & ~63u; c0 < pe; c0 += 64) {
+                const uint32_t cs = max(c0, p0);  // cells before cs are stored already
+                const uint32_t p = c0 + (uint32_t)lane;
+                // owner of cell p: the last token starting at or before it
+                uint32_t own = ka;
+                for (uint32_t k = ka + 1; k < 64; ++k) {
+                    const uint32_t sk = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k));
+                    if (sk > c0 + 63) break;
+                    if (p >= sk) own = k;
+                }
+                ka = uni32((uint32_t)__builtin_amdgcn_readlane((int)own, 63));
+                bool done = !(p >= cs && p < pe);
+                uint32_t pp = p, kk = own, val = 0, mode = 0;  // mode 1: load cell q, 2: load input byte q
+                uint32_t q = 0;
+                while (__ballot(!done)) {
+                    const uint32_t wk = (uint32_t)__shfl((int)w, (int)kk, 64);
+                    const uint32_t sk = (uint32_t)__shfl((int)S, (int)kk, 64);
+                    const uint32_t ak = (uint32_t)__shfl((int)aux, (int)kk, 64);
+                    uint32_t next = pp;
+                    if (!done) {
+                        const uint32_t off = pp - sk;
+                        const uint32_t ty = wk >> 30;
+                        if (ty == TOK_LIT) {
+                            val = (wk >> (8 * off)) & 0xffu;
+                            done = true;
+                        } else if (ty == TOK_STORED) {
+                            q = (ak & 0x3fffffffu) + off;
+                            mode = 2;
+                            done = true;
+                        } else {
+                            const uint32_t d = (wk & 0x7fffu) + 1u;
+                            const uint32_t r = off < d ? off : (d == 1 ? 0u : off % d);
+                            if (sk + r < d) {  // before the segment's output: a marker
+                                val = 0x8000u | (d - (sk + r) - 1u);
+                                done = true;
+                            } else {
+                                q = sk + r - d;
+                                if (q < cs) {
+                                    mode = 1;
+                                    done = true;
+                                } else {
+                                    next = q;  // in this window: follow it
+                                }
+                            }
+                        }
+                    }
+                    // every lane takes part in the shuffle
+                    const uint32_t nk = (uint32_t)__shfl((int)own, (int)((next - c0) & 63u), 64);
+                    if (!done) {
+                        pp = next;
+                        kk = nk;
+                    }
+                }
+                if (mode == 1) val = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
+                else if (mode == 2) val = src[q];
+                if (p >= cs && p < pe) {
+                    if (u8) o8[p] = (uint8_t)val;
+                    else c16[p] = (uint16_t)val;
+                }
+            }
+            p0 += total;
+        }
+    }
+}
+
+int expand_blocks(int n_cu) { return n_cu * 8; }
+
+hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint32_t n_items, uint32_t* d_counter,
+                         const uint8_t* d_in8, uint16_t* d_cells, uint8_t* d_out, int n_cu, hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    const uint32_t waves = (uint32_t)expand_blocks(n_cu) * (EXPAND_THREADS / 64);
+    const uint32_t blocks = (std::min<uint32_t>(waves, n_items) + EXPAND_THREADS / 64 - 1) / (EXPAND_THREADS / 64);
+    hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(EXPAND_THREADS), 0, s, d_toks, d_items, n_items, d_counter, d_in8,
+                       d_cells, d_out);
     return hipGetLastError();
 }
 

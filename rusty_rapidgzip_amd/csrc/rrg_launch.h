@@ -35,7 +35,8 @@ hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const L
                             const uint64_t* d_leg_found, const uint64_t* d_leg_exit, const uint64_t* d_leg_rec,
                             uint64_t* d_out_found, uint64_t* d_out_next, uint32_t n, hipStream_t s);
 
-// Persistent inflate over the segment indices in d_work[0..n_work).
+// Pass 1 of the inflate: persistent lane-per-segment Huffman decode of the
+// segment indices in d_work[0..n_work) into tokens (rrg_device.h).
 struct InflateGeom {
     int lanes_per_block;  // threads per workgroup (one lane decodes one segment)
     int blocks;           // persistent workgroups
@@ -44,9 +45,15 @@ struct InflateGeom {
 int inflate_lds_per_lane(int lut_bits);
 int inflate_scratch_u16_per_lane();  // per-lane global scratch (d_sorted), in u16
 int inflate_blocks_per_cu(int lut_bits, int lanes);
-hipError_t launch_inflate(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
-                          const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint16_t* d_cells,
-                          uint16_t* d_sorted, const InflateGeom& g, hipStream_t s);
+hipError_t launch_tokens(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                         const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint32_t* d_toks,
+                         uint16_t* d_sorted, const InflateGeom& g, hipStream_t s);
+
+// Pass 2: one wavefront per item expands its tokens into u16 cells at
+// d_cells[dst] (or bytes at d_out[dst] with EXP_U8); persistent waves take
+// the items in order from *d_counter (cleared by the caller).
+hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint32_t n_items, uint32_t* d_counter,
+                         const uint8_t* d_in8, uint16_t* d_cells, uint8_t* d_out, int n_cu, hipStream_t s);
 
 // Tail chain: blocked scan over blocks of G chain segments.  d_scratch holds
 // tail_scratch_bytes(n_seg, G); d_err[s] is set to 1 for a chain segment s

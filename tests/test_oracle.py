@@ -378,4 +378,4 @@ def test_native_library_exports():
     lib = ctypes.CDLL(_native.GPU_LIB)
     for sym in declared:
         assert hasattr(lib, sym), sym
-    assert lib.rrg_gpu_abi_version() == 2
+    assert lib.rrg_gpu_abi_version() == 3
