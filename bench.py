@@ -511,7 +511,7 @@ def main() -> int:
     ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
     ap.add_argument("--chunk-mib", type=int, default=4)
     ap.add_argument("--segment-kib", type=int, default=0, help="0 = automatic")
-    ap.add_argument("--lut-bits", type=int, default=8)
+    ap.add_argument("--lut-bits", type=int, default=9)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")

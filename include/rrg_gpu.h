@@ -85,7 +85,7 @@ typedef struct {
     int verbose;              /* Config.verbose: 0 = Off, 1 = On (stage timings on stderr) */
     /* GPU decomposition (no reference counterpart; 0 = automatic) */
     size_t segment_bytes;     /* compressed bytes per lane-decoded segment */
-    int lut_bits;             /* literal/length LUT bits: 8 (default), 9 or 10 */
+    int lut_bits;             /* literal/length LUT bits: 8, 9 (default) or 10 */
     int lanes_per_block;      /* lanes (segments in flight) per workgroup (default 16) */
 } rrg_cfg;
 

@@ -220,7 +220,7 @@ class Config:
     chunk_size_bytes: int = 4 * 1024 * 1024
     verbose: Verbosity = Verbosity.Off
     segment_bytes: int = 0
-    lut_bits: int = 8
+    lut_bits: int = 9
     lanes_per_block: int = 16
 
     def _c(self) -> _Cfg:

@@ -214,7 +214,7 @@ double now_ms() {
 
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
-    G.g.lut_bits = (cfg.lut_bits == 9 || cfg.lut_bits == 10) ? cfg.lut_bits : 8;
+    G.g.lut_bits = (cfg.lut_bits == 8 || cfg.lut_bits == 10) ? cfg.lut_bits : 9;
     int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 16;
     if (lanes > 64) lanes = 64;
     while (lanes > 1 && (size_t)lanes * inflate_lds_per_lane(G.g.lut_bits) > 65536) lanes /= 2;
@@ -323,6 +323,14 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
         for (size_t k = 0; k < big.size() && k < 4; ++k)
             std::fprintf(stderr, " %llu cells from %llu B", (unsigned long long)big[k].first,
                          (unsigned long long)((res[big[k].second].end_bit - desc[big[k].second].start_bit) / 8));
+        // the input spans (the lane's work), largest and by quantile
+        std::vector<uint64_t> span;
+        for (uint32_t j : work) span.push_back((res[j].end_bit - desc[j].start_bit) / 8);
+        std::sort(span.begin(), span.end());
+        if (!span.empty())
+            std::fprintf(stderr, "; input spans: median %llu B, p99 %llu B, max %llu B",
+                         (unsigned long long)span[span.size() / 2], (unsigned long long)span[span.size() * 99 / 100],
+                         (unsigned long long)span.back());
         std::fprintf(stderr, "\n");
     }
     return RRG_OK;
@@ -1191,7 +1199,71 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     int rc = run_finder(c, targets, limits, scan_ends, found, err, &stops, &exits);
     if (!rc) rc = finish_open(c, targets, limits, exits, found, err);
     if (rc) return rc;
-    // 3. segment descriptors
+    // 3. segment starts: chunk starts (the reference's, strictly increasing)
+    // and every distinct segment find.  A segment cannot be shorter than a
+    // deflate block, and one that spans two blocks (its target and the next
+    // found the same block start) doubles the longest lane's work: gaps well
+    // above the median get one more target, a quarter of the way in.
+    {
+        std::vector<uint64_t> gt, gl, ge;
+        std::vector<std::pair<size_t, size_t>> gat;  // (job, gap index)
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            Job& J = jobs[ji];
+            if (J.body == UINT64_MAX) continue;
+            const uint64_t base = J.abs_bits(J.body);
+            std::vector<uint64_t> B{base};
+            if (J.stream || J.serial) B = J.stream_B;
+            for (size_t i = J.t_lo; i < J.t_hi; ++i)
+                if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
+            // segment finds come in target order, which is almost always their
+            // order too: merge them with the chunk starts (sort only when not)
+            std::vector<uint64_t> F;
+            F.reserve(J.t_hi - J.t_lo);
+            for (size_t i = J.t_lo; i < J.t_hi; ++i)
+                if (!is_chunk[i] && found[i] != UINT64_MAX) F.push_back(found[i]);
+            if (!std::is_sorted(F.begin(), F.end())) std::sort(F.begin(), F.end());
+            std::vector<uint64_t> M(B.size() + F.size());
+            std::merge(B.begin(), B.end(), F.begin(), F.end(), M.begin());
+            M.erase(std::unique(M.begin(), M.end()), M.end());
+            J.stream_B = B;
+            J.orig_start = M;
+            if (M.size() < 16) continue;
+            std::vector<uint64_t> span(M.size() - 1);
+            for (size_t j = 0; j + 1 < M.size(); ++j) span[j] = M[j + 1] - M[j];
+            std::nth_element(span.begin(), span.begin() + (long)(span.size() / 2), span.end());
+            const uint64_t med = span[span.size() / 2];
+            for (size_t j = 0; j < M.size(); ++j) {
+                // the last segment runs to the end of the file
+                const uint64_t nx = j + 1 < M.size() ? M[j + 1] : J.end_bits();
+                const uint64_t g = nx - M[j];
+                if (g > med + med / 2 && g > 8 * 16384) {
+                    // a block start hidden in the gap lies about halfway: search from a quarter on
+                    gt.push_back(M[j] + g / 4);
+                    gl.push_back(J.end_bits() | kAnyStart);
+                    ge.push_back(nx);
+                    gat.push_back({ji, j});
+                }
+            }
+        }
+        if (!gt.empty()) {
+            std::vector<uint64_t> gf;
+            rc = run_finder(c, gt, gl, ge, gf, err);
+            if (rc) return rc;
+            std::vector<std::vector<uint64_t>> extra(jobs.size());
+            for (size_t i = 0; i < gt.size(); ++i)
+                if (gf[i] != UINT64_MAX) extra[gat[i].first].push_back(gf[i]);
+            for (size_t ji = 0; ji < jobs.size(); ++ji) {
+                if (extra[ji].empty()) continue;
+                std::vector<uint64_t>& M = jobs[ji].orig_start;
+                std::vector<uint64_t> N(M.size() + extra[ji].size());
+                std::sort(extra[ji].begin(), extra[ji].end());
+                std::merge(M.begin(), M.end(), extra[ji].begin(), extra[ji].end(), N.begin());
+                N.erase(std::unique(N.begin(), N.end()), N.end());
+                M.swap(N);
+            }
+        }
+    }
+    // 4. segment descriptors
     std::vector<SegDesc> desc;
     uint64_t tok_used = 0, mb_used = 0;
     for (Job& J : jobs) {
@@ -1232,21 +1304,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         // increasing finds), segments at every distinct find
         J.mp_lo = J.mp_hi = desc.size();
         if (J.body == UINT64_MAX) continue;
-        const uint64_t base = J.abs_bits(J.body);
-        std::vector<uint64_t> B{base};
-        if (J.stream || J.serial) B = J.stream_B;
-        for (size_t i = J.t_lo; i < J.t_hi; ++i)
-            if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
-        // segment finds come in target order, which is almost always their
-        // order too: merge them with the chunk starts (sort only when not)
-        std::vector<uint64_t> F;
-        F.reserve(J.t_hi - J.t_lo);
-        for (size_t i = J.t_lo; i < J.t_hi; ++i)
-            if (!is_chunk[i] && found[i] != UINT64_MAX) F.push_back(found[i]);
-        if (!std::is_sorted(F.begin(), F.end())) std::sort(F.begin(), F.end());
-        std::vector<uint64_t> M(B.size() + F.size());
-        std::merge(B.begin(), B.end(), F.begin(), F.end(), M.begin());
-        M.erase(std::unique(M.begin(), M.end()), M.end());
+        std::vector<uint64_t>& M = J.orig_start;  // the segment starts (chunk starts marked below)
+        std::vector<uint64_t>& B = J.stream_B;    // the chunk starts
         const size_t S = M.size();
         J.seg_is_chunk.assign(S, 0);
         for (size_t j = 0, bi = 0; j < S && bi < B.size(); ++j)
@@ -1270,7 +1329,6 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             desc.push_back(d);
         }
         J.mp_hi = desc.size();
-        J.orig_start = M;
         J.order.resize(S);
         for (size_t j = 0; j < S; ++j) J.order[j] = (uint32_t)(J.mp_lo + j);
     }
@@ -1598,7 +1656,7 @@ void rrg_gpu_debug_fail_alloc(int n) { g_fail_alloc.store(n > 0 ? n : 0); }
 void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);
     cfg->chunk_size_bytes = 4u << 20;
-    cfg->lut_bits = 8;
+    cfg->lut_bits = 9;
     cfg->lanes_per_block = 16;
 }
 

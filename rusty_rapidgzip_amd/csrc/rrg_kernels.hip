@@ -1000,12 +1000,18 @@ constexpr int DIST_LB = 7;
 
 // Per-lane LDS: literal/length LUT (2^LB u16 entries (sym << 4) | len),
 // distance LUT (128 u8 entries (sym << 3) | len, len <= 7; the precode LUT
-// while a header is read), and four 16-entry u16 arrays (code-length counts of
-// both tables, canonical next code and sorted offset while a table is built).
+// while a header is read), four 16-entry u16 arrays (code-length counts of
+// both tables, canonical next code and sorted offset while a table is built),
+// the symbols of the codes the LUTs do not hold, in canonical order (the
+// first kLongLit literal/length ones, u16, and every distance one, u8), and
+// per table where the canonical walk for those codes starts (first code of
+// length LUT bits + 1 | symbols with shorter codes << 16).
+constexpr int kLongLit = 48;
+constexpr int kLongDist = 32;
 template <int LB>
 struct LaneTables {
     static constexpr int kLit = 1 << LB;
-    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) + 4 * 32;
+    static constexpr int kBytes = kLit * 2 + (1 << DIST_LB) + 4 * 32 + 2 * kLongLit + kLongDist + 16;
 };
 // per-lane global scratch, in u16: 320 length-sorted symbols, 320 B of code
 // lengths, 32 B of precode lengths
@@ -1023,19 +1029,21 @@ int inflate_lds_per_lane(int lut_bits) {
 
 // Canonical table build (huffman.rs:215-325 semantics, incomplete accepted).
 // Returns false when oversubscribed with >= 2 symbols.
-template <typename E>
+template <typename E, typename L>
 __device__ bool build_table(const uint8_t* __restrict__ lens, int n, E* __restrict__ lut, int lbits,
                             uint16_t* __restrict__ cnt, uint16_t* __restrict__ tnext, uint16_t* __restrict__ toffs,
-                            uint16_t* __restrict__ sorted) {
+                            uint16_t* __restrict__ sorted, L* __restrict__ longs, int long_cap,
+                            uint32_t* __restrict__ head) {
     // entry: (symbol << 4) | length in u16 tables, (symbol << 3) | length in u8 ones
     constexpr int kSh = sizeof(E) == 1 ? 3 : 4;
     for (int l = 0; l < 16; ++l) cnt[l] = 0;
     for (int s = 0; s < n; ++s) cnt[lens[s]]++;
     cnt[0] = 0;
-    uint32_t total = 0, nsym = 0;
+    uint32_t total = 0, nsym = 0, nshort = 0;
     for (int l = 1; l <= 15; ++l) {
         total += (uint32_t)cnt[l] << (15 - l);
         nsym += cnt[l];
+        if (l <= lbits) nshort += cnt[l];
     }
     if (nsym >= 2 && total > 32768u) return false;
     uint32_t code = 0, off = 0;
@@ -1045,6 +1053,7 @@ __device__ bool build_table(const uint8_t* __restrict__ lens, int n, E* __restri
         toffs[l] = (uint16_t)off;
         off += cnt[l];
     }
+    if (head) *head = (lbits < 15 ? (uint32_t)tnext[lbits + 1] : 0u) | (nshort << 16);
     uint32_t* lut32 = reinterpret_cast<uint32_t*>(lut);
     for (int i = 0; i < (int)(((1u << lbits) * sizeof(E)) / 4); ++i) lut32[i] = 0;
     for (int s = 0; s < n; ++s) {
@@ -1052,29 +1061,51 @@ __device__ bool build_table(const uint8_t* __restrict__ lens, int n, E* __restri
         if (!l) continue;
         const uint32_t c = tnext[l];
         tnext[l] = (uint16_t)(c + 1);
-        if (sorted) sorted[toffs[l]] = (uint16_t)s;
-        toffs[l] = (uint16_t)(toffs[l] + 1);
+        const uint32_t at = toffs[l];
+        if (sorted) sorted[at] = (uint16_t)s;
+        toffs[l] = (uint16_t)(at + 1);
         if (l <= lbits) {
             const uint32_t rev = __brev(c) >> (32 - l);
             const E e = (E)((s << kSh) | l);
             for (uint32_t idx = rev; idx < (1u << lbits); idx += 1u << l) lut[idx] = e;
+        } else if (longs && at - nshort < (uint32_t)long_cap) {
+            longs[at - nshort] = (L)s;
         }
     }
     return true;
 }
 
-// Slow path for a LUT miss: canonical walk from length 1 (RFC 1951 3.2.2).
-__device__ bool decode_long(uint64_t bits, const uint16_t* __restrict__ cnt, const uint16_t* __restrict__ sorted,
-                            uint32_t* sym, uint32_t* len) {
-    int code = 0, first = 0, index = 0;
-    for (int l = 1; l <= 15; ++l) {
-        code |= (int)((bits >> (l - 1)) & 1u);
+// Slow path for a LUT miss: the canonical walk (RFC 1951 3.2.2) over the
+// code lengths above the LUT's LBITS, from the first code of length LBITS + 1
+// and the count of shorter codes (`head`, from build_table).  The symbol
+// comes from the LDS list of long-code symbols, or, past its kcap entries,
+// from the length-sorted table in the lane's global scratch.  With `seq`
+// nonzero the long codes stand for consecutive symbols from seq (the fixed
+// literal code's 9-bit codes: 144-255).
+template <int LBITS, typename L>
+__device__ __forceinline__ bool decode_long(uint64_t bits, const uint16_t* __restrict__ cnt,
+                                            const uint16_t* __restrict__ sorted, const L* __restrict__ longs,
+                                            uint32_t kcap, uint32_t head, uint32_t seq, uint32_t* sym,
+                                            uint32_t* len) {
+    const uint32_t nshort = head >> 16;
+    int first = (int)(head & 0xffffu), index = (int)nshort;
+    int code = (int)(__brev((uint32_t)bits) >> (31 - LBITS));  // the first LBITS + 1 bits, first bit highest
+#pragma unroll
+    for (int l = LBITS + 1; l <= 15; ++l) {
         const int count = cnt[l];
         if (code - count < first) {
-            uint32_t v = sorted[index + (code - first)];
-            // use the loaded value here: the wait for this load then stays on the
-            // slow path instead of becoming a full vmcnt wait on every symbol
-            asm volatile("" : "+v"(v));
+            const uint32_t at = (uint32_t)(index + (code - first));
+            uint32_t v;
+            if (seq) {
+                v = seq + (at - nshort);
+            } else if (at - nshort < kcap) {
+                v = longs[at - nshort];
+            } else {
+                v = sorted[at];
+                // use the loaded value here: the wait for this load then stays on
+                // the slow path instead of becoming a full vmcnt wait on every symbol
+                asm volatile("" : "+v"(v));
+            }
             *sym = v;
             *len = (uint32_t)l;
             return true;
@@ -1082,7 +1113,7 @@ __device__ bool decode_long(uint64_t bits, const uint16_t* __restrict__ cnt, con
         index += count;
         first += count;
         first <<= 1;
-        code <<= 1;
+        code = (code << 1) | (int)((bits >> l) & 1u);
     }
     return false;
 }
@@ -1138,6 +1169,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     uint16_t* cntD = cntL + 16;
     uint16_t* tnext = cntD + 16;
     uint16_t* toffs = tnext + 16;
+    uint16_t* longL = toffs + 16;
+    uint8_t* longD = reinterpret_cast<uint8_t*>(longL + kLongLit);
+    uint32_t* lhead = reinterpret_cast<uint32_t*>(longD + kLongDist);  // [0] literal/length, [1] distance
     // code lengths and length-sorted symbols live in per-lane global scratch: they
     // are touched once per block header, not per symbol
     uint16_t* sortedL = sorted_scratch + ((size_t)blockIdx.x * blockDim.x + tid) * LANE_SCRATCH_U16;
@@ -1150,65 +1184,49 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     uint64_t pos = 0, hint = 0, bitbuf = 0, seg_byte = 0;
     uint64_t nbits = 0;  // end of the segment's file in the input buffer (bits)
     uint32_t bitcnt = 0;
-    // input words: A0..A3 = in64[wb .. wb+3] (settled), B0..B3 = in64[wb+4 ..
-    // wb+7] (B1..B3 still in flight after a rotation); a refill reads two
-    // consecutive A words, so the words loaded at a rotation are first used
-    // ~192 bits of input later
-    uint64_t wb = 0, A0 = 0, A1 = 0, A2 = 0, A3 = 0, B0 = 0, B1 = 0, B2 = 0, B3 = 0;
+    // the input window of the symbol loop: C0, C1, C2 = in64[ci], in64[ci + 1],
+    // in64[ci + 2].  A refill reads C0 and C1; C2 is the word loaded ahead,
+    // at the previous word crossing, so a refill waits for a load issued
+    // about 64 input bits earlier, never for one it issued itself.
+    uint64_t ci = 0, C0 = 0, C1 = 0, C2 = 0;
     uint32_t* tk = toks;
     uint32_t tok_cap = 0, n_tok = 0;
     uint32_t out_n = 0, member_base = 0;
     // up to three literals wait in lit_acc; up to three tokens wait in q1..q3
     // (q0 is the oldest) and are stored four at a time as one 16-byte write
     uint32_t lit_acc = 0, lit_n = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0, nq = 0;
-    uint32_t spec = 0, mflags = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0;
+    uint32_t spec = 0, mflags = 0, bfinal = 0, n_mb = 0, mb_base = 0, mb_cap = 0, fixed_seq = 0;
     // counting: the token region is full; decode on without storing so the
     // segment reports the size it needs (one regrow instead of repeated ones)
     uint32_t counting = 0;
     uint32_t status = ST_OK, detail = DET_NONE, final_block = 0;
 
-#define RRG_LOAD_ALL(q)                \
-    do {                               \
-        wb = (q);                      \
-        A0 = in64[wb];                 \
-        A1 = in64[wb + 1];             \
-        A2 = in64[wb + 2];             \
-        A3 = in64[wb + 3];             \
-        B0 = in64[wb + 4];             \
-        B1 = in64[wb + 5];             \
-        B2 = in64[wb + 6];             \
-        B3 = in64[wb + 7];             \
+// 64 bits at pos (the input buffer is padded, so words up to 8 past the end
+// exist).  The asm use completes the loads inside the refill, so that code
+// merging with a refill-free path does not inherit a pending-load wait.
+#define RRG_REFILL()                      \
+    do {                                  \
+        bitbuf = bits64_at(in64, pos);    \
+        asm volatile("" : "+v"(bitbuf));  \
+        bitcnt = 64;                      \
     } while (0)
-// 64 bits at pos from the word cache (the input buffer is padded, so words
-// up to 8 past the end exist).  Past the cached words: rotate by three (the
-// B words were loaded a rotation ago), or reload at a jump.
-#define RRG_REFILL()                                                                   \
-    do {                                                                               \
-        const uint64_t _q = pos >> 6;                                                  \
-        uint64_t _d = _q - wb;                                                         \
-        if (_d >= 3) {                                                                 \
-            if (_d < 6) {                                                              \
-                asm volatile("" : "+v"(B0), "+v"(B1), "+v"(B2));                       \
-                A0 = A3;                                                               \
-                A1 = B0;                                                               \
-                A2 = B1;                                                               \
-                A3 = B2;                                                               \
-                B0 = B3;                                                               \
-                wb += 3;                                                               \
-                B1 = in64[wb + 5];                                                     \
-                B2 = in64[wb + 6];                                                     \
-                B3 = in64[wb + 7];                                                     \
-                _d -= 3;                                                               \
-            } else {                                                                   \
-                RRG_LOAD_ALL(_q);                                                      \
-                asm volatile("" : "+v"(A0), "+v"(A1), "+v"(A2), "+v"(A3));             \
-                _d = 0;                                                                \
-            }                                                                          \
-        }                                                                              \
-        const uint64_t _lo = _d == 0 ? A0 : (_d == 1 ? A1 : A2);                       \
-        const uint64_t _hi = _d == 0 ? A1 : (_d == 1 ? A2 : A3);                       \
-        bitbuf = funnel_lo(_lo, _hi, (uint32_t)(pos & 63));                            \
-        bitcnt = 64;                                                                   \
+// the same from the symbol loop's window, moved on to pos first: one word per
+// crossing (the common case), or three fresh words after a jump
+#define RRG_REFILL_W()                                                              \
+    do {                                                                            \
+        const uint64_t _q = pos >> 6;                                               \
+        if (_q != ci) {                                                             \
+            uint32_t _st = 3;                                                       \
+            if (_q > ci && _q - ci < 3) _st = (uint32_t)(_q - ci);                   \
+            ci = _q;                                                                \
+            for (uint32_t _k = _st; _k > 0; --_k) {                                 \
+                C0 = C1;                                                            \
+                C1 = C2;                                                            \
+                C2 = in64[ci + 3 - _k];                                             \
+            }                                                                       \
+        }                                                                           \
+        bitbuf = funnel_lo(C0, C1, (uint32_t)(pos & 63));                           \
+        bitcnt = 64;                                                                \
     } while (0)
 #define RRG_CONSUME(n)           \
     do {                         \
@@ -1258,6 +1276,64 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }                                                            \
     } while (0)
 
+// One Huffman symbol into the token stream; CHK = 1 applies the reference's
+// end-of-input checks.  `break` leaves the symbol loop at end-of-block,
+// `continue` goes on with the next symbol.
+#define RRG_SYMBOL(CHK)                                                                          \
+    {                                                                                            \
+        uint32_t e = litlut[bitbuf & LMASK];                                                     \
+        uint32_t l = e & 15u, sym = e >> 4;                                                      \
+        if (l == 0) {                                                                            \
+            if ((CHK) && pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);                           \
+            if (!decode_long<LB>(bitbuf, cntL, sortedL, longL, kLongLit, lhead[0], fixed_seq, &sym, &l)) \
+                RRG_FAIL(ST_INVALID, DET_NO_CODE);                                               \
+        }                                                                                        \
+        if ((CHK) && pos + l > nbits) RRG_FAIL(ST_EOF, DET_NONE);                                \
+        RRG_CONSUME(l);                                                                          \
+        if (sym < 256) {                                                                         \
+            lit_acc |= sym << (8 * lit_n);                                                       \
+            if (++lit_n == 3) {                                                                  \
+                RRG_EMIT((TOK_LIT << 30) | (3u << 24) | lit_acc);                                \
+                lit_n = 0;                                                                       \
+                lit_acc = 0;                                                                     \
+            }                                                                                    \
+            ++out_n;                                                                             \
+            continue;                                                                            \
+        }                                                                                        \
+        if (sym == 256) {                                                                        \
+            state = S_BLOCK_END;                                                                 \
+            break;                                                                               \
+        }                                                                                        \
+        if (sym > 285) RRG_FAIL(ST_INVALID, DET_LITLEN_RANGE);                                   \
+        const uint32_t li = sym - 257;                                                           \
+        const uint32_t lx = len_extra(li);                                                       \
+        if ((CHK) && pos + lx > nbits) RRG_FAIL(ST_EOF, DET_NONE);                               \
+        const uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));          \
+        RRG_CONSUME(lx);                                                                         \
+        uint32_t de = distlut[bitbuf & ((1u << DIST_LB) - 1)];                                   \
+        uint32_t dl = de & 7u, dsym = de >> 3;                                                   \
+        if (dl == 0) {                                                                           \
+            if ((CHK) && pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);                           \
+            if (!decode_long<DIST_LB>(bitbuf, cntD, sortedD, longD, kLongDist, lhead[1], 0u, &dsym, &dl)) \
+                RRG_FAIL(ST_INVALID, DET_NO_CODE);                                               \
+        }                                                                                        \
+        if ((CHK) && pos + dl > nbits) RRG_FAIL(ST_EOF, DET_NONE);                               \
+        RRG_CONSUME(dl);                                                                         \
+        if (dsym >= 30) RRG_FAIL(ST_INVALID, DET_DIST_RANGE);                                    \
+        const uint32_t dx = dist_extra(dsym);                                                    \
+        if ((CHK) && pos + dx > nbits) RRG_FAIL(ST_EOF, DET_NONE);                               \
+        const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));     \
+        RRG_CONSUME(dx);                                                                         \
+        /* a back-reference before the segment's first byte becomes markers in */                \
+        /* the first member of a speculative segment (fast_inflate.rs:2209-2222); */             \
+        /* elsewhere it is an error */                                                           \
+        if (distance > out_n - member_base && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);         \
+        RRG_FLUSH_LITS();                                                                        \
+        RRG_EMIT((TOK_MATCH << 30) | ((length - 3) << 16) | (distance - 1));                     \
+        out_n += length;                                                                         \
+        if (out_n >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);                               \
+    }
+
     while (true) {
         if (state == S_FETCH) {
             const uint32_t w = atomicAdd(counter, 1u);
@@ -1273,6 +1349,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 nbits = d.end_byte * 8;
                 tk = toks + d.tok_base;
                 tok_cap = (uint32_t)d.tok_cap;
+                ci = ~0ull;  // the window is reloaded at the first symbol refill
                 n_tok = 0;
                 nq = 0;
                 lit_n = 0;
@@ -1292,7 +1369,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     status = ST_EOF;
                     goto finish;
                 }
-                RRG_LOAD_ALL(pos >> 6);
                 RRG_REFILL();
                 state = S_BLOCK_START;
             }
@@ -1301,58 +1377,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 
         if (state == S_SYM) {
             // ---- Huffman symbols (fast_inflate.rs:2065-2229), as tokens ----
+            // A symbol reads at most 48 bits (litlen <= 15 + 5 extra, dist <=
+            // 15 + 13 extra), so with 48 bits buffered and 64 left before the
+            // end of the input none of its end-of-input checks can fail: that
+            // case skips them.
             for (int it = 0; it < 64; ++it) {
-                // 48 bits cover a whole match: litlen <= 15 + 5 extra, dist <= 15 + 13 extra
-                if (bitcnt < 48) RRG_REFILL();
-                uint32_t e = litlut[bitbuf & LMASK];
-                uint32_t l = e & 15u, sym = e >> 4;
-                if (l == 0) {
-                    if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                    if (!decode_long(bitbuf, cntL, sortedL, &sym, &l)) RRG_FAIL(ST_INVALID, DET_NO_CODE);
+                if (bitcnt < 48) RRG_REFILL_W();
+                if (pos + 64 <= nbits) {
+                    RRG_SYMBOL(0)
+                } else {
+                    RRG_SYMBOL(1)
                 }
-                if (pos + l > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                RRG_CONSUME(l);
-                if (sym < 256) {
-                    lit_acc |= sym << (8 * lit_n);
-                    if (++lit_n == 3) {
-                        RRG_EMIT((TOK_LIT << 30) | (3u << 24) | lit_acc);
-                        lit_n = 0;
-                        lit_acc = 0;
-                    }
-                    ++out_n;
-                    continue;
-                }
-                if (sym == 256) {
-                    state = S_BLOCK_END;
-                    break;
-                }
-                if (sym > 285) RRG_FAIL(ST_INVALID, DET_LITLEN_RANGE);
-                const uint32_t li = sym - 257;
-                const uint32_t lx = len_extra(li);
-                if (pos + lx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                const uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));
-                RRG_CONSUME(lx);
-                uint32_t de = distlut[bitbuf & ((1u << DIST_LB) - 1)];
-                uint32_t dl = de & 7u, dsym = de >> 3;
-                if (dl == 0) {
-                    if (pos + 15 > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                    if (!decode_long(bitbuf, cntD, sortedD, &dsym, &dl)) RRG_FAIL(ST_INVALID, DET_NO_CODE);
-                }
-                if (pos + dl > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                RRG_CONSUME(dl);
-                if (dsym >= 30) RRG_FAIL(ST_INVALID, DET_DIST_RANGE);
-                const uint32_t dx = dist_extra(dsym);
-                if (pos + dx > nbits) RRG_FAIL(ST_EOF, DET_NONE);
-                const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));
-                RRG_CONSUME(dx);
-                // a back-reference before the segment's first byte becomes
-                // markers in the first member of a speculative segment
-                // (fast_inflate.rs:2209-2222); elsewhere it is an error
-                if (distance > out_n - member_base && !spec) RRG_FAIL(ST_INVALID, DET_DIST_FAR);
-                RRG_FLUSH_LITS();
-                RRG_EMIT((TOK_MATCH << 30) | ((length - 3) << 16) | (distance - 1));
-                out_n += length;
-                if (out_n >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);
             }
         } else if (state == S_BLOCK_START) {
             // ---- block header (fast_inflate.rs:1914-1935, inflate.rs:74-145) ----
@@ -1375,20 +1410,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 if (pos + 8ull * ln > nbits) RRG_FAIL(ST_EOF, DET_NONE);
                 if (ln) {
                     const uint64_t off = (pos >> 3) - seg_byte;
-                    if (off >= (1ull << 30) || out_n + ln >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);
+                    if (off + ln >= (1ull << 30) || out_n + ln >= kMaxSegCells) RRG_FAIL(ST_TOO_BIG, DET_NONE);
                     RRG_FLUSH_LITS();
-                    RRG_EMIT((TOK_STORED << 30) | ln);
-                    RRG_EMIT((TOK_OFFSET << 30) | (uint32_t)off);
+                    // pieces of at most 256 bytes: no token stands for more than 258 cells
+                    for (uint32_t k = 0; k < ln; k += 256) {
+                        RRG_EMIT((TOK_STORED << 30) | min(ln - k, 256u));
+                        RRG_EMIT((TOK_OFFSET << 30) | (uint32_t)(off + k));
+                    }
                     out_n += ln;
                     pos += 8ull * ln;
                 }
                 RRG_REFILL();
                 state = S_BLOCK_END;
             } else if (bt == 1) {
+                // codes above LB bits are all 9-bit ones (LB is 8 here; 9 and 10 hold every code)
+                fixed_seq = LB < 9 ? 144u : 0u;
                 for (int s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
-                build_table(lens, 288, litlut, LB, cntL, tnext, toffs, sortedL);
+                build_table(lens, 288, litlut, LB, cntL, tnext, toffs, sortedL, longL, kLongLit, lhead);
                 for (int s = 0; s < 30; ++s) lens[s] = 5;
-                build_table(lens, 30, distlut, DIST_LB, cntD, tnext, toffs, sortedD);
+                build_table(lens, 30, distlut, DIST_LB, cntD, tnext, toffs, sortedD, longD, kLongDist, lhead + 1);
                 state = S_SYM;
             } else if (bt == 2) {
                 uint32_t a, b2, c2;
@@ -1405,7 +1445,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     cl[c_cl_order[i]] = (uint8_t)x;
                 }
                 // precode LUT (7 bits) lives in the distance LUT while lengths are read
-                if (!build_table(cl, 19, distlut, DIST_LB, cntD, tnext, toffs, nullptr))
+                if (!build_table(cl, 19, distlut, DIST_LB, cntD, tnext, toffs, nullptr, (uint8_t*)nullptr, 0,
+                                 (uint32_t*)nullptr))
                     RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
                 const int total = hlit + hdist;
                 int i = 0;
@@ -1451,9 +1492,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     }
                 }
                 if (lens[256] == 0) RRG_FAIL(ST_INVALID, DET_EOB_ZERO);
-                if (!build_table(lens, hlit, litlut, LB, cntL, tnext, toffs, sortedL))
+                fixed_seq = 0;
+                if (!build_table(lens, hlit, litlut, LB, cntL, tnext, toffs, sortedL, longL, kLongLit, lhead))
                     RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
-                if (!build_table(lens + hlit, hdist, distlut, DIST_LB, cntD, tnext, toffs, sortedD))
+                if (!build_table(lens + hlit, hdist, distlut, DIST_LB, cntD, tnext, toffs, sortedD, longD, kLongDist,
+                                 lhead + 1))
                     RRG_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
                 state = S_SYM;
             } else {
@@ -1534,7 +1577,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         res[seg] = r;
     }
     }
-#undef RRG_LOAD_ALL
+#undef RRG_SYMBOL
+#undef RRG_REFILL_W
 #undef RRG_REFILL
 #undef RRG_CONSUME
 #undef RRG_FAIL
@@ -1579,31 +1623,35 @@ hipError_t launch_tokens(const uint64_t* d_in64, const SegDesc* d_segs, SegResul
 
 // ===========================================================================
 // Expansion: one wavefront per segment turns its tokens into cells (or, for
-// members that cannot hold markers, bytes), 64 tokens at a time.  A wave
-// prefix sum over the tokens' lengths gives their first output positions;
-// then each 64-cell window of the batch is produced by the 64 lanes, one
-// cell each, and stored as one coalesced line.  A match cell copies the cell
-// d before it: by periodicity, the cell at (token start - d + (offset mod d)),
-// which lies before the token.  If that cell is in the window too, the lane
-// follows it to its own token (register shuffles; the chain ends, since every
-// step moves to an earlier token); else the cell was stored before (an earlier
-// window) and is loaded, or it lies before the segment's output and becomes
-// the marker 0x8000 | (offset before the segment) of fast_inflate.rs:2209-2222.
-// Literals come from the token, stored bytes from the input.
+// members that cannot hold markers, bytes).  Lane k of a batch owns token k:
+// a wave prefix sum over the tokens' lengths places them (at most 64 tokens
+// and kBatchCells cells per batch), and each lane writes its token's cells
+// into the wave's LDS ring of recent output (kRing cells).  Literals and
+// stored bytes need nothing earlier; a match copies the cells d before it,
+// by periodicity always from cells before its own start, and runs in the
+// first round in which every one of those cells is complete (they lie before
+// the batch's first unfinished token, whose own source always does: each
+// round finishes at least that token).  Sources still in the ring are read
+// from LDS, older ones from the stored output, and sources before the
+// segment's output become the markers 0x8000 | (offset before the segment)
+// of fast_inflate.rs:2209-2222.  Complete 64-cell lines then go from the
+// ring to memory as 16-byte stores.
 // ===========================================================================
-constexpr int EXPAND_THREADS = 256;
+constexpr uint32_t kRing = 2048;        // cells of recent output per wave, in LDS (8 waves per SIMD)
+constexpr uint32_t kBatchCells = 1024;  // cells per batch (<= kRing / 2)
 
 __device__ __forceinline__ uint32_t tok_len(uint32_t w) {
     const uint32_t ty = w >> 30;
     return ty == TOK_LIT ? (w >> 24) & 3u : ty == TOK_MATCH ? ((w >> 16) & 0xffu) + 3u : ty == TOK_STORED ? (w & 0xffffu) : 0u;
 }
 
-__global__ __launch_bounds__(EXPAND_THREADS) void k_expand(const uint32_t* __restrict__ toks,
-                                                         const ExpandItem* __restrict__ items,
-                                                         uint32_t n_items, uint32_t* __restrict__ counter,
-                                                         const uint8_t* __restrict__ in8, uint16_t* __restrict__ cells,
-                                                         uint8_t* __restrict__ out) {
-    const int lane = (int)(threadIdx.x & 63);
+__global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks, const ExpandItem* __restrict__ items,
+                                               uint32_t n_items, uint32_t* __restrict__ counter,
+                                               const uint8_t* __restrict__ in8, uint16_t* __restrict__ cells,
+                                               uint8_t* __restrict__ out) {
+    __shared__ __align__(16) uint16_t ring[kRing];
+    constexpr uint32_t RM = kRing - 1;
+    const int lane = (int)threadIdx.x;
     while (true) {
         uint32_t wi = 0;
         if (lane == 0) wi = atomicAdd(counter, 1u);
@@ -1618,11 +1666,15 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(const uint32_t* __res
         const uint32_t* tk = toks + tok_base;
         uint16_t* c16 = cells + dst;
         uint8_t* o8 = out + dst;
-        const uint8_t* src = in8 + in_byte;
-        uint32_t p0 = 0;  // first cell of the batch
-        for (uint32_t t0 = 0; t0 < n_tok; t0 += 64) {
+        const uint8_t* src_in = in8 + in_byte;
+        uint32_t P0 = 0;       // first cell of the batch
+        uint32_t flushed = 0;  // cells stored to memory (a multiple of 64 until the end)
+        // the tokens of the next batch are loaded while this one is expanded
+        uint32_t wn = (uint32_t)lane < n_tok ? tk[lane] : (TOK_OFFSET << 30);
+        for (uint32_t t0 = 0; t0 < n_tok;) {
             const uint32_t i = t0 + (uint32_t)lane;
-            const uint32_t w = i < n_tok ? tk[i] : (TOK_OFFSET << 30);
+            const uint32_t w = wn;
+            const uint32_t ty = w >> 30;
             const uint32_t L = tok_len(w);
             uint32_t incl = L;
 #pragma unroll
@@ -1630,87 +1682,106 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(const uint32_t* __res
                 const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 64);
                 if (lane >= d) incl += o;
             }
-            const uint32_t total = uni32((uint32_t)__shfl((int)incl, 63, 64));
-            const uint32_t S = p0 + incl - L;  // this token's first cell
-            // a stored token's offset word is the next token
+            // the tokens of this batch: a prefix of the lanes (every token is
+            // at most 258 cells, so the first always fits)
+            const uint32_t nb = (uint32_t)__popcll(__ballot(incl <= kBatchCells && i < n_tok));
+            const uint32_t T = uni32((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)nb - 1));
+            {
+                const uint32_t n2 = t0 + nb + (uint32_t)lane;  // lane's token of the next batch
+                const uint32_t keep = (uint32_t)__shfl((int)w, (int)min((uint32_t)lane + nb, 63u), 64);
+                // tokens already loaded move down; the rest are loaded now
+                wn = (uint32_t)lane + nb < 64 ? keep : (n2 < n_tok ? tk[n2] : (TOK_OFFSET << 30));
+            }
+            const uint32_t S = P0 + incl - L;  // this token's first cell
             uint32_t aux = (uint32_t)__shfl((int)w, lane < 63 ? lane + 1 : 63, 64);
-            if ((w >> 30) == TOK_STORED && lane == 63 && i + 1 < n_tok) aux = tk[i + 1];
-            const uint32_t pe = min(p0 + total, n_cells);
-            uint32_t ka = 0;  // the token owning the window's first cell
-            for (uint32_t c0 = p0 & ~63u; c0 < pe; c0 += 64) {
-                const uint32_t cs = max(c0, p0);  // cells before cs are stored already
-                const uint32_t p = c0 + (uint32_t)lane;
-                // owner of cell p: the last token starting at or before it
-                uint32_t own = ka;
-                for (uint32_t k = ka + 1; k < 64; ++k) {
-                    const uint32_t sk = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k));
-                    if (sk > c0 + 63) break;
-                    if (p >= sk) own = k;
-                }
-                ka = uni32((uint32_t)__builtin_amdgcn_readlane((int)own, 63));
-                bool done = !(p >= cs && p < pe);
-                uint32_t pp = p, kk = own, val = 0, mode = 0;  // mode 1: load cell q, 2: load input byte q
-                uint32_t q = 0;
-                while (__ballot(!done)) {
-                    const uint32_t wk = (uint32_t)__shfl((int)w, (int)kk, 64);
-                    const uint32_t sk = (uint32_t)__shfl((int)S, (int)kk, 64);
-                    const uint32_t ak = (uint32_t)__shfl((int)aux, (int)kk, 64);
-                    uint32_t next = pp;
-                    if (!done) {
-                        const uint32_t off = pp - sk;
-                        const uint32_t ty = wk >> 30;
-                        if (ty == TOK_LIT) {
-                            val = (wk >> (8 * off)) & 0xffu;
-                            done = true;
-                        } else if (ty == TOK_STORED) {
-                            q = (ak & 0x3fffffffu) + off;
-                            mode = 2;
-                            done = true;
-                        } else {
-                            const uint32_t d = (wk & 0x7fffu) + 1u;
-                            const uint32_t r = off < d ? off : (d == 1 ? 0u : off % d);
-                            if (sk + r < d) {  // before the segment's output: a marker
-                                val = 0x8000u | (d - (sk + r) - 1u);
-                                done = true;
-                            } else {
-                                q = sk + r - d;
-                                if (q < cs) {
-                                    mode = 1;
-                                    done = true;
-                                } else {
-                                    next = q;  // in this window: follow it
-                                }
-                            }
+            if (ty == TOK_STORED && lane == 63 && i + 1 < n_tok) aux = tk[i + 1];
+            const bool mine = (uint32_t)lane < nb;
+            bool done = !mine || ty == TOK_OFFSET || L == 0;
+            if (!done && ty == TOK_LIT) {
+                for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = (uint16_t)((w >> (8 * j)) & 0xffu);
+                done = true;
+            } else if (!done && ty == TOK_STORED) {
+                const uint8_t* sp = src_in + (aux & 0x3fffffffu);
+                for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = sp[j];
+                done = true;
+            }
+            // matches, in rounds (32-bit cell arithmetic: a segment has < 2^32 cells)
+            const uint32_t d = (w & 0x7fffu) + 1u;
+            const uint32_t reach = S + min(L, d);
+            const uint32_t need = reach > d ? reach - d : 0u;  // the cells it reads lie below this (or before the segment)
+            const uint32_t ring_lo = P0 + T > kRing ? P0 + T - kRing : 0u;  // older cells are no longer in the ring
+            while (true) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint64_t dm = __ballot(done);
+                if (dm == ~0ull) break;
+                const uint32_t k0 = (uint32_t)__builtin_ctzll(~dm);
+                const uint32_t frontier = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k0));
+                if (!done && need <= frontier) {
+                    uint32_t jm = 0;  // j mod d
+                    for (uint32_t j0 = 0; j0 < L; j0 += 8) {
+                        uint32_t v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const uint32_t sj = S + jm;
+                            const uint32_t q = sj - d;  // wraps when the source precedes the segment
+                            const bool mk = sj < d;
+                            const bool rg = !mk && q >= ring_lo;
+                            const uint32_t xr = ring[q & RM];
+                            uint32_t x = mk ? (0x8000u | (d - sj - 1u)) : xr;
+                            if (!mk && !rg && j0 + (uint32_t)u < L) x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
+                            v[u] = x;
+                            jm = (jm + 1 == d) ? 0u : jm + 1;
                         }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (j0 + (uint32_t)u < L) ring[(S + j0 + (uint32_t)u) & RM] = (uint16_t)v[u];
                     }
-                    // every lane takes part in the shuffle
-                    const uint32_t nk = (uint32_t)__shfl((int)own, (int)((next - c0) & 63u), 64);
-                    if (!done) {
-                        pp = next;
-                        kk = nk;
-                    }
-                }
-                if (mode == 1) val = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
-                else if (mode == 2) val = src[q];
-                if (p >= cs && p < pe) {
-                    if (u8) o8[p] = (uint8_t)val;
-                    else c16[p] = (uint16_t)val;
+                    done = true;
                 }
             }
-            p0 += total;
+            // store the complete lines (all of the rest after the last batch)
+            const uint32_t end = P0 + T;
+            const bool last = t0 + nb >= n_tok;
+            const uint32_t upto = last ? min(end, n_cells) : (end & ~63u);
+            for (uint32_t c = flushed + 8u * (uint32_t)lane; c < upto; c += 512) {
+                const uint32_t r = c & RM;  // 8 cells never wrap: c and kRing are multiples of 8
+                if (c + 8 <= upto) {
+                    const uint4 g = *reinterpret_cast<const uint4*>(ring + r);
+                    if (u8) {
+                        const uint32_t lo = __builtin_amdgcn_perm(g.y, g.x, 0x06040200u);
+                        const uint32_t hi = __builtin_amdgcn_perm(g.w, g.z, 0x06040200u);
+                        *reinterpret_cast<uint2*>(o8 + c) = make_uint2(lo, hi);
+                    } else {
+                        *reinterpret_cast<uint4*>(c16 + c) = g;
+                    }
+                } else {
+                    for (uint32_t k = c; k < upto; ++k) {
+                        if (u8) o8[k] = (uint8_t)ring[k & RM];
+                        else c16[k] = ring[k & RM];
+                    }
+                }
+            }
+            flushed = upto;
+            P0 = end;
+            t0 += nb;
         }
+        // the next item reuses the ring: its first reads come after its own writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
-
-int expand_blocks(int n_cu) { return n_cu * 8; }
 
 hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint32_t n_items, uint32_t* d_counter,
                          const uint8_t* d_in8, uint16_t* d_cells, uint8_t* d_out, int n_cu, hipStream_t s) {
     if (n_items == 0) return hipSuccess;
-    const uint32_t waves = (uint32_t)expand_blocks(n_cu) * (EXPAND_THREADS / 64);
-    const uint32_t blocks = (std::min<uint32_t>(waves, n_items) + EXPAND_THREADS / 64 - 1) / (EXPAND_THREADS / 64);
-    hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(EXPAND_THREADS), 0, s, d_toks, d_items, n_items, d_counter, d_in8,
-                       d_cells, d_out);
+    // persistent single-wave workgroups, as many as the LDS rings allow
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand, 64, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)n_cu, n_items);
+    hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(64), 0, s, d_toks, d_items, n_items, d_counter, d_in8, d_cells,
+                       d_out);
     return hipGetLastError();
 }
 
