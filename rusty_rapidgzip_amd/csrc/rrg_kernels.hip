@@ -1696,11 +1696,13 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
             uint32_t aux = (uint32_t)__shfl((int)w, lane < 63 ? lane + 1 : 63, 64);
             if (ty == TOK_STORED && lane == 63 && i + 1 < n_tok) aux = tk[i + 1];
             const bool mine = (uint32_t)lane < nb;
+            // long matches and stored pieces are copied by the whole wave
+            const bool longtok = L > 32 && (ty == TOK_MATCH || ty == TOK_STORED);
             bool done = !mine || ty == TOK_OFFSET || L == 0;
             if (!done && ty == TOK_LIT) {
                 for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = (uint16_t)((w >> (8 * j)) & 0xffu);
                 done = true;
-            } else if (!done && ty == TOK_STORED) {
+            } else if (!done && ty == TOK_STORED && !longtok) {
                 const uint8_t* sp = src_in + (aux & 0x3fffffffu);
                 for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = sp[j];
                 done = true;
@@ -1718,7 +1720,7 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                 if (dm == ~0ull) break;
                 const uint32_t k0 = (uint32_t)__builtin_ctzll(~dm);
                 const uint32_t frontier = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k0));
-                if (!done && need <= frontier) {
+                if (!done && !longtok && need <= frontier) {
                     uint32_t jm = 0;  // j mod d
                     for (uint32_t j0 = 0; j0 < L; j0 += 8) {
                         uint32_t v[8];
@@ -1739,6 +1741,32 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                             if (j0 + (uint32_t)u < L) ring[(S + j0 + (uint32_t)u) & RM] = (uint16_t)v[u];
                     }
                     done = true;
+                }
+                // the first unfinished token is always ready; if it is long, the
+                // 64 lanes copy it together, 64 cells at a time (cell j comes
+                // from the cell j mod d after the start of its source)
+                if (uni32((uint32_t)__builtin_amdgcn_readlane((int)(longtok ? 1u : 0u), (int)k0))) {
+                    const uint32_t kw = uni32((uint32_t)__builtin_amdgcn_readlane((int)w, (int)k0));
+                    const uint32_t kL = tok_len(kw);
+                    const uint32_t kS = frontier;
+                    if ((kw >> 30) == TOK_STORED) {
+                        const uint8_t* sp =
+                            src_in + (uni32((uint32_t)__builtin_amdgcn_readlane((int)aux, (int)k0)) & 0x3fffffffu);
+                        for (uint32_t j = (uint32_t)lane; j < kL; j += 64) ring[(kS + j) & RM] = sp[j];
+                    } else {
+                        const uint32_t kd = (kw & 0x7fffu) + 1u;
+                        for (uint32_t j = (uint32_t)lane; j < kL; j += 64) {
+                            const uint32_t jm = kd == 1 ? 0u : (j < kd ? j : j % kd);
+                            const uint32_t sj = kS + jm;
+                            const uint32_t q = sj - kd;
+                            uint32_t x;
+                            if (sj < kd) x = 0x8000u | (kd - sj - 1u);
+                            else if (q >= ring_lo) x = ring[q & RM];
+                            else x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
+                            ring[(kS + j) & RM] = (uint16_t)x;
+                        }
+                    }
+                    if ((uint32_t)lane == k0) done = true;
                 }
             }
             // store the complete lines (all of the rest after the last batch)
