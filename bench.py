@@ -3,15 +3,19 @@
 Default workload (the headline configuration): 4 GiB of synthetic FASTQ-like
 text (4-line records '@r<id>', 150 bp ACGT with ~0.1% N, '+', 150-char Phred
 walk), gzip level 6, one member, read_gz chunk size 4 MiB.  One step = one
-complete device-resident decode of the file: block finder, speculative
-inflate, chain verification, tail chain, marker resolution, CRC32 + ISIZE
-checks, output left in HBM with the reference's chunk boundaries.  The
-compressed input is already resident in HBM when the timed region starts.
+complete device-resident decode of the file: block finder, inflate (Huffman
+tokens, then expansion), chain verification, tail chain, marker resolution,
+CRC32 + ISIZE checks, output left in HBM with the reference's chunk
+boundaries.  The compressed input is already resident in HBM when the timed
+region starts.
 
 Prints ONE JSON line.  `cpu_baseline` times the CPU restatement of read_gz
-(oracle/, "port") on this host's cores over a bounded sample of the same
-workload.  `roofline.traffic` is the HBM traffic per inflate launch measured
-with rocprofv3 counters (profiles/traffic_fastq4g.json), when present.
+(oracle/, "port") on this host's cores over the whole workload, with zlib and
+`gzip -dc` single-thread anchors.  `roofline` is for the dominant kernel
+(inflate pass 1, k_tokens); `roofline.traffic` is its HBM traffic per launch
+measured with rocprofv3 counters (profiles/traffic_fastq4g.json), when
+present.  `other_configs` holds the BGZF, mixed-entropy and 64 MiB text
+configurations of the same run (each output-verified against its generator).
 
 Other workloads (--workload), each one JSON line of its own:
   bgzf    the same 4 GiB as BGZF (65280-byte members, level 6), one batch
@@ -37,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fastq4g.json")
+STAGES = ("ms_find", "ms_inflate", "ms_expand", "ms_tail", "ms_resolve", "ms_crc")
 
 
 def log(*a):
@@ -137,11 +142,14 @@ def run_fastq(args, threads):
     def avg(key):
         return sum(p[key] for p in prof) / len(prof)
 
+    # dominant kernel: inflate pass 1 (k_tokens).  Algorithmic bytes per
+    # launch: the compressed stream read once plus the 32-bit tokens written
+    # (C + 4 * tokens); pass 2 (k_expand) reads the tokens and writes the u16
+    # cells (4 * tokens + 2U).
     ms_inflate = avg("ms_inflate")
     launches = max(1.0, avg("n_inflate_launches"))
-    # dominant kernel: speculative inflate.  Algorithmic bytes per launch: the
-    # compressed stream read once plus the u16 cells written (C + 2U).
-    infl_bytes = (C + 2 * n) / launches
+    n_tok = avg("n_tokens")
+    infl_bytes = (C + 4 * n_tok) / launches
     achieved = infl_bytes / (ms_inflate / launches / 1e3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -150,15 +158,17 @@ def run_fastq(args, threads):
         if (t.get("compressed_bytes") == C and t.get("uncompressed_bytes") == n
                 and t.get("lut_bits") == cfg.lut_bits and t.get("lanes_per_block") == cfg.lanes_per_block
                 and not cfg.segment_bytes):
-            traffic = t.get("k_inflate_bytes_per_launch")
+            traffic = t.get("k_tokens_bytes_per_launch")
     result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, CRC-checked)", value, "GB/s",
                    args.steps, args.warmup, elapsed * 1e3 / args.steps)
     result["config"] = {
-        "workload": f"fastq-like {args.gib:g} GiB, gzip -6 single member, 4-line records, 150 bp",
+        "workload": f"fastq-like {args.gib:g} GiB, gzip -6 single member, 4-line records, 150 bp "
+                    "(written by zlib in 64 MiB pieces joined by sync flushes)",
         "chunk_size_bytes": args.chunk_mib << 20,
         "compressed_bytes": C,
         "uncompressed_bytes": n,
         "segments": int(avg("n_segments")),
+        "tokens": int(n_tok),
         "lut_bits": cfg.lut_bits,
         "lanes_per_block": cfg.lanes_per_block,
         "segment_bytes": cfg.segment_bytes or "automatic",
@@ -166,10 +176,15 @@ def run_fastq(args, threads):
         "input": "resident in HBM",
     }
     result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
-    result["stages_ms"] = {k: round(avg(k), 3) for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+    result["stages_ms"] = {k: round(avg(k), 3) for k in STAGES}
+    result["finder"] = {
+        "candidate_bit_offsets_per_s": round(avg("find_bits") / (avg("ms_find") / 1e3), 1),
+        "scanned_bits": int(avg("find_bits")),
+        "how": "bit offsets from each target to its find (or scan end), per second of the find stage",
+    }
     result["roofline"] = {
         "bound": "hbm",
-        "kernel": "k_inflate",
+        "kernel": "k_tokens",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -178,14 +193,15 @@ def run_fastq(args, threads):
         "algorithmic_bytes_per_launch": int(infl_bytes),
         "end_to_end_frac": round((C + n) * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 5),
         "other_kernels": {
+            "k_expand": round((4 * n_tok + 2 * n) / (avg("ms_expand") / 1e3) / 1e9, 1),
             "k_resolve": round(3 * n / (avg("ms_resolve") / 1e3) / 1e9, 1),
             "k_crc_pieces": round(n / (avg("ms_crc") / 1e3) / 1e9, 1),
-            "unit": "GB/s algorithmic (resolve 3U, CRC U)",
+            "unit": "GB/s algorithmic (expand 4*tokens + 2U, resolve 3U, CRC U)",
         },
     }
     result["cpu_baseline"] = None
     d_in.free()
-    del raw
+    raw_keep = raw
     if not args.no_e2e:
         # the same decode through the host-buffer entry point (rrg_gpu_decode):
         # H2D of the file, decode, one D2H per chunk into a callback; never the value
@@ -211,39 +227,154 @@ def run_fastq(args, threads):
         log(f"host to host {result['host_to_host']}")
 
     if not args.no_cpu:
-        from oracle import oracle
+        result["cpu_baseline"] = cpu_baseline(args, threads, raw_keep, gz, n)
+    result["_raw"] = raw_keep
+    return result
 
-        sn = min(n, args.cpu_sample_mib << 20)
-        sraw = synth.fastq(sn, threads=threads)
-        sgz = synth.gzip(sraw, 6, threads=threads, piece=64 << 20)
-        times = []
-        for _ in range(3):
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, threads, raw, gz, n):
+    """The CPU restatement of read_gz (oracle/, "port") on the whole workload,
+    all host threads, median of 5; plus two single-thread anchors on a
+    bounded sample: zlib inflate (Python's zlib) and `gzip -dc > /dev/null`."""
+    import subprocess
+    import tempfile
+    import zlib
+
+    from oracle import oracle
+
+    times = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        rc, _, st, _ = oracle.read_gz(gz, chunk_size=args.chunk_mib << 20, threads=threads, keep=False)
+        times.append(time.perf_counter() - t1)
+        if rc != 0 or st.uncompressed_bytes != n:
+            raise RuntimeError("CPU baseline decode failed")
+    t_med = sorted(times)[len(times) // 2]
+    # anchors on the first 256 MiB of output (a gzip file of its own)
+    from rusty_rapidgzip_amd import synth
+
+    an = min(n, 256 << 20)
+    agz = synth.gzip(raw[:an], 6, threads=threads, piece=64 << 20).tobytes()
+    t1 = time.perf_counter()
+    d = zlib.decompressobj(31)
+    out = 0
+    for k in range(0, len(agz), 1 << 22):
+        out += len(d.decompress(agz[k:k + (1 << 22)]))
+    out += len(d.flush())
+    z_gbs = an / (time.perf_counter() - t1) / 1e9
+    assert out == an
+    g_gbs = None
+    with tempfile.NamedTemporaryFile(suffix=".gz", delete=False) as f:
+        f.write(agz)
+        path = f.name
+    try:
+        t1 = time.perf_counter()
+        r = subprocess.run(["gzip", "-dc", path], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        if r.returncode == 0:
+            g_gbs = an / (time.perf_counter() - t1) / 1e9
+    except OSError:
+        pass
+    finally:
+        os.unlink(path)
+    res = {
+        "value": round(n / t_med / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the whole workload ({n >> 20} MiB out, gzip -6, same chunking); median of 5 "
+                  f"({', '.join(f'{t:.3f}' for t in times)} s)",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "anchors": {"zlib_1_thread_GBps": round(z_gbs, 3),
+                    "gzip_dc_GBps": round(g_gbs, 3) if g_gbs else None,
+                    "anchor_sample": f"first {an >> 20} MiB of the workload as its own gzip -6 file"},
+    }
+    log(f"cpu baseline {res}")
+    return res
+
+
+def _sub(result):
+    """The fields of a workload's result kept in the main line's other_configs."""
+    keep = ("metric", "value", "unit", "ms_per_step", "config", "latency_ms", "stages_ms", "inflate_launches",
+            "cpu_read_gz", "verified")
+    return {k: result[k] for k in keep if k in result}
+
+
+def run_text64m(args, threads):
+    """Config 1: 64 MiB of order-2 Markov text, gzip -6, one member: GPU
+    decode and the CPU restatement of read_gz (all threads and one)."""
+    import numpy as np
+
+    import rusty_rapidgzip_amd as rrg
+    from oracle import oracle
+    from rusty_rapidgzip_amd import hipmem, synth
+
+    n = 64 << 20
+    raw = synth.text(n, threads=threads)
+    gz = synth.gzip(raw, 6, threads=threads, piece=64 << 20)
+    C = int(gz.size)
+    d_in = hipmem.DeviceBuffer.from_host(gz)
+    cfg = rrg.Config()
+    dec = rrg.Decoder(0)
+
+    def step():
+        return dec.decode_device(d_in.ptr, C, gz, cfg)
+
+    ptr, out_len, ends, st = step()
+    ok = out_len == n and np.array_equal(hipmem.to_host(ptr, n), raw)
+    if not ok:
+        raise RuntimeError("text64m: GPU output differs from the generated bytes")
+    prof = []
+
+    def step_p():
+        step()
+        prof.append(dec.profile().as_dict())
+
+    steps = max(3, args.steps)
+    elapsed, lat = timed_steps(step_p, steps)
+    result = _base("decompressed GB/s on 1 MI355X (device-resident read_gz, text)", n * steps / elapsed / 1e9,
+                   "GB/s", steps, 1, elapsed * 1e3 / steps)
+    result["config"] = {"workload": "64 MiB order-2 Markov text (64 symbols, Dirichlet 0.3 rows), gzip -6, one member",
+                        "compressed_bytes": C, "uncompressed_bytes": n, "chunks": len(ends)}
+    result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
+    result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3) for k in STAGES}
+    result["verified"] = True
+    cpu = {}
+    for th in (() if args.no_cpu else (threads, 1)):
+        ts = []
+        for _ in range(5):
             t1 = time.perf_counter()
-            rc, _, st, _ = oracle.read_gz(sgz, chunk_size=args.chunk_mib << 20, threads=threads, keep=False)
-            times.append(time.perf_counter() - t1)
-            if rc != 0 or st.uncompressed_bytes != sn:
-                raise RuntimeError("CPU baseline decode failed")
-        t_med = sorted(times)[len(times) // 2]
-        result["cpu_baseline"] = {
-            "value": round(sn / t_med / 1e9, 3),
-            "unit": "GB/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{sn >> 20} MiB of the same FASTQ-like generator, gzip -6, same chunking; median of 3",
-        }
-        log(f"cpu baseline {result['cpu_baseline']}")
+            rc, _, cst, _ = oracle.read_gz(gz, chunk_size=4 << 20, threads=th, keep=False)
+            ts.append(time.perf_counter() - t1)
+            if rc != 0 or cst.uncompressed_bytes != n:
+                raise RuntimeError("text64m: CPU read_gz failed")
+        cpu[f"threads_{th}"] = round(n / sorted(ts)[2] / 1e9, 3)
+    if cpu:
+        result["cpu_read_gz"] = {**cpu, "unit": "GB/s", "kind": "port", "how": "median of 5"}
+    d_in.free()
     return result
 
 
 # ---------------------------------------------------------------------------
-def run_bgzf(args, threads):
+def run_bgzf(args, threads, raw=None):
     import numpy as np
 
     import rusty_rapidgzip_amd as rrg
     from rusty_rapidgzip_amd import hipmem, synth
 
     n = int(args.gib * (1 << 30))
-    raw = synth.fastq(n, threads=threads)
+    if raw is None:
+        raw = synth.fastq(n, threads=threads)
     t0 = time.time()
     bg = synth.bgzf(raw, 6, threads=threads)
     C = int(bg.size)
@@ -274,7 +405,8 @@ def run_bgzf(args, threads):
                         "members": int(prof[-1]["n_segments"]), "input": "resident in HBM"}
     result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
     result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3)
-                           for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+                           for k in STAGES}
+    result["verified"] = True
     d_in.free()
     return result
 
@@ -323,7 +455,7 @@ def run_members(args, threads):
                         "input": "resident in HBM"}
     result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
     result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3)
-                           for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+                           for k in STAGES}
     d_in.free()
     return result
 
@@ -372,8 +504,9 @@ def run_mixed(args, threads):
                         "segments": int(prof[-1]["n_segments"]), "input": "resident in HBM"}
     result["latency_ms"] = {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}
     result["stages_ms"] = {k: round(sum(p[k] for p in prof) / len(prof), 3)
-                           for k in ("ms_find", "ms_inflate", "ms_tail", "ms_resolve", "ms_crc")}
+                           for k in STAGES}
     result["inflate_launches"] = int(prof[-1]["n_inflate_launches"])
+    result["verified"] = True
     d_in.free()
     return result
 
@@ -505,7 +638,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="fastq4g", choices=("fastq4g", "bgzf", "members", "mixed", "batch", "serve"))
+    ap.add_argument("--workload", default="fastq4g",
+                    choices=("fastq4g", "bgzf", "members", "mixed", "text64m", "batch", "serve"))
     ap.add_argument("--member-kib", type=int, default=1024, help="member size of the members workload")
     ap.add_argument("--mixed-gib", type=float, default=2.0, help="uncompressed size of the mixed workload")
     ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
@@ -516,6 +650,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
+    ap.add_argument("--no-others", action="store_true",
+                    help="fastq4g only: skip the other configurations (BGZF, mixed entropy, text)")
     ap.add_argument("--autotune", action="store_true", help="pick lanes / LUT width / segment size with the autotuner")
     ap.add_argument("--batch-files", type=int, default=64)
     ap.add_argument("--batch-mib", type=int, default=64)
@@ -530,9 +666,29 @@ def main() -> int:
         log("only --gpus 1 is supported: the decoder runs on a single MI355X")
         return 2
     threads = min(16, os.cpu_count() or 1)
-    fn = {"fastq4g": run_fastq, "bgzf": run_bgzf, "members": run_members, "mixed": run_mixed, "batch": run_batch,
-          "serve": run_serve}[args.workload]
+    fn = {"fastq4g": run_fastq, "bgzf": run_bgzf, "members": run_members, "mixed": run_mixed, "text64m": run_text64m,
+          "batch": run_batch, "serve": run_serve}[args.workload]
     result = fn(args, threads)
+    result.pop("_raw", None)
+    if args.workload == "fastq4g" and not args.no_others:
+        # the other configurations of the scope table, each output-verified
+        # against its generator (one step = one whole decode), each in a
+        # process of its own
+        others = {}
+        for name in ("bgzf", "mixed", "text64m"):
+            cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(args.steps),
+                   "--warmup", "1", "--no-e2e", "--lut-bits", str(args.lut_bits), "--lanes", str(args.lanes)]
+            if args.no_cpu:
+                cmd.append("--no-cpu")
+            import subprocess
+
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if r.returncode != 0 or not lines:
+                raise RuntimeError(f"{name}: child bench failed with status {r.returncode}")
+            others[name] = _sub(json.loads(lines[-1]))
+            log(f"{name} {others[name]['value']} GB/s")
+        result["other_configs"] = others
     print(json.dumps(result), flush=True)
     return 0
 

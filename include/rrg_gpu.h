@@ -109,6 +109,7 @@ typedef struct {
     uint64_t n_grown;         /* token regions grown after an overflow */
     double ms_expand;         /* inflate pass 2 (tokens to cells / bytes) */
     uint64_t n_tokens;        /* tokens of the segments expanded */
+    uint64_t find_bits;       /* bit offsets the block finder scanned (targets to finds or scan ends) */
 } rrg_profile;
 
 int rrg_gpu_abi_version(void);

@@ -82,6 +82,7 @@ class Profile(ctypes.Structure):
         ("n_grown", ctypes.c_uint64),
         ("ms_expand", ctypes.c_double),
         ("n_tokens", ctypes.c_uint64),
+        ("find_bits", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -930,6 +930,11 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
         std::fprintf(stderr, "[rrg-gpu] finder: %zu targets, %llu without a find, longest scan %llu KiB\n", n,
                      (unsigned long long)none, (unsigned long long)(longest / 8192));
     }
+    // bit offsets scanned: from each target to its find, or to where its scan ended
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t e = found[i] != UINT64_MAX ? found[i] : exits ? std::min((*exits)[i], scan_ends[i]) : scan_ends[i];
+        if (e > targets[i]) c.prof.find_bits += e - targets[i];
+    }
     // a find is a bit offset inside the target's scan range; anything else is a device fault
     for (size_t i = 0; i < n; ++i)
         if (found[i] != UINT64_MAX && (found[i] < targets[i] || found[i] >= scan_ends[i])) {
