@@ -17,6 +17,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 GPU_LIB = os.path.join(PKG_DIR, "librrg_gpu.so")
 SYNTH_LIB = os.path.join(PKG_DIR, "synth", "librrg_synth.so")
+CLI_BIN = os.path.join(PKG_DIR, "rapidgzip-gpu")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -42,6 +43,16 @@ def build_gpu_lib() -> str:
     _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO_DIR, "include"), *srcs, "-o", tmp, "-lpthread"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
+
+
+def build_cli() -> str:
+    """The command-line drop-in (cli/rrg_cli.cpp), linked against librrg_gpu.so next to it."""
+    src = os.path.join(PKG_DIR, "cli", "rrg_cli.cpp")
+    tmp = CLI_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(REPO_DIR, "include"), src, "-o", tmp, "-L", PKG_DIR,
+          "-lrrg_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, CLI_BIN)
+    return CLI_BIN
 
 
 def build_synth_lib() -> str:

@@ -379,3 +379,17 @@ def test_native_library_exports():
     for sym in declared:
         assert hasattr(lib, sym), sym
     assert lib.rrg_gpu_abi_version() == 3
+
+
+def test_cli_arguments():
+    # the command-line drop-in parses the reference's arguments (main.rs:12-26)
+    import subprocess
+
+    from rusty_rapidgzip_amd import _native
+
+    r = subprocess.run([_native.CLI_BIN, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "-P, --threads" in r.stdout and "--chunk-size" in r.stdout
+    r = subprocess.run([_native.CLI_BIN, "--bogus", "x.gz"], capture_output=True, text=True)
+    assert r.returncode == 2 and "bad argument" in r.stderr
+    r = subprocess.run([_native.CLI_BIN], capture_output=True, text=True)
+    assert r.returncode == 2
