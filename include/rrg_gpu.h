@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define RRG_GPU_ABI_VERSION 3
+#define RRG_GPU_ABI_VERSION 4
 
 /* Error kinds; values match the variants of the reference error enums. */
 enum rrg_error_kind {
@@ -87,6 +87,12 @@ typedef struct {
     size_t segment_bytes;     /* compressed bytes per lane-decoded segment */
     int lut_bits;             /* literal/length LUT bits: 8, 9 (default) or 10 */
     int lanes_per_block;      /* lanes (segments in flight) per workgroup (default 16) */
+    size_t device_budget_bytes; /* rrg_gpu_decode / rrg_gpu_read_gz on a file: device memory the
+                                 decode aims to stay within (0 = half the device's free memory).
+                                 The file is decoded in slabs of whole chunks sized to it, and the
+                                 chunks of each slab go to the callback while the next slab
+                                 decodes.  A slab holds at least one chunk, so a chunk whose
+                                 output alone exceeds the budget still decodes. */
 } rrg_cfg;
 
 /* Return nonzero to close the stream (the reference's dropped Receiver). */
@@ -110,6 +116,9 @@ typedef struct {
     double ms_expand;         /* inflate pass 2 (tokens to cells / bytes) */
     uint64_t n_tokens;        /* tokens of the segments expanded */
     uint64_t find_bits;       /* bit offsets the block finder scanned (targets to finds or scan ends) */
+    double ms_first_chunk;    /* host wall clock from the call to the first chunk handed to the callback */
+    uint64_t n_slabs;         /* slabs the file was decoded in (host-output decode of a file) */
+    uint64_t device_bytes_peak; /* device memory the context held at the peak of the call */
 } rrg_profile;
 
 int rrg_gpu_abi_version(void);

@@ -38,7 +38,8 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_gpu_lib() -> str:
-    srcs = [os.path.join(CSRC, f) for f in ("rrg_kernels.hip", "rrg_gpu.cpp", "rrg_serve.cpp", "rrg_noalloc.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("rrg_kernels.hip", "rrg_gpu.cpp", "rrg_serve.cpp", "rrg_slab.cpp",
+                                              "rrg_noalloc.hip")]
     tmp = GPU_LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO_DIR, "include"), *srcs, "-o", tmp, "-lpthread"])
     os.replace(tmp, GPU_LIB)

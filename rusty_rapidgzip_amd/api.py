@@ -60,6 +60,7 @@ class _Cfg(ctypes.Structure):
         ("segment_bytes", ctypes.c_size_t),
         ("lut_bits", ctypes.c_int),
         ("lanes_per_block", ctypes.c_int),
+        ("device_budget_bytes", ctypes.c_size_t),
     ]
 
 
@@ -83,6 +84,9 @@ class Profile(ctypes.Structure):
         ("ms_expand", ctypes.c_double),
         ("n_tokens", ctypes.c_uint64),
         ("find_bits", ctypes.c_uint64),
+        ("ms_first_chunk", ctypes.c_double),
+        ("n_slabs", ctypes.c_uint64),
+        ("device_bytes_peak", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -215,7 +219,9 @@ class Verbosity(enum.Enum):
 class Config:
     """lib.rs:49-100.  ``num_threads == 1`` gives the reference's single-worker chunking (chunks
     end at block ends, pipeline.rs:1211-1345 / lib.rs:244-361); the GPU decode is the same for
-    any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition."""
+    any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition;
+    ``device_budget_bytes`` bounds the device memory of a host-output decode (0 = half the
+    device's free memory), which then runs in slabs of whole chunks."""
 
     num_threads: int = 0
     chunk_size_bytes: int = 4 * 1024 * 1024
@@ -223,6 +229,7 @@ class Config:
     segment_bytes: int = 0
     lut_bits: int = 9
     lanes_per_block: int = 16
+    device_budget_bytes: int = 0
 
     def _c(self) -> _Cfg:
         c = _Cfg()
@@ -233,6 +240,7 @@ class Config:
         c.segment_bytes = int(self.segment_bytes)
         c.lut_bits = int(self.lut_bits)
         c.lanes_per_block = int(self.lanes_per_block)
+        c.device_budget_bytes = int(self.device_budget_bytes)
         return c
 
 
@@ -426,11 +434,31 @@ class Decoder:
         return self._run(lambda c, cb, st, er: self._lib.rrg_gpu_read_gz(self._ctx, p, c, cb, None, st, er),
                          config, sink)
 
-    def decode_bytes(self, data: bytes, sink, config: Config | None = None) -> DecodeStats:
-        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data) if data else (ctypes.c_uint8 * 1)()
+    def decode_bytes(self, data, sink, config: Config | None = None) -> DecodeStats:
+        """read_gz over a gzip file in host memory (bytes, bytearray or a uint8 array; not copied)."""
+        ptr, keep = _host_ptr(data)
+        n = len(keep) if keep is not None else 0
+        if not ptr:
+            keep = (ctypes.c_uint8 * 1)()
+            ptr = ctypes.addressof(keep)
         return self._run(
-            lambda c, cb, st, er: self._lib.rrg_gpu_decode(self._ctx, ctypes.addressof(buf), len(data), c, cb, None,
-                                                           st, er), config, sink)
+            lambda c, cb, st, er: self._lib.rrg_gpu_decode(self._ctx, ptr, n, c, cb, None, st, er), config, sink)
+
+    def decode_bytes_native(self, data, cb_addr: int, user_addr: int, config: Config | None = None) -> DecodeStats:
+        """decode_bytes with a native rrg_chunk_cb (its address and user pointer):
+        no Python runs per chunk."""
+        config = config or Config()
+        ptr, keep = _host_ptr(data)
+        n = len(keep) if keep is not None else 0
+        if not ptr:
+            keep = (ctypes.c_uint8 * 1)()
+            ptr = ctypes.addressof(keep)
+        st, er = _Stats(), _Err()
+        rc = self._lib.rrg_gpu_decode(self._ctx, ptr, n, ctypes.byref(config._c()), _CB(cb_addr),
+                                      ctypes.c_void_p(user_addr), ctypes.byref(st), ctypes.byref(er))
+        if rc != RRG_OK:
+            _raise(rc, er)
+        return _stats(st)
 
     def decode_fd(self, fd: int, sink, config: Config | None = None) -> DecodeStats:
         """read_gz over a non-seekable source (pipe, stdin): the streaming path's chunking."""

@@ -104,11 +104,15 @@ struct Ctx {
     int device = 0;
     hipStream_t s = nullptr;
     hipStream_t s2 = nullptr;  // a second stream (default priority) for work that overlaps s
+    hipStream_t su = nullptr;  // uploads of the next slab's input (slab decode)
+    hipStream_t sc = nullptr;  // copies of decoded chunks to the host (slab decode)
     hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
     uint64_t max_lanes = 0;  // cap on inflate lanes in flight (0 = as many as fit)
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, toks, xitems, cells, sorted, chain, tz,
         tiles, out, poff, plen, pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
+    DevBuf in2, out2;  // the other input and output buffer of the slab decode
+    DevBuf win;        // the last 32 KiB of output before the next slab
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
     std::vector<std::vector<uint64_t>> batch_ends;  // chunk ends of the last batched call, per file
     uint8_t* pin = nullptr;
@@ -121,7 +125,8 @@ struct Ctx {
     std::vector<DevBuf*> bufs() {
         return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,
                 &toks,  &xitems,  &cells,  &sorted,    &chain, &tz,    &tiles, &out,  &poff, &plen,
-                &pcrc,  &err,     &reach,  &first,     &stops, &exits, &recs,  &legtg, &lfound, &lnext, &exact_idx};
+                &pcrc,  &err,     &reach,  &first,     &stops, &exits, &recs,  &legtg, &lfound, &lnext, &exact_idx,
+                &in2,   &out2,    &win};
     }
     ~Ctx() {
         for (DevBuf* b : bufs()) b->release();
@@ -133,6 +138,8 @@ struct Ctx {
         for (hipEvent_t e : evf)
             if (e) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
+        if (su) (void)hipStreamDestroy(su);
+        if (sc) (void)hipStreamDestroy(sc);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -194,6 +201,27 @@ struct Job {
     // pipeline.rs:1211-1345, 2 = the streaming serial path of lib.rs:244-361
     // (non-seekable input).  Chunks end at block ends; no speculation counted.
     int serial = 0;
+    // One slab of a bounded-memory decode (rrg_slab.cpp).  `slab`: the member
+    // path's chunk starts are given in stream_B (stream_B[0] is where the
+    // chain starts) and its last segment stops at the first block start at or
+    // past stop_bit (UINT64_MAX: the final block).  `resume`: the chain starts
+    // inside the stream at stream_B[0], after output that earlier slabs
+    // produced; no header is parsed.
+    bool slab = false, resume = false;
+    uint64_t stop_bit = UINT64_MAX;
+    uint64_t win_len = 0;              // output bytes just before the slab that markers may reach (<= 32 KiB)
+    const uint8_t* win_src = nullptr;  // device: those bytes
+    uint64_t body_ref = 0;             // c.in byte that reported trailer offsets count from (wrapping)
+    uint32_t carry_raw = 0;            // raw CRC and length of the open member's output before the slab
+    uint64_t carry_len = 0;
+    uint64_t carry_member = 0;         // members whose trailers came before the slab
+    uint64_t bgzf_target0 = 65536;     // BGZF batch target the slab's first batch starts with
+    // results of a slab: the open member at the end of the chain
+    uint32_t open_raw = 0;
+    uint64_t open_len = 0, members_done = 0;
+    size_t open_r = SIZE_MAX;  // its CRC range
+    uint64_t chain_end = 0;    // end bit of the member-path chain
+    uint64_t bgzf_target_end = 65536;
     // host plan
     struct Mb {
         uint64_t off, size;  // file offsets
@@ -224,6 +252,9 @@ struct Job {
 
     uint64_t abs_bits(uint64_t file_byte) const { return (in_off + file_byte) * 8; }
     uint64_t end_bits() const { return (in_off + len) * 8; }
+    // where the member-path chain starts, and where its last segment may end
+    uint64_t chain_start() const { return resume ? stream_B[0] : abs_bits(body); }
+    uint64_t last_end() const { return stop_bit != UINT64_MAX ? stop_bit : end_bits(); }
 };
 
 void set_err(rrg_error* e, int kind, const char* fmt, ...);
@@ -238,6 +269,26 @@ void begin_profile(Ctx& c, uint64_t len);
 void print_profile(const Ctx& c);
 // Decode every job (inputs already in c.in[0, c.in_used)) into `out`.
 int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out, rrg_error* err);
+// Exact walks of the block finder (block_finder.rs:109-141) in c.in: found[i]
+// = the first verified block start on the skip chain from targets[i] below
+// scan_ends[i] (UINT64_MAX if none); limits[i] = the end of the target's
+// file.  Walks are cut at stops[i] (reported in exits[i]) and finish_open
+// continues the cut ones to their end.
+int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
+               const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err,
+               const std::vector<uint64_t>* stops = nullptr, std::vector<uint64_t>* exits = nullptr,
+               bool want_recs = false);
+int finish_open(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
+                const std::vector<uint64_t>& exits, std::vector<uint64_t>& found, rrg_error* err);
+constexpr uint64_t kExactCapBits = 8ull * 131072;  // bits an exact walk covers before it continues in legs
+// rrg_gpu_decode on a file in host memory, in slabs under a device-byte
+// budget, with the chunks of decoded slabs handed to cb while later slabs
+// decode (rrg_slab.cpp).  Not for the single-worker or streaming chunking.
+// Returns kNotSlabbed, having done nothing, for a file without a first
+// deflate body or BGZF member to cut (the whole-file decode reports it).
+constexpr int kNotSlabbed = -1;
+int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, rrg_chunk_cb cb, void* user,
+                   rrg_stats* stats, rrg_error* err);
 // The device-resident decode of rrg_gpu_decode_device into c.out (the file at
 // d_file in device memory); the out-parameters as there, minus d_out.
 int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,

@@ -244,7 +244,10 @@ uint64_t tok_slots(uint64_t span) { return ((span * 3) / 2 + 16384 + 3) & ~3ull;
 uint64_t engine_footprint(uint64_t len, uint64_t out) {
     const uint64_t n_seg = len / (24 * 1024) + 1;
     const uint64_t toks = (len * 3) / 2 + n_seg * (16384 + 4);
-    return len + 64 + 4 * toks + 2 * (out + 64 * n_seg) + out;
+    // the tail chain keeps an entry and an exit window per block of segments
+    // (about 4 blocks per CU: at most ~1024 blocks)
+    const uint64_t tail = std::min<uint64_t>(n_seg, 1024) * 2 * 65536 + 65536 * 64;
+    return len + 64 + 4 * toks + 2 * (out + 64 * n_seg) + out + tail;
 }
 
 // Member-record slots of a segment spanning `span` input bytes: a few member
@@ -512,8 +515,8 @@ constexpr uint64_t kAnyStart = 1ull << 63;
 // segment targets between them.
 void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& targets, std::vector<uint64_t>& limits,
                   std::vector<uint64_t>& scan_ends, std::vector<uint8_t>& is_chunk) {
-    const uint64_t base = J.abs_bits(J.body);
-    const uint64_t total = J.end_bits() - base;
+    const uint64_t base = J.chain_start();
+    const uint64_t total = J.last_end() - base;
     const uint64_t chunk_bits = (uint64_t)std::max<size_t>(J.chunk_size, 65536) * 8;
     uint64_t seg_bytes = J.segment_bytes;
     if (!seg_bytes) {
@@ -527,11 +530,11 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
     const uint64_t seg_bits = J.serial ? seg_bytes * 8 : chunk_bits / D;
-    if (J.stream || J.serial) {
+    if (J.stream || J.serial || J.slab) {
         // chunk starts already fixed: segment targets inside each chunk
         for (size_t k = 0; k < J.stream_B.size(); ++k) {
             const uint64_t lo = J.stream_B[k];
-            const uint64_t hi = k + 1 < J.stream_B.size() ? J.stream_B[k + 1] : J.end_bits();
+            const uint64_t hi = k + 1 < J.stream_B.size() ? J.stream_B[k + 1] : J.last_end();
             for (uint64_t t = lo + seg_bits; t < hi; t += seg_bits) {
                 targets.push_back(t);
                 limits.push_back(J.end_bits() | kAnyStart);
@@ -578,7 +581,7 @@ struct Redo {
 void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std::vector<SegResult>& res,
                  std::vector<Redo>& redo) {
     const size_t before = redo.size();
-    uint64_t expected = J.abs_bits(J.body);
+    uint64_t expected = J.chain_start();
     J.n_chain = 0;
     J.fatal = -1;
     J.saw_final = false;
@@ -612,6 +615,7 @@ void chain_round(Job& J, size_t job, const std::vector<SegDesc>& desc, const std
             expected = desc[J.order[k + 1]].start_bit;
         }
     }
+    J.chain_end = expected;
 }
 
 // Errors, CRC/ISIZE checks and chunk emission of the member-path part of a
@@ -637,7 +641,7 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
     for (size_t j = 1; j < n_chain; ++j)
         if (J.seg_is_chunk[j] && (J.orig_start[j] != desc[J.order[j]].start_bit || !J.first_ok[j]))
             ++dec.spec_failures;
-    const uint64_t body_abs = J.in_off + J.body;
+    const uint64_t body_abs = J.resume ? J.body_ref : J.in_off + J.body;
     // earliest failure, as the chunk index it stops the stream at
     uint32_t fail_chunk = UINT32_MAX;
     rrg_error ferr{};
@@ -671,43 +675,61 @@ void finish_member_path(Job& J, const std::vector<SegDesc>& desc, const std::vec
         for (uint32_t k = 0; k < res[J.order[j]].n_mb; ++k, ++mi) {
             const MemberRec& m = recs[desc[J.order[j]].mb_base + k];
             const uint64_t e = off[j] + m.out_off;
-            const uint64_t len = e - mstart;
+            uint64_t len = e - mstart;
             mstart = e;
             const uint32_t ck = chunk_of[j];
             if (ck >= fail_chunk) {
                 stop = true;
                 break;
             }
-            const uint32_t got = std_crc(raw[r_lo + mi], len);
+            uint32_t r = raw[r_lo + mi];
+            if (mi == 0 && J.resume) {
+                // the member began in an earlier slab
+                r = gf2_mul(x8n(len), J.carry_raw) ^ r;
+                len += J.carry_len;
+            }
+            const uint32_t got = std_crc(r, len);
             const uint64_t tb = m.trailer_byte - body_abs;
+            const uint64_t mnum = J.carry_member + mi;
             if (got != m.crc) {
                 fail_chunk = ck;
                 set_err(&ferr, RRG_E_CRC,
-                        "CRC32 mismatch in member #%zu (uncompressed bytes: %llu, trailer at byte %llu): expected "
+                        "CRC32 mismatch in member #%llu (uncompressed bytes: %llu, trailer at byte %llu): expected "
                         "%#010x, got %#010x",
-                        mi, (unsigned long long)len, (unsigned long long)tb, m.crc, got);
+                        (unsigned long long)mnum, (unsigned long long)len, (unsigned long long)tb, m.crc, got);
                 ferr.expected = m.crc;
                 ferr.got = got;
                 ferr.got64 = len;
-                ferr.member = (uint32_t)mi;
+                ferr.member = (uint32_t)mnum;
                 ferr.trailer_byte = tb;
                 stop = true;
                 break;
             }
             if ((uint32_t)len != m.isize) {
                 fail_chunk = ck;
-                set_err(&ferr, RRG_E_ISIZE, "ISIZE mismatch in member #%zu (trailer at byte %llu): expected %u, got %llu",
-                        mi, (unsigned long long)tb, m.isize, (unsigned long long)len);
+                set_err(&ferr, RRG_E_ISIZE, "ISIZE mismatch in member #%llu (trailer at byte %llu): expected %u, got %llu",
+                        (unsigned long long)mnum, (unsigned long long)tb, m.isize, (unsigned long long)len);
                 ferr.expected = m.isize;
                 ferr.got64 = len;
-                ferr.member = (uint32_t)mi;
+                ferr.member = (uint32_t)mnum;
                 ferr.trailer_byte = tb;
                 stop = true;
                 break;
             }
         }
     }
-    if (fail_chunk == UINT32_MAX && !J.saw_final) {
+    if (J.slab && !stop) {
+        // the member still open where the slab's chain ends
+        J.members_done = J.carry_member + mi;
+        const uint64_t tail = off[n_chain] - mstart;
+        J.open_raw = J.open_r == SIZE_MAX ? 0u : raw[J.open_r];
+        J.open_len = tail;
+        if (mi == 0 && J.resume) {
+            J.open_raw = gf2_mul(x8n(tail), J.carry_raw) ^ J.open_raw;
+            J.open_len += J.carry_len;
+        }
+    }
+    if (fail_chunk == UINT32_MAX && !J.saw_final && J.stop_bit == UINT64_MAX) {
         // the chain ran past every segment without reaching the end of the file
         fail_chunk = n_chain ? chunk_of[n_chain - 1] + 1 : 0;
         set_err(&ferr, RRG_E_IO, "parallel decode produced no final block");
@@ -881,8 +903,7 @@ int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc
 // with want_recs, their first 256 chain positions (left on the device).
 int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
                const std::vector<uint64_t>& scan_ends, std::vector<uint64_t>& found, rrg_error* err,
-               const std::vector<uint64_t>* stops = nullptr, std::vector<uint64_t>* exits = nullptr,
-               bool want_recs = false) {
+               const std::vector<uint64_t>* stops, std::vector<uint64_t>* exits, bool want_recs) {
     found.assign(targets.size(), UINT64_MAX);
     if (targets.empty()) return RRG_OK;
     const size_t n = targets.size();
@@ -1024,10 +1045,6 @@ int resolve_open(Ctx& c, std::vector<OpenTarget> open, std::vector<uint64_t>& fo
     return RRG_OK;
 }
 
-// The first walk of an exact target covers this much input; targets without
-// a find by then are continued in parallel legs (resolve_open).
-constexpr uint64_t kExactCapBits = 8ull * 131072;
-
 // Runs of a finder launch whose exact targets (limits without kAnyStart) had
 // `stops`: continue those that ended open.
 int finish_open(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<uint64_t>& limits,
@@ -1166,12 +1183,13 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
 int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out, rrg_error* err) {
     const uint64_t total_bits = (uint64_t)c.in_used * 8;
     // 1. host plan
-    for (Job& J : jobs) plan_job(J);
+    for (Job& J : jobs)
+        if (!J.resume) plan_job(J);
     // 2. block finder over every member-path body
     // streaming chunking (streaming.rs:278-346): target = previous boundary +
     // chunk_bits, first verified candidate at or after it, until none is left
     for (Job& J : jobs) {
-        if (J.body == UINT64_MAX) continue;
+        if (J.body == UINT64_MAX || J.slab) continue;
         if (J.serial) {
             // one chunk: the single-worker paths cut chunks at block ends instead
             J.stream_B.assign(1, J.abs_bits(J.body));
@@ -1215,9 +1233,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         for (size_t ji = 0; ji < jobs.size(); ++ji) {
             Job& J = jobs[ji];
             if (J.body == UINT64_MAX) continue;
-            const uint64_t base = J.abs_bits(J.body);
-            std::vector<uint64_t> B{base};
-            if (J.stream || J.serial) B = J.stream_B;
+            std::vector<uint64_t> B{J.chain_start()};
+            if (J.stream || J.serial || J.slab) B = J.stream_B;
             for (size_t i = J.t_lo; i < J.t_hi; ++i)
                 if (is_chunk[i] && found[i] != UINT64_MAX && found[i] > B.back()) B.push_back(found[i]);
             // segment finds come in target order, which is almost always their
@@ -1239,7 +1256,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             const uint64_t med = span[span.size() / 2];
             for (size_t j = 0; j < M.size(); ++j) {
                 // the last segment runs to the end of the file
-                const uint64_t nx = j + 1 < M.size() ? M[j + 1] : J.end_bits();
+                const uint64_t nx = j + 1 < M.size() ? M[j + 1] : J.last_end();
                 const uint64_t g = nx - M[j];
                 if (g > med + med / 2 && g > 8 * 16384) {
                     // a block start hidden in the gap lies about halfway: search from a quarter on
@@ -1321,13 +1338,13 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         for (size_t j = 0; j < S; ++j) {
             SegDesc d{};
             d.start_bit = M[j];
-            d.hint_bit = j + 1 < S ? M[j + 1] : UINT64_MAX;
+            d.hint_bit = j + 1 < S ? M[j + 1] : J.stop_bit;
             d.end_byte = J.in_off + J.len;
-            const uint64_t span = ((j + 1 < S ? M[j + 1] : J.end_bits()) - M[j]) / 8;
+            const uint64_t span = ((j + 1 < S ? M[j + 1] : J.last_end()) - M[j]) / 8;
             d.tok_base = tok_used;
             d.tok_cap = tok_slots(span);
             tok_used += d.tok_cap;
-            d.flags = (j ? SEG_SPEC : 0u) | (J.serial ? SEG_BLOCK_ENDS : 0u);
+            d.flags = (j || J.resume ? SEG_SPEC : 0u) | (J.serial ? SEG_BLOCK_ENDS : 0u);
             d.mb_base = (uint32_t)mb_used;
             d.mb_cap = mb_slots(span, J.serial);
             mb_used += d.mb_cap;
@@ -1400,7 +1417,24 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     auto expandable = [&](size_t j) -> uint32_t {
         return overflowed(res[j].status) ? 0u : (uint32_t)std::min<uint64_t>(res[j].n_tok, desc[j].tok_cap);
     };
+    struct Widen {
+        const uint8_t* src;
+        uint64_t cell, n;
+    };
+    std::vector<Widen> windows;
     for (Job& J : jobs) {
+        const bool win = J.resume && J.win_len && J.body != UINT64_MAX;
+        if (win) {
+            // the output before a resumed slab: a marker-free segment just
+            // before the job's output, which the tail chain carries into the
+            // slab's first segment
+            out_total = ((out_total + J.win_len + 255) & ~255ull) - J.win_len;
+            cs.push_back(ChainSeg{cells_total, J.win_len, 0, 0u, 1u});
+            cs_out.push_back(out_total);
+            windows.push_back(Widen{J.win_src, cells_total, J.win_len});
+            cells_total = (cells_total + J.win_len + 63) & ~63ull;
+            out_total += J.win_len;
+        }
         J.dec.out_base = out_total;
         J.c_lo = cs.size();
         J.off.clear();
@@ -1419,7 +1453,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             J.off.push_back(o);
             const uint64_t base = cells_total;
             cells_total = (cells_total + res[j].n_cells + 63) & ~63ull;
-            cs.push_back(ChainSeg{base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u, k == 0 ? 1u : 0u});
+            cs.push_back(ChainSeg{base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
+                                  k == 0 && !win ? 1u : 0u});
             cs_out.push_back(out_total + o);
             if (res[j].n_cells)
                 items.push_back(ExpandItem{desc[j].tok_base, base, desc[j].start_bit >> 3, res[j].n_cells,
@@ -1434,6 +1469,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     HIPCHK(c.cells.ensure((cells_total + 64) * sizeof(uint16_t)));
     rc = run_expand(c, items, out, err);
     if (rc) return rc;
+    for (const Widen& w : windows)
+        HIPCHK(launch_widen(w.src, c.cells.as<uint16_t>() + w.cell, w.n, c.s));
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
         HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
@@ -1464,6 +1501,12 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
                 mstart = e;
             }
         }
+        J.open_r = SIZE_MAX;
+        if (J.slab && J.body != UINT64_MAX) {
+            // the output of the member still open at the end of the slab
+            J.open_r = ranges.size();
+            ranges.push_back({ob + mstart, ob + J.off[nb + J.n_out]});
+        }
     }
     std::vector<uint32_t> raw;
     rc = crc_ranges(c, out, ranges, raw, err);
@@ -1491,7 +1534,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             }
             // batches ramp 64 KiB -> chunk size (pipeline.rs:1488-1551)
             const uint64_t tmax = std::max<size_t>(J.chunk_size, 65536);
-            uint64_t target = 65536, cur = 0;
+            uint64_t target = J.bgzf_target0, cur = 0;
             size_t first = 0;
             for (size_t k = 0; k < nb; ++k) {
                 cur += J.mem[k].size;
@@ -1503,6 +1546,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
                     first = k + 1;
                 }
             }
+            J.bgzf_target_end = target;
             if (bad < nb) {
                 dec.failed = true;
                 set_err(&dec.error, RRG_E_INVALID, "invalid deflate stream: libdeflate gzip decode failed");
@@ -1568,11 +1612,13 @@ void print_profile(const Ctx& c) {
     const rrg_profile& p = c.prof;
     std::fprintf(stderr,
                  "[rrg-gpu] %llu -> %llu bytes in %.3f ms | find %.3f inflate %.3f (x%llu, %llu re-decoded, %llu "
-                 "grown) expand %.3f (%llu tokens) tail %.3f (%llu blocks) resolve %.3f crc %.3f | %llu segments\n",
+                 "grown) expand %.3f (%llu tokens) tail %.3f (%llu blocks) resolve %.3f crc %.3f | %llu segments, "
+                 "%llu slabs, first chunk at %.3f ms, %.1f MiB of device memory\n",
                  (unsigned long long)p.compressed_bytes, (unsigned long long)p.uncompressed_bytes, p.ms_total,
                  p.ms_find, p.ms_inflate, (unsigned long long)p.n_inflate_launches, (unsigned long long)p.n_redecoded,
                  (unsigned long long)p.n_grown, p.ms_expand, (unsigned long long)p.n_tokens, p.ms_tail,
-                 (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc, (unsigned long long)p.n_segments);
+                 (unsigned long long)p.n_tail_rounds, p.ms_resolve, p.ms_crc, (unsigned long long)p.n_segments,
+                 (unsigned long long)p.n_slabs, p.ms_first_chunk, (double)p.device_bytes_peak / 1048576.0);
 }
 
 int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,
@@ -1636,6 +1682,7 @@ bool init_ctx(Ctx& c, int device) {
         hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess || hipStreamCreate(&c.s2) != hipSuccess ||
+        hipStreamCreate(&c.su) != hipSuccess || hipStreamCreate(&c.sc) != hipSuccess ||
         hipEventCreateWithFlags(&c.evf[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evf[1], hipEventDisableTiming) != hipSuccess)
         return false;
@@ -1782,6 +1829,15 @@ static int decode_host(void* ctx, const uint8_t* file, size_t len, const rrg_cfg
         if (stream) set_err(err, RRG_E_TRUNCATED, "truncated gzip stream");
         else set_err(err, RRG_E_IO, "empty input");
         return err ? err->kind : (stream ? RRG_E_TRUNCATED : RRG_E_IO);
+    }
+    if (!stream && cfg.num_threads != 1) {
+        // the speculative chunking: in slabs, chunks handed out as they are ready
+        const int src = decode_slabbed(c, file, len, cfg, cb, user, stats, err);
+        if (src != kNotSlabbed) {
+            c.prof.ms_total = now_ms() - t0;
+            if (cfg.verbose) print_profile(c);
+            return src;
+        }
     }
     int rc = load_host(c, file, len, err);
     if (rc) return rc;

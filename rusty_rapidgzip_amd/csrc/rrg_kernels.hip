@@ -2350,6 +2350,19 @@ hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint1
     return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) k_widen(const uint8_t* __restrict__ src, uint16_t* __restrict__ cells,
+                                               uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        cells[i] = src[i];
+}
+
+hipError_t launch_widen(const uint8_t* d_src, uint16_t* d_cells, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_widen, dim3((uint32_t)(blocks < 1024 ? blocks : 1024)), dim3(256), 0, s, d_src, d_cells, n);
+    return hipGetLastError();
+}
+
 // The fold constants, computed once (thread-safe static initialisation).
 static CrcConsts make_crc_consts() {
     auto mul = [](uint32_t a, uint32_t b) {

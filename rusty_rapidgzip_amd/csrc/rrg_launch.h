@@ -72,6 +72,10 @@ hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const Ch
 hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint16_t* d_cells, uint32_t* d_first,
                                hipStream_t s);
 
+// cells[i] = src[i] for i < n: output bytes as marker-free cells (the
+// window a resumed decode starts from).
+hipError_t launch_widen(const uint8_t* d_src, uint16_t* d_cells, uint64_t n, hipStream_t s);
+
 // Raw (init 0, no final xor) CRC-32 of each piece [off, off+len), len <= 256 KiB.
 hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, const uint32_t* d_piece_len,
                              uint32_t n_pieces, uint32_t* d_crc, hipStream_t s);

@@ -46,6 +46,8 @@ def _lib():
         lib.rrg_members.argtypes = [u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, u8p, ctypes.c_size_t,
                                     ctypes.c_int]
         lib.rrg_members.restype = ctypes.c_size_t
+        lib.rrg_sink_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        lib.rrg_sink_copy.restype = ctypes.c_int
         _typed = True
     return lib
 
@@ -133,3 +135,22 @@ def bgzf(raw: np.ndarray, level: int = 6, threads: int | None = None) -> np.ndar
     if n == 0:
         raise RuntimeError("bgzf writer failed")
     return out[:n].copy()
+
+
+class CopySink(ctypes.Structure):
+    """State of rrg_sink_copy: a C chunk callback that copies every chunk into
+    one reused buffer of `cap` bytes and counts bytes and chunks."""
+
+    _fields_ = [("buf", ctypes.c_void_p), ("cap", ctypes.c_size_t), ("bytes", ctypes.c_uint64),
+                ("chunks", ctypes.c_uint64)]
+
+    def __init__(self, cap: int = 8 << 20):
+        super().__init__()
+        self._mem = np.empty(cap, np.uint8)
+        self.buf = self._mem.ctypes.data
+        self.cap = cap
+
+    @staticmethod
+    def callback() -> int:
+        """Address of rrg_sink_copy (an rrg_chunk_cb)."""
+        return ctypes.cast(_lib().rrg_sink_copy, ctypes.c_void_p).value

@@ -594,3 +594,27 @@ size_t rrg_gzip_segments(const uint8_t* in, size_t n, const int* seg_level, cons
                          uint8_t* out, size_t cap, int nthreads, size_t piece) {
     return gzip_common(in, n, 6, seg_level, seg_end, nseg, out, cap, nthreads, piece);
 }
+
+/*
+ * A consumer for timing the host-output decode: an rrg_chunk_cb that copies
+ * each chunk into one reused buffer (as a consumer that parses or writes the
+ * bytes would touch them) and counts chunks and bytes.
+ */
+typedef struct {
+    uint8_t* buf;
+    size_t cap;
+    uint64_t bytes, chunks;
+} RrgCopySink;
+
+int rrg_sink_copy(void* user, const uint8_t* data, size_t len) {
+    RrgCopySink* s = (RrgCopySink*)user;
+    size_t off = 0;
+    while (off < len) {
+        const size_t n = len - off < s->cap ? len - off : s->cap;
+        memcpy(s->buf, data + off, n);
+        off += n;
+    }
+    s->bytes += len;
+    s->chunks += 1;
+    return 0;
+}
