@@ -133,17 +133,23 @@ void rrg_gpu_destroy(void* ctx);
 int rrg_gpu_read_gz(void* ctx, const char* path, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user,
                     rrg_stats* stats, rrg_error* err);
 
-/* Same over a file already in host memory. */
+/* Same over a file already in host memory.  The file is decoded in slabs of
+ * whole chunks under cfg->device_budget_bytes, and each slab's chunks reach
+ * cb while the next slab decodes (cb runs on the calling thread, one chunk
+ * at a time, in order). */
 int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user,
                    rrg_stats* stats, rrg_error* err);
 
-/* read_gz over a non-seekable source (pipe, stdin, socket) read to its end:
- * chunk k starts at the first verified block at or after the start of chunk
- * k-1 plus chunk_size (the reference's streaming dispatcher), BGZF is not
+/* read_gz over a non-seekable source (pipe, stdin, socket): chunk k starts
+ * at the first verified block at or after the start of chunk k-1 plus
+ * chunk_size (the reference's streaming dispatcher), BGZF is not
  * special-cased and speculation_failures is 0, as on the reference's
- * streaming path.  rrg_gpu_read_gz takes this path for non-regular and empty
- * files.  The GPU still verifies every chunk start against its predecessor's
- * end. */
+ * streaming path.  The input is read as the decode goes, a slab of chunks at
+ * a time, and each slab's chunks reach cb before the rest of the input is
+ * read; host memory holds about one slab of input.  With num_threads == 1
+ * (the single-worker streaming path) the input is read to its end first.
+ * rrg_gpu_read_gz takes this path for non-regular and empty files.  The GPU
+ * still verifies every chunk start against its predecessor's end. */
 int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, void* user, rrg_stats* stats,
                       rrg_error* err);
 

@@ -289,6 +289,11 @@ constexpr uint64_t kExactCapBits = 8ull * 131072;  // bits an exact walk covers 
 constexpr int kNotSlabbed = -1;
 int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, rrg_chunk_cb cb, void* user,
                    rrg_stats* stats, rrg_error* err);
+// The same over a non-seekable input read as the decode goes (the streaming
+// chunking, streaming.rs:278-346).  kNotSlabbed: no deflate body after the
+// first header; `head` then holds the bytes read so far.
+int decode_stream_slabbed(Ctx& c, int fd, std::vector<uint8_t>& head, const rrg_cfg& cfg, rrg_chunk_cb cb,
+                          void* user, rrg_stats* stats, rrg_error* err);
 // The device-resident decode of rrg_gpu_decode_device into c.out (the file at
 // d_file in device memory); the out-parameters as there, minus d_out.
 int decode_device_impl(Ctx& c, const void* d_file, size_t len, const uint8_t* h_file, const rrg_cfg* cfg_in,

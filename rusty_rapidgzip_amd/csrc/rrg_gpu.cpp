@@ -1927,10 +1927,9 @@ int rrg_gpu_decode(void* ctx, const uint8_t* file, size_t len, const rrg_cfg* cf
     return decode_host(ctx, file, len, cfg_in, cb, user, stats, err, false);
 }
 
-// Everything readable from fd, for non-seekable sources.
+// Everything readable from fd, after the bytes already in buf.
 static int read_all(int fd, std::vector<uint8_t>& buf, rrg_error* err) {
-    buf.clear();
-    size_t n = 0;
+    size_t n = buf.size();
     for (;;) {
         if (buf.size() < n + (1u << 20)) buf.resize(std::max<size_t>(2 * buf.size(), n + (1u << 20)));
         const ssize_t got = read(fd, buf.data() + n, buf.size() - n);
@@ -1951,6 +1950,26 @@ int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, vo
     if (err) std::memset(err, 0, sizeof *err);
     if (stats) std::memset(stats, 0, sizeof *stats);
     std::vector<uint8_t> buf;
+    if (ctx && (!cfg || cfg->num_threads != 1)) {
+        // the streaming chunking, decoded in slabs while the input is read
+        Ctx& c = *static_cast<Ctx*>(ctx);
+        HIPCHK(hipSetDevice(c.device));
+        rrg_cfg cf;
+        if (cfg) cf = *cfg;
+        else rrg_cfg_default(&cf);
+        const double t0 = now_ms();
+        begin_profile(c, 0);
+        c.verbose = cf.verbose != 0;
+        const int src = decode_stream_slabbed(c, fd, buf, cf, cb, user, stats, err);
+        if (src != kNotSlabbed) {
+            c.prof.compressed_bytes = stats ? stats->compressed_bytes : 0;
+            c.prof.ms_total = now_ms() - t0;
+            if (cf.verbose) print_profile(c);
+            return src;
+        }
+    }
+    // the single-worker streaming path, or no deflate body to cut: the whole
+    // input, then one decode
     const int rc = read_all(fd, buf, err);
     if (rc) return rc;
     return decode_host(ctx, buf.data(), buf.size(), cfg, cb, user, stats, err, true);

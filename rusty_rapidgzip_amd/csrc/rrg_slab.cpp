@@ -36,6 +36,10 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <functional>
+#include <cerrno>
+#include <cstring>
+#include <unistd.h>
 
 #include "rrg_engine.h"
 
@@ -164,10 +168,55 @@ int load_range(Ctx& c, const uint8_t* file, uint64_t lo, uint64_t hi, rrg_error*
     return RRG_OK;
 }
 
+// The member-path chain as it crosses from one slab to the next.
+struct Carry {
+    uint64_t chain = 0;    // where the next slab's chain starts (file bits): a block start
+    uint64_t guess = 0;    // the next slab's first chunk start as found
+    uint32_t raw = 0;      // raw CRC and length of the output of the member still open
+    uint64_t len = 0;
+    uint64_t member = 0;   // members whose trailers came before
+    uint64_t win_len = 0;  // output bytes before the chain held in c.win (<= 32 KiB)
+};
+
+// Compressed bytes of a non-seekable input (pipe, socket) read as the decode
+// needs them; bytes before the current slab are dropped.
+struct PipeSource {
+    int fd = -1;
+    std::vector<uint8_t> buf;  // bytes [base, base + n) of the stream
+    uint64_t base = 0, n = 0;
+    bool eof = false;
+
+    uint64_t end() const { return base + n; }
+    const uint8_t* at(uint64_t pos) const { return buf.data() + (pos - base); }
+    // read until bytes up to `want` are held or the stream ends
+    int fill(uint64_t want, rrg_error* err) {
+        while (!eof && end() < want) {
+            if (buf.size() - n < (1u << 20)) buf.resize(std::max<size_t>(n + (1u << 20), buf.size() + buf.size() / 2));
+            const size_t ask = (size_t)std::min<uint64_t>(buf.size() - n, want - end());
+            const ssize_t got = read(fd, buf.data() + n, ask);
+            if (got < 0) {
+                if (errno == EINTR) continue;
+                set_err(err, RRG_E_IO, "read failed: %s", std::strerror(errno));
+                return RRG_E_IO;
+            }
+            if (got == 0) eof = true;
+            n += (uint64_t)got;
+        }
+        return RRG_OK;
+    }
+    void drop_before(uint64_t pos) {
+        if (pos <= base) return;
+        const uint64_t k = std::min(pos - base, n);
+        std::memmove(buf.data(), buf.data() + k, n - k);
+        n -= k;
+        base += k;
+    }
+};
+
 struct Worker {
     Ctx& c;
-    const uint8_t* file;
-    uint64_t len;
+    const uint8_t* file;  // the file in host memory (nullptr for a pipe)
+    uint64_t len;         // its length
     const rrg_cfg& cfg;
     Pipe& P;
     Geometry G;
@@ -176,6 +225,10 @@ struct Worker {
     double slack = 1.25;                 // device bytes held over the estimate (buffers grow in steps)
     uint64_t in_done = 0, out_done = 0;  // compressed and output bytes of the slabs decoded
     uint64_t biggest = 0;                // largest slab input decoded so far
+    struct Mb {
+        uint64_t off, size;
+    };
+    std::vector<Mb> mem;  // BGZF members (file offsets)
 
     double ratio() const { return in_done > (1u << 20) ? (double)out_done / (double)in_done : kRatio0; }
     double estimate(uint64_t in, double r) const { return slack * slab_bytes(in, r) + (double)fixed; }
@@ -185,11 +238,13 @@ struct Worker {
         return std::min(cap, prev ? prev * kGrowth : kFirstSlab);
     }
     // after a slab of `in` input bytes and `out` output bytes: the peak so
-    // far, and the slack the estimate needs to cover what the buffers hold
+    // far, and, under an explicit budget (the context then holds no more than
+    // the budget from earlier calls), the slack the estimate needs to cover
+    // what the buffers hold
     void measured(uint64_t in, uint64_t out) {
         const uint64_t held = device_bytes(c);
         c.prof.device_bytes_peak = std::max<uint64_t>(c.prof.device_bytes_peak, held);
-        if (in >= biggest && in) {
+        if (cfg.device_budget_bytes && in >= biggest && in) {
             biggest = in;
             const double e = slab_bytes(in, (double)out / (double)in);
             if (held > fixed) slack = std::max(slack, (double)(held - fixed) / e);
@@ -203,17 +258,144 @@ struct Worker {
         return rc;
     }
 
-    struct Mb {
-        uint64_t off, size;
-    };
-    std::vector<Mb> mem;  // BGZF members (file offsets)
+    // Input of a slab whose chain may have run into the end of its input:
+    // c.in made to hold [lo, end of the stream); sets hi and h_lo.
+    using Rest = std::function<int(uint64_t lo, uint64_t& hi, const uint8_t*& h_lo, rrg_error* err)>;
 
+    int member_slab(uint64_t s, uint64_t body, bool stream, const uint8_t* h_lo, uint64_t lo, uint64_t hi,
+                    bool at_end, const std::vector<uint64_t>& Gs, uint64_t stop, Carry& K, const Rest& rest,
+                    bool& done, rrg_error* err);
     int member_path(uint64_t body, rrg_error* err);
+    int stream_path(PipeSource& src, uint64_t body, rrg_error* err);
     int bgzf(rrg_error* err);
 };
 
+// Decode member-path slab s, whose compressed bytes [lo, hi) are in c.in (and
+// at h_lo on the host): chunk starts as found Gs (Gs[0] = K.guess, strictly
+// increasing), its chain from K.chain to the first block start at or past
+// `stop` (UINT64_MAX: the final block).  Hands the chunks to the calling
+// thread and moves K on; `done` when the stream ends here.
+int Worker::member_slab(uint64_t s, uint64_t body, bool stream, const uint8_t* h_lo, uint64_t lo, uint64_t hi,
+                        bool at_end, const std::vector<uint64_t>& Gs, uint64_t stop, Carry& K, const Rest& rest,
+                        bool& done, rrg_error* err) {
+    done = false;
+    // chunks whose start as found the chain has already passed are empty
+    // (the first block start at or past it is the chain start)
+    size_t m = 0;
+    while (m < Gs.size() && Gs[m] <= K.chain) ++m;
+    uint64_t spec_pre = 0;
+    for (size_t i = 0; i + 1 < m; ++i) spec_pre += Gs[i] != K.chain;
+    const uint64_t spec_first = Gs[m - 1] != K.chain ? 1 : 0;  // m >= 1: Gs[0] = guess <= chain
+    SlabOut so;
+    so.buf = (int)(s & 1);
+    so.ends.assign(m - 1, 0);
+    if (!P.wait_free(s)) {
+        done = true;
+        return RRG_OK;
+    }
+    if (stop <= K.chain) {
+        // every chunk of the slab is empty: nothing to decode
+        so.ends.push_back(0);
+        if (!stream) P.spec += spec_pre + spec_first;
+        P.slabs++;
+        P.push(std::move(so));
+        K.guess = stop;
+        return RRG_OK;
+    }
+    std::vector<Job> jobs(1);
+    Job& J = jobs[0];
+    auto setup = [&]() {
+        J = Job{};
+        J.h = h_lo;
+        J.len = hi - lo;
+        J.chunk_size = cfg.chunk_size_bytes;
+        J.segment_bytes = cfg.segment_bytes;
+        J.stream = stream;  // no BGZF framing on a non-seekable input
+        J.slab = true;
+        J.resume = s > 0;
+        J.stop_bit = stop == UINT64_MAX ? UINT64_MAX : stop - lo * 8;
+        J.stream_B.assign(1, K.chain - lo * 8);
+        for (size_t i = m; i < Gs.size(); ++i) J.stream_B.push_back(Gs[i] - lo * 8);
+        if (J.resume) {
+            J.body = 0;
+            J.win_len = K.win_len;
+            J.win_src = c.win.as<uint8_t>();
+            J.body_ref = body - lo;  // wraps: trailer offsets count from the first deflate body
+            J.carry_raw = K.raw;
+            J.carry_len = K.len;
+            J.carry_member = K.member;
+        }
+    };
+    setup();
+    int rc = decode_one(jobs, so.buf, err);
+    if (rc) return rc;
+    bool last = stop == UINT64_MAX || J.saw_final;
+    if (stop != UINT64_MAX && (J.saw_final || J.dec.failed) && !at_end) {
+        // the chain may have run into the end of the slab's input: decode
+        // the slab again with the rest of the stream behind it
+        if ((rc = rest(lo, hi, h_lo, err))) return rc;
+        setup();
+        if ((rc = decode_one(jobs, so.buf, err))) return rc;
+        last = J.saw_final;
+    }
+    const uint64_t chain_end = J.chain_end + lo * 8;
+    if (c.verbose)
+        std::fprintf(stderr,
+                     "[rrg-gpu] slab %llu: input bytes %llu..%llu, %zu chunks, %llu output bytes, %.1f MiB of device "
+                     "memory\n",
+                     (unsigned long long)s, (unsigned long long)(K.chain / 8), (unsigned long long)(chain_end / 8),
+                     so.ends.size() + J.dec.chunk_ends.size(), (unsigned long long)J.U,
+                     (double)device_bytes(c) / 1048576.0);
+    so.base = J.dec.out_base;
+    for (uint64_t e : J.dec.chunk_ends) so.ends.push_back(e);
+    so.failed = J.dec.failed;
+    so.error = J.dec.error;
+    so.last = last || so.failed;
+    // the streaming path counts no speculation failures (streaming.rs:389-395)
+    if (!stream) P.spec += spec_pre + J.dec.spec_failures + (J.n_chain ? spec_first : 0);
+    P.slabs++;
+    in_done += chain_end / 8 - K.chain / 8;
+    out_done += J.U;
+    measured(chain_end / 8 - K.chain / 8, J.U);
+    if (!so.last) {
+        // the window for the next slab: the last 32 KiB of output, which may
+        // reach back into this slab's own window
+        const uint64_t W = std::min<uint64_t>(kWindow, K.win_len + J.U);
+        const uint8_t* ob = (so.buf ? c.out2 : c.out).as<uint8_t>();
+        HIPCHK(c.win.ensure(kWindow));
+        if (W) HIPCHK(hipMemcpyAsync(c.win.p, ob + J.dec.out_base + J.U - W, W, hipMemcpyDeviceToDevice, c.s));
+        HIPCHK(hipStreamSynchronize(c.s));
+        K.win_len = W;
+        K.raw = J.open_raw;
+        K.len = J.open_len;
+        K.member = J.members_done;
+        K.chain = chain_end;
+        K.guess = stop;
+    }
+    done = so.last;
+    P.push(std::move(so));
+    return RRG_OK;
+}
+
+// One finder launch for exact targets `tv` (bits in c.in) over the input
+// [0, data_end); finds at or past scan_end come back as UINT64_MAX.
+int exact_finds(Ctx& c, const std::vector<uint64_t>& tv, uint64_t data_end, uint64_t scan_end,
+                std::vector<uint64_t>& fv, bool& any_late, rrg_error* err) {
+    std::vector<uint64_t> lv(tv.size(), data_end), sv(tv.size(), scan_end), stv, xv;
+    for (uint64_t t : tv) stv.push_back(t + kExactCapBits);
+    int rc = run_finder(c, tv, lv, sv, fv, err, &stv, &xv);
+    if (!rc) rc = finish_open(c, tv, lv, xv, fv, err);
+    any_late = false;
+    for (uint64_t& f : fv)
+        if (f != UINT64_MAX && f >= scan_end) {
+            f = UINT64_MAX;
+            any_late = true;
+        }
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
-// The member path: slabs of chunk targets.
+// The member path of a file: slabs of chunk targets.
 // ---------------------------------------------------------------------------
 int Worker::member_path(uint64_t body, rrg_error* err) {
     const uint64_t body_bit = body * 8, end_bit = len * 8;
@@ -237,14 +419,19 @@ int Worker::member_path(uint64_t body, rrg_error* err) {
 
     uint64_t a = 1;             // first target of the slab; b: its stop target
     uint64_t lastB = body_bit;  // the last chunk start kept (file bits)
-    uint64_t guess = body_bit;  // the slab's first chunk start as found
-    uint64_t chain = body_bit;  // where the slab's chain starts (file bits)
-    uint32_t carry_raw = 0;
-    uint64_t carry_len = 0, carry_member = 0, win_len = 0;
+    Carry K;
+    K.chain = K.guess = body_bit;
     uint64_t b = a + targets_for(next_input(0));
     uint64_t lo = 0, hi = input_end(b);
     Upload up;
     up.start(c, c.in2, file, lo, hi);
+    const Rest rest = [&](uint64_t lo_, uint64_t& hi_, const uint8_t*& h_lo, rrg_error* e) -> int {
+        int r = up.join(e);
+        if (!r) r = load_range(c, file, lo_, len, e);
+        hi_ = len;
+        h_lo = file + lo_;
+        return r;
+    };
     for (uint64_t s = 0;; ++s) {
         int rc = up.join(err);
         if (rc) return rc;
@@ -261,25 +448,20 @@ int Worker::member_path(uint64_t body, rrg_error* err) {
             const bool to_end = hi == len;
             // a candidate needs real input behind it for its verification
             const uint64_t scan_end = to_end ? data_end : data_end - std::min(data_end, kSafeBits);
-            std::vector<uint64_t> ks;
+            std::vector<uint64_t> ks, tv, fv;
             for (uint64_t k = a; k <= nT; ++k) {
                 if (k > b && target(k) >= base + scan_end) break;
-                if (!known[k]) ks.push_back(k);
+                if (!known[k]) {
+                    ks.push_back(k);
+                    tv.push_back(target(k) - base);
+                }
             }
             if (!ks.empty()) {
-                std::vector<uint64_t> tv, lv, sv, stv, fv, xv;
-                for (uint64_t k : ks) {
-                    tv.push_back(target(k) - base);
-                    lv.push_back(data_end);
-                    sv.push_back(scan_end);
-                    stv.push_back(target(k) - base + kExactCapBits);
-                }
-                rc = run_finder(c, tv, lv, sv, fv, err, &stv, &xv);
-                if (!rc) rc = finish_open(c, tv, lv, xv, fv, err);
-                if (rc) return rc;
+                bool late = false;
+                if ((rc = exact_finds(c, tv, data_end, scan_end, fv, late, err))) return rc;
                 bool grow = false;
                 for (size_t i = 0; i < ks.size(); ++i) {
-                    if (fv[i] != UINT64_MAX && fv[i] < scan_end) {
+                    if (fv[i] != UINT64_MAX) {
                         f[ks[i]] = fv[i] + base;
                         known[ks[i]] = 1;
                     } else if (to_end) {
@@ -312,7 +494,7 @@ int Worker::member_path(uint64_t body, rrg_error* err) {
         }
         const bool last_planned = b > nT;
         // 2. the slab's chunk starts as found (strictly increasing)
-        std::vector<uint64_t> Gs{guess};
+        std::vector<uint64_t> Gs{K.guess};
         for (uint64_t k = a; k < b && k <= nT; ++k)
             if (f[k] != UINT64_MAX && f[k] > lastB) {
                 Gs.push_back(f[k]);
@@ -325,115 +507,96 @@ int Worker::member_path(uint64_t body, rrg_error* err) {
         const uint64_t a_next = b + 1;
         uint64_t b_next = 0, lo_next = 0, hi_next = 0;
         if (!last_planned) {
-            const uint64_t slab_in = std::max<uint64_t>(stop / 8 - std::min(stop / 8, chain / 8), chunk_bytes);
+            const uint64_t slab_in = std::max<uint64_t>(stop / 8 - std::min(stop / 8, K.chain / 8), chunk_bytes);
             b_next = a_next + targets_for(next_input(slab_in));
             lo_next = (std::min(stop, target(a_next)) / 8) & ~7ull;
             hi_next = input_end(b_next);
             up.start(c, c.in2, file, lo_next, hi_next);
         }
-        // chunks whose start as found the chain has already passed are empty
-        // (the first block start at or past it is the chain start)
-        size_t m = 0;
-        while (m < Gs.size() && Gs[m] <= chain) ++m;
-        uint64_t spec_pre = 0;
-        for (size_t i = 0; i + 1 < m; ++i) spec_pre += Gs[i] != chain;
-        const uint64_t spec_first = Gs[m - 1] != chain ? 1 : 0;  // m >= 1: Gs[0] = guess <= chain
-        SlabOut so;
-        so.buf = (int)(s & 1);
-        so.ends.assign(m - 1, 0);
-        if (!P.wait_free(s)) {
+        // 4. decode
+        bool done = false;
+        rc = member_slab(s, body, false, file + lo, lo, hi, hi == len, Gs, stop, K, rest, done, err);
+        if (rc || done) {
             (void)up.join(nullptr);
-            return RRG_OK;
+            return rc;
         }
-        if (stop <= chain) {
-            // every chunk of the slab is empty: nothing to decode
-            so.ends.push_back(0);
-            P.spec += spec_pre + spec_first;
-            P.slabs++;
-            P.push(std::move(so));
-        } else {
-            // 4. decode the slab
-            std::vector<Job> jobs(1);
-            Job& J = jobs[0];
-            auto setup = [&]() {
-                J = Job{};
-                J.h = file + lo;
-                J.len = hi - lo;
-                J.chunk_size = cfg.chunk_size_bytes;
-                J.segment_bytes = cfg.segment_bytes;
-                J.slab = true;
-                J.resume = s > 0;
-                J.stop_bit = stop == UINT64_MAX ? UINT64_MAX : stop - lo * 8;
-                J.stream_B.assign(1, chain - lo * 8);
-                for (size_t i = m; i < Gs.size(); ++i) J.stream_B.push_back(Gs[i] - lo * 8);
-                if (J.resume) {
-                    J.body = 0;
-                    J.win_len = win_len;
-                    J.win_src = c.win.as<uint8_t>();
-                    J.body_ref = body - lo;  // wraps: trailer offsets count from the first deflate body
-                    J.carry_raw = carry_raw;
-                    J.carry_len = carry_len;
-                    J.carry_member = carry_member;
-                }
-            };
-            setup();
-            if ((rc = decode_one(jobs, so.buf, err))) {
-                (void)up.join(nullptr);
-                return rc;
-            }
-            bool last = last_planned || J.saw_final;
-            if (!last_planned && (J.saw_final || J.dec.failed) && hi < len) {
-                // the chain may have run into the end of the slab's input:
-                // decode the slab again with the rest of the file behind it
-                if ((rc = up.join(err))) return rc;
-                hi = len;
-                if ((rc = load_range(c, file, lo, hi, err))) return rc;
-                setup();
-                if ((rc = decode_one(jobs, so.buf, err))) return rc;
-                last = J.saw_final;
-            }
-            const uint64_t chain_end = J.chain_end + lo * 8;
-            if (c.verbose)
-                std::fprintf(stderr,
-                             "[rrg-gpu] slab %llu: input bytes %llu..%llu, %zu chunks, %llu output bytes, %.1f MiB "
-                             "of device memory\n",
-                             (unsigned long long)s, (unsigned long long)(chain / 8),
-                             (unsigned long long)(chain_end / 8), so.ends.size() + J.dec.chunk_ends.size(),
-                             (unsigned long long)J.U, (double)device_bytes(c) / 1048576.0);
-            so.base = J.dec.out_base;
-            for (uint64_t e : J.dec.chunk_ends) so.ends.push_back(e);
-            so.failed = J.dec.failed;
-            so.error = J.dec.error;
-            so.last = last || so.failed;
-            P.spec += spec_pre + J.dec.spec_failures + (J.n_chain ? spec_first : 0);
-            P.slabs++;
-            in_done += chain_end / 8 - chain / 8;
-            out_done += J.U;
-            measured(chain_end / 8 - chain / 8, J.U);
-            if (!so.last) {
-                // the window for the next slab: the last 32 KiB of output,
-                // which may reach back into this slab's own window
-                const uint64_t W = std::min<uint64_t>(kWindow, win_len + J.U);
-                const uint8_t* ob = (so.buf ? c.out2 : c.out).as<uint8_t>();
-                HIPCHK(c.win.ensure(kWindow));
-                if (W)
-                    HIPCHK(hipMemcpyAsync(c.win.p, ob + J.dec.out_base + J.U - W, W, hipMemcpyDeviceToDevice, c.s));
-                HIPCHK(hipStreamSynchronize(c.s));
-                win_len = W;
-                carry_raw = J.open_raw;
-                carry_len = J.open_len;
-                carry_member = J.members_done;
-                chain = chain_end;
-            }
-            const bool stop_now = so.last;
-            P.push(std::move(so));
-            if (stop_now) return up.join(err);
-        }
-        guess = stop;
         a = a_next;
         b = b_next;
         lo = lo_next;
         hi = hi_next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The member path of a non-seekable input (streaming.rs:278-346): chunk k
+// starts at the first verified block at or after chunk k-1's start plus the
+// chunk size, so the starts are found one after the other as bytes arrive.
+// ---------------------------------------------------------------------------
+int Worker::stream_path(PipeSource& src, uint64_t body, rrg_error* err) {
+    const uint64_t chunk_bytes = std::max<size_t>(cfg.chunk_size_bytes, 65536);
+    const uint64_t chunk_bits = chunk_bytes * 8;
+    const uint64_t margin = std::min<uint64_t>(4ull << 20, std::max<uint64_t>(256ull << 10, chunk_bytes / 2));
+    Carry K;
+    K.chain = K.guess = body * 8;
+    uint64_t lo = 0, prev_in = 0;
+    // c.in := [lo, hi) of the stream, reading up to `want` first
+    auto load = [&](uint64_t want, uint64_t& hi, rrg_error* e) -> int {
+        int r = src.fill(want, e);
+        if (r) return r;
+        hi = std::min(src.end(), want);
+        return load_range(c, src.at(lo), 0, hi - lo, e);
+    };
+    const Rest rest = [&](uint64_t lo_, uint64_t& hi_, const uint8_t*& h_lo, rrg_error* e) -> int {
+        int r = src.fill(UINT64_MAX, e);
+        if (!r) r = load_range(c, src.at(lo_), 0, src.end() - lo_, e);
+        hi_ = src.end();
+        h_lo = src.at(lo_);
+        return r;
+    };
+    for (uint64_t s = 0;; ++s) {
+        if (P.is_closed()) return RRG_OK;
+        const uint64_t want = next_input(prev_in);
+        uint64_t hi = 0;
+        int rc = load(K.chain / 8 + want + margin, hi, err);
+        if (rc) return rc;
+        // the chunk starts of the slab, one find after the other, until the
+        // slab holds `want` input bytes (the find that ends it is its stop)
+        std::vector<uint64_t> Gs{K.guess};
+        uint64_t stop = UINT64_MAX;
+        for (;;) {
+            const bool at_end = src.eof && hi == src.end();
+            const uint64_t t = Gs.back() + chunk_bits;
+            if (at_end && t >= hi * 8) break;  // no target left: to the final block
+            const uint64_t data_end = (hi - lo) * 8;
+            const uint64_t scan_end = at_end ? data_end : data_end - std::min(data_end, kSafeBits);
+            std::vector<uint64_t> fv;
+            bool late = false;
+            if (t < lo * 8 + scan_end) {
+                if ((rc = exact_finds(c, {t - lo * 8}, data_end, scan_end, fv, late, err))) return rc;
+            } else {
+                fv.assign(1, UINT64_MAX);
+            }
+            if (fv[0] == UINT64_MAX) {
+                if (at_end) break;  // no block start up to the end of the stream
+                // not known in this input: read more
+                if ((rc = load(std::max(hi + 4 * (hi - lo), t / 8 + margin), hi, err))) return rc;
+                continue;
+            }
+            const uint64_t fb = fv[0] + lo * 8;
+            if (fb / 8 - K.chain / 8 >= want) {
+                stop = fb;
+                break;
+            }
+            Gs.push_back(fb);
+        }
+        const bool at_end = src.eof && hi == src.end();
+        bool done = false;
+        rc = member_slab(s, body, true, src.at(lo), lo, hi, at_end, Gs, stop, K, rest, done, err);
+        if (rc || done) return rc;
+        prev_in = std::max<uint64_t>(stop / 8 - std::min(stop / 8, lo), chunk_bytes);
+        // the next slab reads from its first chunk's start on
+        lo = (stop / 8) & ~7ull;
+        src.drop_before(lo);
     }
 }
 
@@ -531,66 +694,58 @@ int Worker::bgzf(rrg_error* err) {
     }
 }
 
-}  // namespace
-
-int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, rrg_chunk_cb cb, void* user,
-                   rrg_stats* stats, rrg_error* err) {
-    const auto t0 = std::chrono::steady_clock::now();
-    uint64_t budget = cfg.device_budget_bytes;
-    if (!budget) {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        budget = std::max<uint64_t>(free_b / 2, 256ull << 20);
-    }
-    // a context holding more than an explicit budget from earlier calls
-    // gives its buffers back first
-    if (cfg.device_budget_bytes && device_bytes(c) > budget) {
-        HIPCHK(hipDeviceSynchronize());
-        for (DevBuf* d : c.bufs()) d->release();
-    }
-    // lanes (and their scratch) for the largest slab the budget allows:
-    // about one per 16 KiB of its input
-    const uint64_t lanes_before = c.max_lanes;
-    const uint64_t lanes =
-        std::max<uint64_t>(1024, max_slab_input(budget, [](uint64_t in) { return slab_bytes(in, 1.0); }) / 16384);
-    if (!c.max_lanes || lanes < c.max_lanes) c.max_lanes = lanes;
-    struct Restore {
-        Ctx& c;
-        uint64_t v;
-        ~Restore() { c.max_lanes = v; }
-    } restore{c, lanes_before};
-    Pipe P;
-    Worker W{c, file, len, cfg, P, pick_geometry(c, cfg)};
-    W.budget = budget;
-    // per-lane scratch, both input buffers' margins and the small buffers
-    W.fixed = W.G.lanes_in_flight * 1024 + 2 * (4ull << 20) + (2ull << 20);
-    // which form: BGZF batches, or the member path from the first header;
-    // a file with neither is left to the whole-file decode
-    const bool is_bgzf = bgzf_block_size(file, len) != 0;
-    long hl = 0;
-    if (is_bgzf) {
-        uint64_t pos = 0;
-        while (pos < len) {
-            const uint32_t sz = bgzf_block_size(file + pos, len - pos);
-            if (!sz || pos + sz > len) break;
-            W.mem.push_back({pos, sz});
-            pos += sz;
+// Budget, lane cap and worker of one slab decode; the context's lane cap is
+// restored when it goes.
+struct Setup {
+    Ctx& c;
+    uint64_t lanes_before;
+    uint64_t budget = 0;
+    explicit Setup(Ctx& cc) : c(cc), lanes_before(cc.max_lanes) {}
+    ~Setup() { c.max_lanes = lanes_before; }
+    int init(const rrg_cfg& cfg, rrg_error* err) {
+        budget = cfg.device_budget_bytes;
+        if (!budget) {
+            size_t free_b = 0, total_b = 0;
+            HIPCHK(hipMemGetInfo(&free_b, &total_b));
+            budget = std::max<uint64_t>(free_b / 2, 256ull << 20);
         }
-        if (W.mem.empty()) return kNotSlabbed;
-    } else {
-        hl = parse_header(file, len, nullptr);
-        if (hl < 0 || (uint64_t)hl >= len) return kNotSlabbed;
+        // a context holding more than an explicit budget from earlier calls
+        // gives its buffers back first
+        if (cfg.device_budget_bytes && device_bytes(c) > budget) {
+            HIPCHK(hipDeviceSynchronize());
+            for (DevBuf* d : c.bufs()) d->release();
+        }
+        // lanes (and their scratch) for the largest slab the budget allows:
+        // about one per 16 KiB of its input
+        const uint64_t lanes =
+            std::max<uint64_t>(1024, max_slab_input(budget, [](uint64_t in) { return slab_bytes(in, 1.0); }) / 16384);
+        if (!c.max_lanes || lanes < c.max_lanes) c.max_lanes = lanes;
+        return RRG_OK;
     }
+    void worker(Worker& W) const {
+        W.budget = budget;
+        // per-lane scratch, both input buffers' margins and the small buffers
+        W.fixed = W.G.lanes_in_flight * 1024 + 2 * (4ull << 20) + (2ull << 20);
+    }
+};
+
+// The calling thread's side of a slab decode: run `work` on a worker thread
+// and hand every chunk of every decoded slab to cb in order (pipeline.rs:
+// 921-937), through two pinned staging halves, the copy of chunk k+1 running
+// while the callback takes chunk k.  A closed sink ends the decode: Ok
+// counting the refused chunk's bytes (pipeline.rs:1028-1030), BrokenPipe for
+// BGZF (pipeline.rs:1575-1578).
+int run_pipeline(Ctx& c, Pipe& P, const std::function<int(rrg_error*)>& work, rrg_chunk_cb cb, void* user,
+                 bool is_bgzf, std::chrono::steady_clock::time_point t0, rrg_stats* stats, uint64_t* compressed,
+                 rrg_error* err) {
     std::thread worker([&] {
         rrg_error e{};
         int rc = RRG_OK;
         if (hipSetDevice(c.device) != hipSuccess) {
             rc = RRG_E_DEVICE;
             set_err(&e, RRG_E_DEVICE, "cannot select device %d", c.device);
-        } else if (is_bgzf) {
-            rc = W.bgzf(&e);
         } else {
-            rc = W.member_path((uint64_t)hl, &e);
+            rc = work(&e);
         }
         std::lock_guard<std::mutex> lk(P.mu);
         P.rc = rc;
@@ -598,9 +753,6 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
         P.done = true;
         P.cv.notify_all();
     });
-    // the calling thread: in-order emission (pipeline.rs:921-937) through two
-    // pinned staging halves, the copy of chunk k+1 running while the callback
-    // takes chunk k
     uint64_t total = 0, sent = 0;
     bool closed = false, first = true;
     int rc = RRG_OK;
@@ -622,20 +774,13 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
             prev = e;
         }
         const size_t half = (biggest + 64 + 255) & ~(size_t)255;
-        auto fail = [&](hipError_t e) {
-            rc = RRG_E_DEVICE;
-            set_err(&ferr, RRG_E_DEVICE, "HIP error %s in the chunk copies", hipGetErrorString(e));
-        };
+        hipError_t he = hipSuccess;
         if (2 * half > c.pin_cap) {
             if (c.pin) (void)hipHostFree(c.pin);
             c.pin = nullptr;
             c.pin_cap = 0;
-            const hipError_t e = hipHostMalloc((void**)&c.pin, 2 * half, hipHostMallocDefault);
-            if (e != hipSuccess) {
-                fail(e);
-                break;
-            }
-            c.pin_cap = 2 * half;
+            he = hipHostMalloc((void**)&c.pin, 2 * half, hipHostMallocDefault);
+            if (he == hipSuccess) c.pin_cap = 2 * half;
         }
         uint8_t* const buf[2] = {c.pin, c.pin + c.pin_cap / 2};
         auto start_of = [&](size_t k) -> uint64_t { return k ? so.ends[k - 1] : 0; };
@@ -646,17 +791,17 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
             if (e == hipSuccess) e = hipEventRecord(c.evc[k & 1], c.sc);
             return e;
         };
-        hipError_t he = so.ends.empty() ? hipSuccess : issue(0);
+        if (he == hipSuccess && !so.ends.empty()) he = issue(0);
         for (size_t k = 0; k < so.ends.size() && he == hipSuccess; ++k) {
             const uint64_t n = so.ends[k] - start_of(k);
             if (k + 1 < so.ends.size() && (he = issue(k + 1)) != hipSuccess) break;
             if ((he = hipEventSynchronize(c.evc[k & 1])) != hipSuccess) break;
             if (first) {
-                c.prof.ms_first_chunk = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                c.prof.ms_first_chunk =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 first = false;
             }
             if (cb && cb(user, buf[k & 1], n)) {
-                // a closed sink: Ok counting the refused chunk's bytes (pipeline.rs:1028-1030)
                 closed = true;
                 total += n;
                 break;
@@ -666,7 +811,8 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
         }
         (void)hipStreamSynchronize(c.sc);  // no copy into the staging buffer outlives the slab
         if (he != hipSuccess) {
-            fail(he);
+            rc = RRG_E_DEVICE;
+            set_err(&ferr, RRG_E_DEVICE, "HIP error %s in the chunk copies", hipGetErrorString(he));
             break;
         }
         {
@@ -697,7 +843,7 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
     c.prof.uncompressed_bytes = total;
     c.prof.n_slabs = P.slabs;
     if (stats) {
-        stats->compressed_bytes = len;
+        stats->compressed_bytes = *compressed;
         stats->uncompressed_bytes = total;
         stats->chunks_decoded = sent;
         stats->speculation_failures = P.spec;
@@ -707,7 +853,6 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
         return rc;
     }
     if (closed) {
-        // BGZF: BrokenPipe (pipeline.rs:1575-1578)
         if (is_bgzf) {
             set_err(err, RRG_E_IO, "sink closed");
             return RRG_E_IO;
@@ -719,6 +864,78 @@ int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, 
         return ferr.kind ? ferr.kind : RRG_E_INVALID;
     }
     return RRG_OK;
+}
+
+}  // namespace
+
+int decode_slabbed(Ctx& c, const uint8_t* file, size_t len, const rrg_cfg& cfg, rrg_chunk_cb cb, void* user,
+                   rrg_stats* stats, rrg_error* err) {
+    const auto t0 = std::chrono::steady_clock::now();
+    // which form: BGZF batches, or the member path from the first header;
+    // a file with neither is left to the whole-file decode
+    const bool is_bgzf = bgzf_block_size(file, len) != 0;
+    std::vector<Worker::Mb> mem;
+    long hl = 0;
+    if (is_bgzf) {
+        uint64_t pos = 0;
+        while (pos < len) {
+            const uint32_t sz = bgzf_block_size(file + pos, len - pos);
+            if (!sz || pos + sz > len) break;
+            mem.push_back({pos, sz});
+            pos += sz;
+        }
+        if (mem.empty()) return kNotSlabbed;
+    } else {
+        hl = parse_header(file, len, nullptr);
+        if (hl < 0 || (uint64_t)hl >= len) return kNotSlabbed;
+    }
+    Setup S(c);
+    int rc = S.init(cfg, err);
+    if (rc) return rc;
+    Pipe P;
+    Worker W{c, file, len, cfg, P, pick_geometry(c, cfg)};
+    S.worker(W);
+    W.mem = std::move(mem);
+    uint64_t compressed = len;
+    return run_pipeline(
+        c, P, [&](rrg_error* e) { return is_bgzf ? W.bgzf(e) : W.member_path((uint64_t)hl, e); }, cb, user, is_bgzf,
+        t0, stats, &compressed, err);
+}
+
+int decode_stream_slabbed(Ctx& c, int fd, std::vector<uint8_t>& head, const rrg_cfg& cfg, rrg_chunk_cb cb,
+                          void* user, rrg_stats* stats, rrg_error* err) {
+    const auto t0 = std::chrono::steady_clock::now();
+    PipeSource src;
+    src.fd = fd;
+    // the first header, read as far as it reaches
+    long hl = -RRG_E_TRUNCATED;
+    for (uint64_t want = 65536;; want *= 4) {
+        int rc = src.fill(want, err);
+        if (rc) return rc;
+        hl = parse_header(src.at(0), src.end(), nullptr);
+        if (hl != -RRG_E_TRUNCATED || src.eof) break;
+    }
+    if (hl < 0 || (uint64_t)hl >= src.end()) {
+        // no deflate body to cut: the caller decodes what was read and the rest
+        head.assign(src.at(0), src.at(0) + src.end());
+        return kNotSlabbed;
+    }
+    Setup S(c);
+    int rc = S.init(cfg, err);
+    if (rc) return rc;
+    Pipe P;
+    Worker W{c, nullptr, 0, cfg, P, pick_geometry(c, cfg)};
+    S.worker(W);
+    uint64_t compressed = 0;
+    rc = run_pipeline(
+        c, P,
+        [&](rrg_error* e) {
+            const int r = W.stream_path(src, (uint64_t)hl, e);
+            compressed = src.end();
+            return r;
+        },
+        cb, user, false, t0, stats, &compressed, err);
+    return rc;
 }
 
 }  // namespace host

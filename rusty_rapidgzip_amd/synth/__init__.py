@@ -48,6 +48,8 @@ def _lib():
         lib.rrg_members.restype = ctypes.c_size_t
         lib.rrg_sink_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         lib.rrg_sink_copy.restype = ctypes.c_int
+        lib.rrg_sink_count.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        lib.rrg_sink_count.restype = ctypes.c_int
         _typed = True
     return lib
 
@@ -151,6 +153,7 @@ class CopySink(ctypes.Structure):
         self.cap = cap
 
     @staticmethod
-    def callback() -> int:
-        """Address of rrg_sink_copy (an rrg_chunk_cb)."""
-        return ctypes.cast(_lib().rrg_sink_copy, ctypes.c_void_p).value
+    def callback(copy: bool = True) -> int:
+        """Address of rrg_sink_copy (an rrg_chunk_cb), or with copy=False of
+        rrg_sink_count, which only counts."""
+        return ctypes.cast(_lib().rrg_sink_copy if copy else _lib().rrg_sink_count, ctypes.c_void_p).value

@@ -618,3 +618,12 @@ int rrg_sink_copy(void* user, const uint8_t* data, size_t len) {
     s->chunks += 1;
     return 0;
 }
+
+/* The same callback without touching the bytes: the decoder's own delivery rate. */
+int rrg_sink_count(void* user, const uint8_t* data, size_t len) {
+    RrgCopySink* s = (RrgCopySink*)user;
+    (void)data;
+    s->bytes += len;
+    s->chunks += 1;
+    return 0;
+}

@@ -12,6 +12,8 @@ import zlib
 
 import numpy as np
 
+from oracle import oracle
+
 HEADER = b"\x1f\x8b\x08\x00\x00\x00\x00\x00\x00\xff"
 WINDOW = 32768
 
@@ -399,3 +401,29 @@ def synced_member_starts(pieces, level=6):
     body.append(co.flush(zlib.Z_FINISH))
     payload = b"".join(pieces)
     return b"".join(body), payload, starts
+
+
+def stream_chunk_lengths(data, chunk):
+    """Chunk lengths of the reference's streaming path (streaming.rs:278-346):
+    each chunk starts at the first verified block at or after the previous
+    start + chunk_size, and ends where the decode from its start reaches the
+    next chunk's start.  Every start must be a true one (checked here)."""
+    hl = oracle.parse_header(data)
+    body = data[hl:]
+    nbits = len(body) * 8
+    cb = max(chunk, 65536) * 8
+    B = [0]
+    while B[-1] + cb < nbits:
+        f = oracle.find_next_dynamic_block(body, B[-1] + cb, nbits)
+        if f is None:
+            break
+        B.append(f)
+    lens = []
+    for k in range(len(B)):
+        end = B[k + 1] if k + 1 < len(B) else (1 << 64) - 1
+        r = oracle.decode_chunk(body, B[k], end)
+        assert r["err"] == 0
+        if k + 1 < len(B):
+            assert r["actual_end_bit"] == B[k + 1]  # a true start: the expected lengths are exact
+        lens.append(len(r["cells"]))
+    return lens

@@ -13,6 +13,7 @@ import zlib
 import numpy as np
 import pytest
 
+import crafted
 from conftest import GOLDEN
 from oracle import oracle
 
@@ -288,32 +289,6 @@ def test_read_gz_file(decoder, tmp_path):
     assert zlib.crc32(b"".join(out)) == zlib.crc32(p)
 
 
-def _stream_expected(data, chunk):
-    """Chunk lengths of the reference's streaming path (streaming.rs:278-346):
-    each chunk starts at the first verified block at or after the previous
-    start + chunk_size, and ends where the decode from its start reaches the
-    next chunk's start.  Every start must be a true one (checked here)."""
-    hl = oracle.parse_header(data)
-    body = data[hl:]
-    nbits = len(body) * 8
-    cb = max(chunk, 65536) * 8
-    B = [0]
-    while B[-1] + cb < nbits:
-        f = oracle.find_next_dynamic_block(body, B[-1] + cb, nbits)
-        if f is None:
-            break
-        B.append(f)
-    lens = []
-    for k in range(len(B)):
-        end = B[k + 1] if k + 1 < len(B) else (1 << 64) - 1
-        r = oracle.decode_chunk(body, B[k], end)
-        assert r["err"] == 0
-        if k + 1 < len(B):
-            assert r["actual_end_bit"] == B[k + 1]  # a true start: the expected lengths are exact
-        lens.append(len(r["cells"]))
-    return lens
-
-
 def test_streaming_pipe(decoder, tmp_path):
     import threading
 
@@ -323,7 +298,7 @@ def test_streaming_pipe(decoder, tmp_path):
     raw = synth.fastq(9 << 20, seed=77)
     g = synth.gzip(raw, 6, piece=4 << 20).tobytes()
     chunk = 1 << 20
-    want = _stream_expected(g, chunk)
+    want = crafted.stream_chunk_lengths(g, chunk)
     cfg = rrg.Config(chunk_size_bytes=chunk)
 
     def feed(fd, data):

@@ -189,3 +189,104 @@ def test_budget_sink_closed(decoder):
     assert b"".join(got) == raw.tobytes()[:st.uncompressed_bytes]
     # the context is usable afterwards
     _check(decoder, g, 256 << 10, 4 << 20, 3, expect=raw.tobytes())
+
+
+def _pipe_decode(decoder, data, cfg, on_first=None, hold=None):
+    """decode_fd over a pipe fed by a thread.  With `hold`, the feeder writes
+    data[:hold], waits (at most 30 s) for the first chunk, then the rest; the
+    returned flag says whether the first chunk came before the last byte."""
+    import os
+    import threading
+
+    first = threading.Event()
+    fed_all = threading.Event()
+
+    def feed(fd):
+        with os.fdopen(fd, "wb") as f:
+            if hold is None:
+                f.write(data)
+            else:
+                f.write(data[:hold])
+                f.flush()
+                first.wait(30)
+                f.write(data[hold:])
+        fed_all.set()
+
+    early = []
+    out = []
+
+    def sink(b):
+        if not out:
+            early.append(not fed_all.is_set())
+            first.set()
+        out.append(b)
+
+    rfd, wfd = os.pipe()
+    t = threading.Thread(target=feed, args=(wfd,))
+    t.start()
+    try:
+        st = decoder.decode_fd(rfd, sink, cfg)
+    finally:
+        first.set()
+        t.join()
+        os.close(rfd)
+    return out, st, bool(early and early[0])
+
+
+@pytest.mark.parametrize("budget", [0, 4 << 20])
+def test_stream_slabs(decoder, budget):
+    """The streaming chunking over a pipe, in slabs: chunk lengths as the
+    reference's streaming path (each start found from the previous one)."""
+    import crafted
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(24 << 20, seed=41)
+    g = synth.gzip(raw, 6, piece=4 << 20).tobytes()
+    want = crafted.stream_chunk_lengths(g, 256 << 10)
+    out, st, _ = _pipe_decode(decoder, g, rrg.Config(chunk_size_bytes=256 << 10, device_budget_bytes=budget))
+    prof = decoder.profile()
+    _say("stream slabs", prof.n_slabs, "chunks", len(out), "peak MiB", prof.device_bytes_peak / 2**20)
+    assert b"".join(out) == raw.tobytes()
+    assert [len(c) for c in out] == want
+    assert st.chunks_decoded == len(want) and st.speculation_failures == 0
+    assert st.compressed_bytes == len(g)
+    if budget:
+        assert prof.n_slabs >= 8
+
+
+def test_stream_first_chunk_before_eof(decoder):
+    """A pipe that stops after its first 1.5 MB until a chunk has come out:
+    the decode hands out chunks before the input ends."""
+    import crafted
+    import rusty_rapidgzip_amd as rrg
+    from rusty_rapidgzip_amd import synth
+
+    raw = synth.fastq(16 << 20, seed=43)
+    g = synth.gzip(raw, 6, piece=4 << 20).tobytes()
+    cfg = rrg.Config(chunk_size_bytes=256 << 10, device_budget_bytes=8 << 20)
+    out, st, early = _pipe_decode(decoder, g, cfg, hold=1_500_000)
+    assert early, "the first chunk came only after the whole input"
+    assert b"".join(out) == raw.tobytes()
+    assert [len(c) for c in out] == crafted.stream_chunk_lengths(g, 256 << 10)
+
+
+def test_stream_members_and_errors(decoder):
+    """Members across stream slabs, and a CRC error in a later member: the
+    error after the same chunks as the streaming path's."""
+    import gzip
+
+    import rusty_rapidgzip_amd as rrg
+
+    parts = [_lcg(n, 500 + i) for i, n in enumerate([700_000, 1, 1_300_000, 0, 900_000, 2_100_000])]
+    g = b"".join(gzip.compress(p, 6, mtime=0) for p in parts)
+    cfg = rrg.Config(chunk_size_bytes=65536, device_budget_bytes=2 << 20)
+    out, st, _ = _pipe_decode(decoder, g, cfg)
+    assert b"".join(out) == b"".join(parts)
+    assert decoder.profile().n_slabs >= 4
+    bad = bytearray(g)
+    bad[-6] ^= 1  # the last member's CRC
+    got = []
+    with pytest.raises(rrg.GzipError) as ei:
+        got, st, _ = _pipe_decode(decoder, bytes(bad), cfg)
+    assert ei.value.kind == "CrcMismatch" and ei.value.member == len(parts) - 1
