@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define RRG_GPU_ABI_VERSION 4
+#define RRG_GPU_ABI_VERSION 5
 
 /* Error kinds; values match the variants of the reference error enums. */
 enum rrg_error_kind {
@@ -93,6 +93,11 @@ typedef struct {
                                  chunks of each slab go to the callback while the next slab
                                  decodes.  A slab holds at least one chunk, so a chunk whose
                                  output alone exceeds the budget still decodes. */
+    int launch_graph;         /* rrg_gpu_decode_batch and the server: the tail, resolve and CRC
+                                 kernels of a batch as one launch of a HIP graph built once per
+                                 context (1) or as separate launches (0, the default: on a batch
+                                 of 64 x 256 KiB files the graph launch measured 0.5 ms slower
+                                 than the seven kernel launches it replaces) */
 } rrg_cfg;
 
 /* Return nonzero to close the stream (the reference's dropped Receiver). */
@@ -119,6 +124,7 @@ typedef struct {
     double ms_first_chunk;    /* host wall clock from the call to the first chunk handed to the callback */
     uint64_t n_slabs;         /* slabs the file was decoded in (host-output decode of a file) */
     uint64_t device_bytes_peak; /* device memory the context held at the peak of the call */
+    uint64_t n_graph_launches;  /* graph launches of tail + resolve + CRC (their time is in ms_resolve) */
 } rrg_profile;
 
 int rrg_gpu_abi_version(void);

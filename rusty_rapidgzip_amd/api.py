@@ -61,6 +61,7 @@ class _Cfg(ctypes.Structure):
         ("lut_bits", ctypes.c_int),
         ("lanes_per_block", ctypes.c_int),
         ("device_budget_bytes", ctypes.c_size_t),
+        ("launch_graph", ctypes.c_int),
     ]
 
 
@@ -87,6 +88,7 @@ class Profile(ctypes.Structure):
         ("ms_first_chunk", ctypes.c_double),
         ("n_slabs", ctypes.c_uint64),
         ("device_bytes_peak", ctypes.c_uint64),
+        ("n_graph_launches", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -221,7 +223,8 @@ class Config:
     end at block ends, pipeline.rs:1211-1345 / lib.rs:244-361); the GPU decode is the same for
     any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition;
     ``device_budget_bytes`` bounds the device memory of a host-output decode (0 = half the
-    device's free memory), which then runs in slabs of whole chunks."""
+    device's free memory), which then runs in slabs of whole chunks.  ``launch_graph`` runs the
+    tail, resolve and CRC kernels of a batched or served decode as one HIP graph launch."""
 
     num_threads: int = 0
     chunk_size_bytes: int = 4 * 1024 * 1024
@@ -230,6 +233,7 @@ class Config:
     lut_bits: int = 9
     lanes_per_block: int = 16
     device_budget_bytes: int = 0
+    launch_graph: bool = False
 
     def _c(self) -> _Cfg:
         c = _Cfg()
@@ -241,6 +245,7 @@ class Config:
         c.lut_bits = int(self.lut_bits)
         c.lanes_per_block = int(self.lanes_per_block)
         c.device_budget_bytes = int(self.device_budget_bytes)
+        c.launch_graph = 1 if self.launch_graph else 0
         return c
 
 

@@ -119,6 +119,8 @@ struct Ctx {
     size_t pin_cap = 0;
     rrg_profile prof{};
     bool verbose = false;  // Config.verbose of the current call: per-round lines on stderr
+    bool graph_post = false;  // tail, resolve and CRC as one graph launch (batched forms)
+    PostGraphs post_graphs;   // those graphs, built once, their nodes set per batch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evc[2] = {nullptr, nullptr};  // staging copies of the chunk emission
     // every device buffer of the context
@@ -138,6 +140,7 @@ struct Ctx {
         for (hipEvent_t e : evf)
             if (e) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
+        post_graphs.release();
         if (su) (void)hipStreamDestroy(su);
         if (sc) (void)hipStreamDestroy(sc);
         if (s) (void)hipStreamDestroy(s);

@@ -358,14 +358,21 @@ int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* e
     return RRG_OK;
 }
 
-// Tail chain + resolve + narrow for the chain segments `cs` into `out`;
-// out_off[k] is chain segment k's first output byte.  flags[k] != 0 when a
-// marker of segment k reaches before the start of its file's output.
-int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>& out_off, DevBuf& out,
-                  std::vector<uint32_t>& flags, rrg_error* err) {
+// The kernels after the expansion, for the chain segments `cs` into `out`
+// (out_off[k]: chain segment k's first output byte) and the CRC ranges
+// [lo, hi) of `out`: tail chain, resolve + narrow, CRC pieces.  Every
+// descriptor goes up first and the error flags are cleared before the
+// kernels run; the kernels then go as one launch of a graph built once per
+// context (c.graph_post) or as separate launches, and one copy back and one
+// wait end the stage.
+// flags[k] != 0 when a marker of segment k reaches before the start of its
+// file's output; raw[i] = the raw CRC of range i.
+int resolve_and_crc(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>& out_off, DevBuf& out,
+                    const std::vector<std::pair<uint64_t, uint64_t>>& ranges, std::vector<uint32_t>& flags,
+                    std::vector<uint32_t>& raw, rrg_error* err) {
     const uint32_t n = (uint32_t)cs.size();
     flags.assign(n, 0);
-    if (n == 0) return RRG_OK;
+    // tail chain: one zone of up to 32 Ki cells per segment
     uint64_t tz = 0;
     bool any_spec = false;
     for (uint32_t j = 0; j < n; ++j) {
@@ -373,21 +380,9 @@ int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>
         tz += std::min<uint64_t>(cs[j].len, 32768);
         any_spec |= cs[j].spec != 0;
     }
-    HIPCHK(c.chain.ensure(n * sizeof(ChainSeg)));
-    HIPCHK(hipMemcpyAsync(c.chain.p, cs.data(), n * sizeof(ChainSeg), hipMemcpyHostToDevice, c.s));
-    HIPCHK(c.err.ensure((size_t)n * 4));
-    HIPCHK(hipMemsetAsync(c.err.p, 0, (size_t)n * 4, c.s));
-    if (any_spec && tz) {
-        StageTimer t(c, &c.prof.ms_tail);
-        // ~4 blocks per CU keeps phase 1 parallel while phase 2 stays short
-        const uint32_t G = std::max<uint32_t>(1, (uint32_t)(n / std::max(1, 4 * c.n_cu)));
-        const uint32_t nb = (n + G - 1) / G;
-        HIPCHK(c.tz.ensure(tail_scratch_bytes(n, G)));
-        HIPCHK(launch_tail(c.chain.as<ChainSeg>(), n, G, c.cells.as<uint16_t>(), c.tz.as<uint16_t>(),
-                           c.err.as<uint32_t>(), c.s));
-        t.stop();
-        c.prof.n_tail_rounds += nb;
-    }
+    const bool tail = n && any_spec && tz;
+    // ~4 blocks per CU keeps phase 1 parallel while phase 2 stays short
+    const uint32_t G = std::max<uint32_t>(1, (uint32_t)(n / std::max(1, 4 * c.n_cu)));
     // resolve tiles: up to 256 KiB of output each, aligned in output
     // coordinates (a tile loads its segment's 32 KiB marker window once)
     const uint64_t TILE = 256 * 1024;
@@ -406,23 +401,7 @@ int resolve_chain(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_t>
             p = q;
         }
     }
-    if (!tiles.empty()) {
-        StageTimer t(c, &c.prof.ms_resolve);
-        HIPCHK(c.tiles.ensure(tiles.size() * sizeof(ResolveTile)));
-        HIPCHK(hipMemcpyAsync(c.tiles.p, tiles.data(), tiles.size() * sizeof(ResolveTile), hipMemcpyHostToDevice,
-                              c.s));
-        HIPCHK(launch_resolve(c.tiles.as<ResolveTile>(), (uint32_t)tiles.size(), c.chain.as<ChainSeg>(),
-                              c.cells.as<uint16_t>(), out.as<uint8_t>(), c.err.as<uint32_t>(), c.s));
-        t.stop();
-    }
-    HIPCHK(hipMemcpyAsync(flags.data(), c.err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.s));
-    HIPCHK(hipStreamSynchronize(c.s));
-    return RRG_OK;
-}
-
-// CRC pieces for byte ranges [lo, hi) of `out`; returns the raw CRC of each range.
-int crc_ranges(Ctx& c, const DevBuf& out, const std::vector<std::pair<uint64_t, uint64_t>>& ranges,
-               std::vector<uint32_t>& raw, rrg_error* err) {
+    // CRC pieces of at most 256 KiB
     const uint64_t PIECE = 256 * 1024;
     std::vector<uint64_t> po;
     std::vector<uint32_t> pl;
@@ -436,19 +415,78 @@ int crc_ranges(Ctx& c, const DevBuf& out, const std::vector<std::pair<uint64_t, 
     }
     first[ranges.size()] = po.size();
     std::vector<uint32_t> pc(po.size());
+    // descriptors up, flags cleared
+    if (n) {
+        HIPCHK(c.chain.ensure(n * sizeof(ChainSeg)));
+        HIPCHK(hipMemcpyAsync(c.chain.p, cs.data(), n * sizeof(ChainSeg), hipMemcpyHostToDevice, c.s));
+        HIPCHK(c.err.ensure((size_t)n * 4));
+        HIPCHK(hipMemsetAsync(c.err.p, 0, (size_t)n * 4, c.s));
+    }
+    if (tail) HIPCHK(c.tz.ensure(tail_scratch_bytes(n, G)));
+    if (!tiles.empty()) {
+        HIPCHK(c.tiles.ensure(tiles.size() * sizeof(ResolveTile)));
+        HIPCHK(hipMemcpyAsync(c.tiles.p, tiles.data(), tiles.size() * sizeof(ResolveTile), hipMemcpyHostToDevice,
+                              c.s));
+    }
     if (!po.empty()) {
-        StageTimer t(c, &c.prof.ms_crc);
         HIPCHK(c.poff.ensure(po.size() * 8));
         HIPCHK(c.plen.ensure(pl.size() * 4));
         HIPCHK(c.pcrc.ensure(pc.size() * 4));
         HIPCHK(hipMemcpyAsync(c.poff.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, c.s));
         HIPCHK(hipMemcpyAsync(c.plen.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, c.s));
-        HIPCHK(launch_crc_pieces(out.as<uint8_t>(), c.poff.as<uint64_t>(), c.plen.as<uint32_t>(), (uint32_t)po.size(),
-                                 c.pcrc.as<uint32_t>(), c.s));
-        HIPCHK(hipMemcpyAsync(pc.data(), c.pcrc.p, pc.size() * 4, hipMemcpyDeviceToHost, c.s));
-        t.stop();
-        HIPCHK(hipStreamSynchronize(c.s));
     }
+    PostLaunch pl_;
+    pl_.tail = tail;
+    pl_.cs = c.chain.as<ChainSeg>();
+    pl_.n_seg = n;
+    pl_.G = G;
+    pl_.cells = c.cells.as<uint16_t>();
+    pl_.scratch = tail ? c.tz.as<uint16_t>() : nullptr;
+    pl_.err = c.err.as<uint32_t>();
+    pl_.tiles = c.tiles.as<ResolveTile>();
+    pl_.n_tiles = (uint32_t)tiles.size();
+    pl_.out = out.as<uint8_t>();
+    pl_.poff = c.poff.as<uint64_t>();
+    pl_.plen = c.plen.as<uint32_t>();
+    pl_.n_pieces = (uint32_t)po.size();
+    pl_.pcrc = c.pcrc.as<uint32_t>();
+    if (c.graph_post) {
+        // one launch of the graph for these kernels (built once per context)
+        if (tail || pl_.n_tiles || pl_.n_pieces) {
+            StageTimer t(c, &c.prof.ms_resolve);
+            HIPCHK(launch_post(pl_, &c.post_graphs, c.s));
+            t.stop();
+            c.prof.n_graph_launches++;
+        }
+    } else {
+        PostLaunch only = pl_;
+        if (tail) {
+            only.n_tiles = only.n_pieces = 0;
+            StageTimer t(c, &c.prof.ms_tail);
+            HIPCHK(launch_post(only, nullptr, c.s));
+            t.stop();
+        }
+        if (pl_.n_tiles) {
+            only = pl_;
+            only.tail = false;
+            only.n_pieces = 0;
+            StageTimer t(c, &c.prof.ms_resolve);
+            HIPCHK(launch_post(only, nullptr, c.s));
+            t.stop();
+        }
+        if (pl_.n_pieces) {
+            only = pl_;
+            only.tail = false;
+            only.n_tiles = 0;
+            StageTimer t(c, &c.prof.ms_crc);
+            HIPCHK(launch_post(only, nullptr, c.s));
+            t.stop();
+        }
+    }
+    if (tail) c.prof.n_tail_rounds += (n + G - 1) / G;
+    if (n) HIPCHK(hipMemcpyAsync(flags.data(), c.err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.s));
+    if (!po.empty()) HIPCHK(hipMemcpyAsync(pc.data(), c.pcrc.p, pc.size() * 4, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
     raw.assign(ranges.size(), 0);
     const uint32_t full_shift = x8n(PIECE);
     for (size_t i = 0; i < ranges.size(); ++i) {
@@ -1481,10 +1519,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<std::vector<uint64_t>> bad(jobs.size());
     rc = member_reach(c, jobs, desc, res, recs, cs, bad, err);
     if (rc) return rc;
-    std::vector<uint32_t> flags;
-    rc = resolve_chain(c, cs, cs_out, out, flags, err);
-    if (rc) return rc;
-    // 7. CRC of every member of every file (BGZF members, then member-path members)
+    // 7. tail chain, resolve and the CRC of every member of every file (BGZF
+    // members, then member-path members)
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     for (Job& J : jobs) {
         J.r_lo = ranges.size();
@@ -1508,8 +1544,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             ranges.push_back({ob + mstart, ob + J.off[nb + J.n_out]});
         }
     }
-    std::vector<uint32_t> raw;
-    rc = crc_ranges(c, out, ranges, raw, err);
+    std::vector<uint32_t> flags, raw;
+    rc = resolve_and_crc(c, cs, cs_out, out, ranges, flags, raw, err);
     if (rc) return rc;
     // 8. per-file verdicts and chunk boundaries
     for (Job& J : jobs) {
@@ -1710,6 +1746,7 @@ void rrg_cfg_default(rrg_cfg* cfg) {
     cfg->chunk_size_bytes = 4u << 20;
     cfg->lut_bits = 9;
     cfg->lanes_per_block = 16;
+    cfg->launch_graph = 0;
 }
 
 void* rrg_gpu_create(int device) {

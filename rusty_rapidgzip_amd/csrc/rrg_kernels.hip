@@ -12,6 +12,8 @@
 // GF(2) arithmetic.  No MFMA.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "rrg_device.h"
 #include "rrg_hdr.h"
 #include "rrg_launch.h"
@@ -2390,6 +2392,148 @@ hipError_t launch_crc_pieces(const uint8_t* d_out, const uint64_t* d_piece_off, 
     static const CrcConsts K = make_crc_consts();
     hipLaunchKernelGGL(k_crc_pieces, dim3(n_pieces), dim3(256), 0, s, d_out, d_piece_off, d_piece_len, d_crc, K);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The post-expansion kernels as one graph.  Kernel nodes are described by
+// their function, grid and argument pointers; the arguments are copied when
+// the node is set, so locals will do.
+// ---------------------------------------------------------------------------
+void PostGraphs::release() {
+    for (int v = 0; v < 8; ++v) {
+        if (exec[v]) (void)hipGraphExecDestroy(exec[v]);
+        if (graph[v]) (void)hipGraphDestroy(graph[v]);
+        exec[v] = nullptr;
+        graph[v] = nullptr;
+        n_nodes[v] = 0;
+    }
+}
+
+namespace {
+
+struct NodeArgs {
+    // the argument values of every node of one launch, alive while the nodes are set
+    const ChainSeg* cs;
+    uint32_t n_seg, G, nb, K, ng, one;
+    uint16_t *cells, *exit_w, *entry_w, *gexit_w, *gentry_w;
+    const uint16_t* null16;
+    uint32_t* err;
+    const ResolveTile* tiles;
+    uint8_t* out;
+    const uint64_t* poff;
+    const uint32_t* plen;
+    uint32_t* pcrc;
+    CrcConsts K_crc;
+    void* a_block[5];
+    void* a_compose[4];
+    void* a_entry1[5];
+    void* a_entry2[5];
+    void* a_patch[6];
+    void* a_resolve[5];
+    void* a_crc[5];
+};
+
+int post_nodes(const PostLaunch& p, NodeArgs& A, hipKernelNodeParams* kp) {
+    int n = 0;
+    auto node = [&](const void* f, uint32_t grid, uint32_t block, void** args) {
+        hipKernelNodeParams q{};
+        q.func = const_cast<void*>(f);
+        q.gridDim = dim3(grid);
+        q.blockDim = dim3(block);
+        q.sharedMemBytes = 0;
+        q.kernelParams = args;
+        q.extra = nullptr;
+        kp[n++] = q;
+    };
+    if (p.tail) {
+        A.cs = p.cs;
+        A.n_seg = p.n_seg;
+        A.G = p.G;
+        A.nb = (p.n_seg + p.G - 1) / p.G;
+        A.K = tail_group(A.nb);
+        A.ng = (A.nb + A.K - 1) / A.K;
+        A.one = 1;
+        A.cells = p.cells;
+        A.exit_w = p.scratch;
+        A.entry_w = A.exit_w + (size_t)A.nb * WIN;
+        A.gexit_w = A.entry_w + (size_t)A.nb * WIN;
+        A.gentry_w = A.gexit_w + (size_t)A.ng * WIN;
+        A.null16 = nullptr;
+        A.err = p.err;
+        void* ab[] = {&A.cs, &A.n_seg, &A.G, &A.cells, &A.exit_w};
+        std::memcpy(A.a_block, ab, sizeof ab);
+        void* ac[] = {&A.exit_w, &A.nb, &A.K, &A.gexit_w};
+        std::memcpy(A.a_compose, ac, sizeof ac);
+        void* ae1[] = {&A.gexit_w, &A.ng, &A.ng, &A.null16, &A.gentry_w};
+        std::memcpy(A.a_entry1, ae1, sizeof ae1);
+        void* ae2[] = {&A.exit_w, &A.nb, &A.K, &A.gentry_w, &A.entry_w};
+        std::memcpy(A.a_entry2, ae2, sizeof ae2);
+        void* ap[] = {&A.cs, &A.n_seg, &A.G, &A.entry_w, &A.cells, &A.err};
+        std::memcpy(A.a_patch, ap, sizeof ap);
+        node((const void*)k_tail_block, A.nb, 1024, A.a_block);
+        node((const void*)k_tail_compose, A.ng, 1024, A.a_compose);
+        node((const void*)k_tail_entry, 1, 1024, A.a_entry1);
+        node((const void*)k_tail_entry, A.ng, 1024, A.a_entry2);
+        node((const void*)k_tail_patch, p.n_seg, 256, A.a_patch);
+    }
+    if (p.n_tiles) {
+        A.tiles = p.tiles;
+        A.cs = p.cs;
+        A.cells = p.cells;
+        A.out = p.out;
+        A.err = p.err;
+        void* ar[] = {&A.tiles, &A.cs, &A.cells, &A.out, &A.err};
+        std::memcpy(A.a_resolve, ar, sizeof ar);
+        node((const void*)k_resolve, p.n_tiles, RESOLVE_THREADS, A.a_resolve);
+    }
+    if (p.n_pieces) {
+        static const CrcConsts Kc = make_crc_consts();
+        A.K_crc = Kc;
+        A.out = p.out;
+        A.poff = p.poff;
+        A.plen = p.plen;
+        A.pcrc = p.pcrc;
+        void* aq[] = {&A.out, &A.poff, &A.plen, &A.pcrc, &A.K_crc};
+        std::memcpy(A.a_crc, aq, sizeof aq);
+        node((const void*)k_crc_pieces, p.n_pieces, 256, A.a_crc);
+    }
+    return n;
+}
+
+}  // namespace
+
+hipError_t launch_post(const PostLaunch& p, PostGraphs* g, hipStream_t s) {
+    if (!g) {
+        hipError_t e = p.tail ? launch_tail(p.cs, p.n_seg, p.G, p.cells, p.scratch, p.err, s) : hipSuccess;
+        if (e == hipSuccess) e = launch_resolve(p.tiles, p.n_tiles, p.cs, p.cells, p.out, p.err, s);
+        if (e == hipSuccess) e = launch_crc_pieces(p.out, p.poff, p.plen, p.n_pieces, p.pcrc, s);
+        return e;
+    }
+    const int v = (p.tail ? 1 : 0) | (p.n_tiles ? 2 : 0) | (p.n_pieces ? 4 : 0);
+    if (!v) return hipSuccess;
+    NodeArgs A{};
+    hipKernelNodeParams kp[7];
+    const int n = post_nodes(p, A, kp);
+    hipError_t e = hipSuccess;
+    if (!g->exec[v]) {
+        // built once: the nodes in a chain, each after the one before
+        e = hipGraphCreate(&g->graph[v], 0);
+        for (int k = 0; k < n && e == hipSuccess; ++k)
+            e = hipGraphAddKernelNode(&g->node[v][k], g->graph[v], k ? &g->node[v][k - 1] : nullptr, k ? 1 : 0,
+                                      &kp[k]);
+        if (e == hipSuccess) e = hipGraphInstantiate(&g->exec[v], g->graph[v], nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (g->graph[v]) (void)hipGraphDestroy(g->graph[v]);
+            g->graph[v] = nullptr;
+            g->exec[v] = nullptr;
+            return e;
+        }
+        g->n_nodes[v] = n;
+    } else {
+        for (int k = 0; k < n && e == hipSuccess; ++k) e = hipGraphExecKernelNodeSetParams(g->exec[v], g->node[v][k], &kp[k]);
+        if (e != hipSuccess) return e;
+    }
+    return hipGraphLaunch(g->exec[v], s);
 }
 
 }  // namespace rrg

@@ -72,6 +72,39 @@ hipError_t launch_resolve(const ResolveTile* d_tiles, uint32_t n_tiles, const Ch
 hipError_t launch_marker_reach(const ReachScan* d_items, uint32_t n, const uint16_t* d_cells, uint32_t* d_first,
                                hipStream_t s);
 
+// The kernels after the expansion of one batch: tail chain (when `tail`),
+// resolve + narrow (n_tiles > 0), CRC pieces (n_pieces > 0), in that order on
+// one stream.
+struct PostLaunch {
+    bool tail = false;
+    const ChainSeg* cs = nullptr;
+    uint32_t n_seg = 0, G = 1;
+    uint16_t* cells = nullptr;
+    uint16_t* scratch = nullptr;
+    uint32_t* err = nullptr;
+    const ResolveTile* tiles = nullptr;
+    uint32_t n_tiles = 0;
+    uint8_t* out = nullptr;
+    const uint64_t* poff = nullptr;
+    const uint32_t* plen = nullptr;
+    uint32_t n_pieces = 0;
+    uint32_t* pcrc = nullptr;
+};
+
+// Graphs of those kernels, one per combination of the three parts, each
+// built and instantiated once; a batch sets the nodes' grids and arguments
+// in the instantiated graph and launches it.
+struct PostGraphs {
+    hipGraph_t graph[8] = {};
+    hipGraphExec_t exec[8] = {};
+    hipGraphNode_t node[8][7] = {};
+    int n_nodes[8] = {};
+    void release();
+};
+
+// With g: one graph launch; without: separate launches.
+hipError_t launch_post(const PostLaunch& p, PostGraphs* g, hipStream_t s);
+
 // cells[i] = src[i] for i < n: output bytes as marker-free cells (the
 // window a resumed decode starts from).
 hipError_t launch_widen(const uint8_t* d_src, uint16_t* d_cells, uint64_t n, hipStream_t s);

@@ -114,7 +114,11 @@ int run_batch(Ctx& c, const rrg_cfg& cfg, const std::vector<Src>& src, const std
         jobs.push_back(std::move(J));
     }
     if (!jobs.empty()) {
+        // the batched and serving forms may run tail, resolve and CRC as one graph
+        const bool graph_before = c.graph_post;
+        c.graph_post = cfg.launch_graph != 0;
         const int rc = decode_batch(c, G, jobs, out, err);
+        c.graph_post = graph_before;
         if (rc) return rc;
     }
     for (size_t k = 0; k < n; ++k) {

@@ -143,6 +143,37 @@ def test_batch_equals_alone(decoder, cases):
         _check(files[i][0], alone[i], _split(r.data, r.chunk_ends), r.stats, r.error)
 
 
+def test_batch_graph_replays(decoder, cases):
+    """Tail, resolve and CRC as one graph launch, batch after batch: the
+    instantiated graph is updated in place when the next batch launches the
+    same kernels and instantiated anew when it does not (a BGZF-only batch has
+    no tail chain), with eager single-file decodes and an eager batch in
+    between.  Every batch gives each file its result alone."""
+    import rusty_rapidgzip_amd as rrg
+
+    _, files, alone = cases
+    cfg = rrg.Config(chunk_size_bytes=CHUNK, launch_graph=True)
+    eager = rrg.Config(chunk_size_bytes=CHUNK, launch_graph=False)
+    everything = list(range(len(files)))
+    bgzf_only = [i for i, (n, _) in enumerate(files) if n == "bgzf"] * 2
+    subsets = [everything, bgzf_only, [0, 1, 2], everything[::-1], bgzf_only, [3, 6, 0]]
+    for rep in range(2):
+        for sub in subsets:
+            _say("graph batch", rep, len(sub))
+            res = decoder.decode_batch([files[i][1] for i in sub], cfg)
+            assert decoder.profile().n_graph_launches == 1
+            for i, r in zip(sub, res):
+                _check(files[i][0], alone[i], _split(r.data, r.chunk_ends), r.stats, r.error)
+            # eager work on the same context in between
+            k = sub[-1]
+            got = _alone(decoder, files[k][1], cfg)
+            assert got[0] == alone[k][0] and got[2] == alone[k][2]
+            res = decoder.decode_batch([files[i][1] for i in sub[:3]], eager)
+            assert decoder.profile().n_graph_launches == 0
+            for i, r in zip(sub[:3], res):
+                _check(files[i][0], alone[i], _split(r.data, r.chunk_ends), r.stats, r.error)
+
+
 def test_batch_device_inputs(decoder, cases):
     from rusty_rapidgzip_amd import hipmem
 
