@@ -621,9 +621,9 @@ def run_serve(args, threads):
     reqs = [pool[i] for i in rng.integers(0, len(pool), args.serve_requests)]
     cfg = rrg.Config(chunk_size_bytes=args.chunk_mib << 20)
 
-    def run(rate):
+    def run(rate, engines=args.serve_engines):
         """rate None: everything submitted at once (closed loop); else Poisson arrivals per second."""
-        with rrg.Server(0, cfg, hbm_budget_bytes=args.serve_budget_gib << 30) as srv:
+        with rrg.Server(0, cfg, hbm_budget_bytes=args.serve_budget_gib << 30, engines=engines) as srv:
             done, lat, batches, total = 0, [], set(), 0
             arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
             t0 = time.perf_counter()
@@ -647,6 +647,8 @@ def run_serve(args, threads):
     run(None)  # warmup: first allocations
     cap, lat0, nb0, el0 = run(None)
     log(f"closed loop: {cap:.2f} GB/s, {nb0} batches, {el0:.2f}s")
+    cap1, lat1, nb1, el1 = run(None, engines=1)
+    log(f"closed loop, one engine: {cap1:.2f} GB/s, {nb1} batches, {el1:.2f}s")
     u_mean = sum(u for _, u in reqs) / len(reqs)
     sweeps = []
     for frac in args.serve_loads:
@@ -661,8 +663,11 @@ def run_serve(args, threads):
                     "text/FASTQ/mixed x levels 1/6/9, inputs from host memory, outputs left in HBM",
         "pool_inputs": len(pool),
         "hbm_budget_gib": args.serve_budget_gib,
+        "engines": args.serve_engines,
     }
     result["latency_ms"] = {"closed_loop_p50": _pct(lat0, 50), "closed_loop_p99": _pct(lat0, 99)}
+    result["one_engine"] = {"GB_per_s": round(cap1, 3), "p50_ms": _pct(lat1, 50), "p99_ms": _pct(lat1, 99),
+                            "batches": nb1}
     result["poisson"] = sweeps
     return result
 
@@ -694,6 +699,7 @@ def main() -> int:
     ap.add_argument("--serve-pool", type=int, default=2, help="inputs per generator x level cell")
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-loads", type=float, nargs="*", default=[0.5, 0.9])
+    ap.add_argument("--serve-engines", type=int, default=2, help="batches the server decodes at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1"}))

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define RRG_GPU_ABI_VERSION 5
+#define RRG_GPU_ABI_VERSION 6
 
 /* Error kinds; values match the variants of the reference error enums. */
 enum rrg_error_kind {
@@ -232,6 +232,8 @@ typedef struct {
     size_t max_batch_bytes;   /* compressed bytes per batch; 0 = 512 MiB */
     int max_batch_items;      /* files per batch; 0 = 4096 */
     rrg_cfg cfg;              /* decode configuration of every request */
+    int engines;              /* batches decoded at once, each by a decode context of its own on its own
+                                 stream (0 = 2; 1 = one batch at a time) */
 } rrg_server_opts;
 
 typedef struct {

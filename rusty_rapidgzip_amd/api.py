@@ -115,6 +115,7 @@ class _ServerOpts(ctypes.Structure):
         ("max_batch_bytes", ctypes.c_size_t),
         ("max_batch_items", ctypes.c_int),
         ("cfg", _Cfg),
+        ("engines", ctypes.c_int),
     ]
 
 
@@ -596,16 +597,18 @@ class Completion:
 class Server:
     """Serving loop on one GPU (rrg_server_*): requests are queued, admitted
     while the device bytes they hold stay under ``hbm_budget_bytes``, and
-    decoded in batches formed from whatever is queued when the previous batch
-    finishes.  Outputs stay in device memory until :meth:`release`."""
+    decoded in batches formed from whatever is queued, ``engines`` batches at
+    a time (each on a decode context and stream of its own; 0 = 2).  Outputs
+    stay in device memory until :meth:`release`."""
 
     def __init__(self, device: int = 0, config: Config | None = None, hbm_budget_bytes: int = 0,
-                 max_batch_bytes: int = 0, max_batch_items: int = 0):
+                 max_batch_bytes: int = 0, max_batch_items: int = 0, engines: int = 0):
         self._lib = _lib()
         o = _ServerOpts()
         o.hbm_budget_bytes = int(hbm_budget_bytes)
         o.max_batch_bytes = int(max_batch_bytes)
         o.max_batch_items = int(max_batch_items)
+        o.engines = int(engines)
         o.cfg = (config or Config())._c()
         self._srv = self._lib.rrg_server_create(int(device), ctypes.byref(o))
         if not self._srv:

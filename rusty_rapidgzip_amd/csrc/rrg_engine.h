@@ -104,8 +104,8 @@ struct Ctx {
     int device = 0;
     hipStream_t s = nullptr;
     hipStream_t s2 = nullptr;  // a second stream (default priority) for work that overlaps s
-    hipStream_t su = nullptr;  // uploads of the next slab's input (slab decode)
-    hipStream_t sc = nullptr;  // copies of decoded chunks to the host (slab decode)
+    hipStream_t su = nullptr;  // uploads of the next slab's input (slab decode; made on first use)
+    hipStream_t sc = nullptr;  // copies of decoded chunks to the host (slab decode; made on first use)
     hipEvent_t evf[2] = {nullptr, nullptr};  // fork / join between s and s2
     int n_cu = 0;
     uint64_t max_lanes = 0;  // cap on inflate lanes in flight (0 = as many as fit)

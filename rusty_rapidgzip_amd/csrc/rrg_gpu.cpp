@@ -1718,7 +1718,6 @@ bool init_ctx(Ctx& c, int device) {
         hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess || hipStreamCreate(&c.s2) != hipSuccess ||
-        hipStreamCreate(&c.su) != hipSuccess || hipStreamCreate(&c.sc) != hipSuccess ||
         hipEventCreateWithFlags(&c.evf[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evf[1], hipEventDisableTiming) != hipSuccess)
         return false;

@@ -175,8 +175,14 @@ struct Staged {
     rrg_error err{};
 };
 
+// Decode engines of a server: each a context (stream, buffers) with a thread
+// of its own, so that one batch's kernels run while another is being planned,
+// verified or laid out on the host.
+constexpr int kEngines = 2;
+
 struct Server {
-    Ctx c;
+    Ctx c;                     // engine 0 (also the device the server was made on)
+    Ctx c1;                    // engine 1
     hipStream_t up = nullptr;  // staging stream
     rrg_cfg cfg{};
     uint64_t budget = 0, max_batch_bytes = 0;
@@ -193,7 +199,10 @@ struct Server {
     uint64_t next_id = 1, held = 0, pending = 0, batches = 0;
     bool stopping = false, failed = false;
     rrg_error fail_err{};
-    std::thread stager, decoder;
+    std::thread stager, decoder, decoder1;
+    int engines = kEngines;
+
+    Ctx& engine(int e) { return e ? c1 : c; }
 
     // admission charge of a request: the device bytes the engine holds for
     // it while it decodes (input, cell regions and output; the output size
@@ -207,10 +216,10 @@ struct Server {
         return engine_footprint(n, std::max<uint64_t>(isz, 4 * (uint64_t)n)) + kAlign + kPad;
     }
 
-    // After an allocation failure: give back every cached device buffer so
-    // that the next batch starts from free memory.
-    void drop_cached() {
-        for (DevBuf* b : c.bufs()) b->release();
+    // After an allocation failure: give back every cached device buffer of
+    // engine `ec` so that its next batch starts from free memory.
+    static void drop_cached(Ctx& ec) {
+        for (DevBuf* b : ec.bufs()) b->release();
     }
 
     void stage_loop() {
@@ -261,7 +270,8 @@ struct Server {
         }
     }
 
-    void decode_loop() {
+    void decode_loop(int eng) {
+        Ctx& c = engine(eng);
         (void)hipSetDevice(c.device);
         for (;;) {
             std::unique_ptr<Staged> st;
@@ -299,10 +309,20 @@ struct Server {
                 t_last_hip = hipSuccess;
                 rc = run_batch(c, cfg, src, st->in_off, {}, ob->buf, results, &e);
                 std::swap(c.in, st->in);
+                if (cfg.verbose) {
+                    uint64_t in_b = 0, out_b = 0;
+                    for (const Request* r : st->reqs) in_b += r->data.size();
+                    for (const FileResult& R : results) out_b += R.out_len;
+                    c.prof.compressed_bytes = in_b;
+                    c.prof.uncompressed_bytes = out_b;
+                    c.prof.ms_total = ms_since(t_start);
+                    std::fprintf(stderr, "[rrg-gpu] engine %d, batch of %zu requests:\n", eng, st->reqs.size());
+                    print_profile(c);
+                }
                 if (rc && t_last_hip == hipErrorOutOfMemory) {
                     // this batch did not fit: it fails alone, later batches go on
                     oom = true;
-                    drop_cached();
+                    drop_cached(c);
                     ob->buf.release();
                 }
             } else if (!was_failed) {
@@ -402,13 +422,15 @@ int rrg_gpu_decode_batch(void* ctx, rrg_batch_item* items, size_t n, const rrg_c
 
 void* rrg_server_create(int device, const rrg_server_opts* opts) {
     Server* S = new Server();
-    if (!init_ctx(S->c, device) || hipStreamCreate(&S->up) != hipSuccess) {
-        delete S;
-        return nullptr;
-    }
     rrg_server_opts o{};
     if (opts) o = *opts;
     else rrg_cfg_default(&o.cfg);
+    S->engines = o.engines == 1 ? 1 : kEngines;
+    if (!init_ctx(S->c, device) || (S->engines > 1 && !init_ctx(S->c1, device)) ||
+        hipStreamCreate(&S->up) != hipSuccess) {
+        delete S;
+        return nullptr;
+    }
     if (!o.cfg.chunk_size_bytes) o.cfg.chunk_size_bytes = 4u << 20;
     S->cfg = o.cfg;
     size_t free_b = 0, total_b = 0;
@@ -416,9 +438,11 @@ void* rrg_server_create(int device, const rrg_server_opts* opts) {
     S->budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : free_b / 2;
     S->max_batch_bytes = o.max_batch_bytes ? o.max_batch_bytes : (512ull << 20);
     S->max_items = o.max_batch_items > 0 ? o.max_batch_items : 4096;
-    for (int i = 0; i < 2; ++i) S->free_in.push_back(std::make_unique<Staged>());
+    // an input buffer per engine being decoded, and one being staged
+    for (int i = 0; i < S->engines + 1; ++i) S->free_in.push_back(std::make_unique<Staged>());
     S->stager = std::thread([S] { S->stage_loop(); });
-    S->decoder = std::thread([S] { S->decode_loop(); });
+    S->decoder = std::thread([S] { S->decode_loop(0); });
+    if (S->engines > 1) S->decoder1 = std::thread([S] { S->decode_loop(1); });
     return S;
 }
 
@@ -539,6 +563,7 @@ void rrg_server_destroy(void* srv) {
     }
     S->stager.join();
     S->decoder.join();
+    if (S->decoder1.joinable()) S->decoder1.join();
     (void)hipSetDevice(S->c.device);
     for (auto& kv : S->reqs) delete kv.second;
     for (OutBlock* b : S->all_out) {

@@ -703,6 +703,9 @@ struct Setup {
     explicit Setup(Ctx& cc) : c(cc), lanes_before(cc.max_lanes) {}
     ~Setup() { c.max_lanes = lanes_before; }
     int init(const rrg_cfg& cfg, rrg_error* err) {
+        // the upload and chunk-copy streams, made on the first slab decode
+        if (!c.su) HIPCHK(hipStreamCreate(&c.su));
+        if (!c.sc) HIPCHK(hipStreamCreate(&c.sc));
         budget = cfg.device_budget_bytes;
         if (!budget) {
             size_t free_b = 0, total_b = 0;

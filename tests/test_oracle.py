@@ -378,7 +378,7 @@ def test_native_library_exports():
     lib = ctypes.CDLL(_native.GPU_LIB)
     for sym in declared:
         assert hasattr(lib, sym), sym
-    assert lib.rrg_gpu_abi_version() == 5
+    assert lib.rrg_gpu_abi_version() == 6
 
 
 def test_cli_arguments():
