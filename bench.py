@@ -684,8 +684,9 @@ def main() -> int:
     ap.add_argument("--gib", type=float, default=4.0, help="uncompressed size of the workload")
     ap.add_argument("--chunk-mib", type=int, default=4)
     ap.add_argument("--segment-kib", type=int, default=0, help="0 = automatic")
-    ap.add_argument("--lut-bits", type=int, default=9)
-    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--lut-bits", type=int, default=0, help="0 = automatic (10 with --lanes 64, 9 otherwise)")
+    ap.add_argument("--lanes", type=int, default=64,
+                    help="64: one wavefront per segment (k_tokens_wave); 8/16/32: one lane per segment (k_tokens)")
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")

@@ -84,9 +84,12 @@ typedef struct {
                                  speculative path's chunking.  The GPU decode is the same. */
     int verbose;              /* Config.verbose: 0 = Off, 1 = On (stage timings on stderr) */
     /* GPU decomposition (no reference counterpart; 0 = automatic) */
-    size_t segment_bytes;     /* compressed bytes per lane-decoded segment */
-    int lut_bits;             /* literal/length LUT bits: 8, 9 (default) or 10 */
-    int lanes_per_block;      /* lanes (segments in flight) per workgroup (default 16) */
+    size_t segment_bytes;     /* compressed bytes per decoded segment */
+    int lut_bits;             /* literal/length LUT bits: 0 = automatic (10 with one wavefront per
+                                 segment, 9 with one lane per segment); 9-11 and 8-10 respectively */
+    int lanes_per_block;      /* 64 (default): one wavefront per segment, its lanes decoding each
+                                 Huffman block together; 8, 16 or 32: one lane per segment, that
+                                 many segments per workgroup */
     size_t device_budget_bytes; /* rrg_gpu_decode / rrg_gpu_read_gz on a file: device memory the
                                  decode aims to stay within (0 = half the device's free memory).
                                  The file is decoded in slabs of whole chunks sized to it, and the
@@ -161,8 +164,9 @@ int rrg_gpu_decode_fd(void* ctx, int fd, const rrg_cfg* cfg, rrg_chunk_cb cb, vo
 
 /* Autotuner (no reference counterpart; the tuning the reference leaves to
  * Config.num_threads).  Decodes the device-resident sample d_file (a complete
- * gzip file) under each combination of lanes_per_block {8, 16}, lut_bits
- * {8, 9} and segment_bytes {automatic, 40 KiB, 56 KiB}, on top of `base`
+ * gzip file) under each combination of decomposition (one lane per segment,
+ * 16 per workgroup, lut_bits {8, 9}; one wavefront per segment, lut_bits
+ * {9, 10, 11}) and segment_bytes {automatic, 40 KiB, 56 KiB}, on top of `base`
  * (chunk_size_bytes and num_threads are kept), reps timed runs after one
  * warm-up each, and returns the fastest configuration in *best and its time
  * in *best_ms.  Every configuration must produce the same chunk boundaries

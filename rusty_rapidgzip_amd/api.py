@@ -222,7 +222,9 @@ class Verbosity(enum.Enum):
 class Config:
     """lib.rs:49-100.  ``num_threads == 1`` gives the reference's single-worker chunking (chunks
     end at block ends, pipeline.rs:1211-1345 / lib.rs:244-361); the GPU decode is the same for
-    any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition;
+    any value.  ``segment_bytes``/``lut_bits``/``lanes_per_block`` tune the GPU decomposition
+    (``lanes_per_block`` 64: one wavefront per segment, the default; 8/16/32: one lane per
+    segment; ``lut_bits`` 0 picks the LUT width for either);
     ``device_budget_bytes`` bounds the device memory of a host-output decode (0 = half the
     device's free memory), which then runs in slabs of whole chunks.  ``launch_graph`` runs the
     tail, resolve and CRC kernels of a batched or served decode as one HIP graph launch."""
@@ -231,8 +233,8 @@ class Config:
     chunk_size_bytes: int = 4 * 1024 * 1024
     verbose: Verbosity = Verbosity.Off
     segment_bytes: int = 0
-    lut_bits: int = 9
-    lanes_per_block: int = 16
+    lut_bits: int = 0
+    lanes_per_block: int = 64
     device_budget_bytes: int = 0
     launch_graph: bool = False
 

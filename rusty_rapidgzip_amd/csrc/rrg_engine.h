@@ -186,7 +186,8 @@ struct Decoded {
 
 struct Geometry {
     InflateGeom g;
-    uint64_t lanes_in_flight;
+    uint64_t lanes_in_flight;  // segments in flight (the automatic segment size aims at one each)
+    uint64_t scratch_bytes;    // inflate scratch (per lane or per wave)
 };
 
 struct Job {

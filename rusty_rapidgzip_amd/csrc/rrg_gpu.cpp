@@ -214,8 +214,23 @@ double now_ms() {
 
 Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     Geometry G;
+    if (cfg.lanes_per_block <= 0 || cfg.lanes_per_block >= 64) {
+        // one wavefront per segment (the default): LUT of 9 to 11 bits
+        G.g.wave = true;
+        G.g.lut_bits = cfg.lut_bits >= 9 && cfg.lut_bits <= 11 ? cfg.lut_bits : cfg.lut_bits == 8 ? 9 : 10;
+        G.g.lanes_per_block = 64;
+        int per_cu = inflate_wave_blocks_per_cu(G.g.lut_bits);
+        if (per_cu < 1) per_cu = 1;
+        G.g.blocks = per_cu * c.n_cu;
+        // a wave's scratch is about that of 16 lanes of the lane-per-segment kernel
+        if (c.max_lanes) G.g.blocks = std::max<int>(1, std::min<int>(G.g.blocks, (int)(c.max_lanes / 16)));
+        // a wave takes segments one after the other: about four each
+        G.lanes_in_flight = (uint64_t)G.g.blocks * 4;
+        G.scratch_bytes = (uint64_t)G.g.blocks * inflate_wave_scratch_bytes();
+        return G;
+    }
     G.g.lut_bits = (cfg.lut_bits == 8 || cfg.lut_bits == 10) ? cfg.lut_bits : 9;
-    int lanes = cfg.lanes_per_block > 0 ? cfg.lanes_per_block : 16;
+    int lanes = cfg.lanes_per_block;
     if (lanes > 64) lanes = 64;
     while (lanes > 1 && (size_t)lanes * inflate_lds_per_lane(G.g.lut_bits) > 65536) lanes /= 2;
     G.g.lanes_per_block = lanes;
@@ -225,6 +240,7 @@ Geometry pick_geometry(Ctx& c, const rrg_cfg& cfg) {
     if (c.max_lanes && (uint64_t)G.g.blocks * (uint64_t)lanes > c.max_lanes)
         G.g.blocks = std::max<int>(1, (int)(c.max_lanes / (uint64_t)lanes));
     G.lanes_in_flight = (uint64_t)G.g.blocks * (uint64_t)lanes;
+    G.scratch_bytes = G.lanes_in_flight * inflate_scratch_u16_per_lane() * sizeof(uint16_t);
     return G;
 }
 
@@ -281,8 +297,7 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
                 uint64_t tok_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
-    HIPCHK(c.sorted.ensure((size_t)G.g.blocks * G.g.lanes_per_block * inflate_scratch_u16_per_lane() *
-                           sizeof(uint16_t)));
+    HIPCHK(c.sorted.ensure(G.scratch_bytes));
     // tokens of segments decoded by earlier launches stay where they are
     HIPCHK(c.toks.grow_keep((tok_used + 64) * sizeof(uint32_t), c.toks.cap, c.s));
     HIPCHK(c.mrec.grow_keep((mb_used + 1) * sizeof(MemberRec), c.mrec.cap, c.s));
@@ -298,9 +313,15 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     const double before = c.prof.ms_inflate;
     {
         StageTimer t(c, &c.prof.ms_inflate);
-        HIPCHK(launch_tokens(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(), c.mrec.as<MemberRec>(),
-                             c.work.as<uint32_t>(), (uint32_t)work.size(), c.counter.as<uint32_t>(),
-                             c.toks.as<uint32_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
+        if (G.g.wave)
+            HIPCHK(launch_tokens_wave(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(),
+                                      c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
+                                      c.counter.as<uint32_t>(), c.toks.as<uint32_t>(), c.sorted.as<uint8_t>(), G.g,
+                                      c.s));
+        else
+            HIPCHK(launch_tokens(c.in.as<uint64_t>(), c.segs.as<SegDesc>(), c.res.as<SegResult>(),
+                                 c.mrec.as<MemberRec>(), c.work.as<uint32_t>(), (uint32_t)work.size(),
+                                 c.counter.as<uint32_t>(), c.toks.as<uint32_t>(), c.sorted.as<uint16_t>(), G.g, c.s));
         t.stop();
     }
     c.prof.n_inflate_launches++;
@@ -1743,8 +1764,8 @@ void rrg_gpu_debug_fail_alloc(int n) { g_fail_alloc.store(n > 0 ? n : 0); }
 void rrg_cfg_default(rrg_cfg* cfg) {
     std::memset(cfg, 0, sizeof *cfg);
     cfg->chunk_size_bytes = 4u << 20;
-    cfg->lut_bits = 9;
-    cfg->lanes_per_block = 16;
+    cfg->lut_bits = 0;
+    cfg->lanes_per_block = 64;
     cfg->launch_graph = 0;
 }
 
@@ -1795,50 +1816,49 @@ int rrg_gpu_autotune(void* ctx, const void* d_file, size_t len, const uint8_t* h
     if (base) b = *base;
     else rrg_cfg_default(&b);
     if (reps < 1) reps = 1;
-    // lanes per workgroup x literal LUT width x segment size (0 = automatic)
-    static const int kLanes[] = {8, 16};
-    static const int kLut[] = {8, 9};
+    // decomposition (lanes per workgroup, literal LUT width) x segment size (0 = automatic)
+    static const int kGeom[][2] = {{16, 8}, {16, 9}, {64, 9}, {64, 10}, {64, 11}};
     static const size_t kSeg[] = {0, 40960, 57344};
     std::vector<uint64_t> ref_ends, ends(1u << 20);
     double best_t = 1e300;
     bool first = true;
-    for (int lanes : kLanes)
-        for (int lut : kLut)
-            for (size_t seg : kSeg) {
-                rrg_cfg cfg = b;
-                cfg.lanes_per_block = lanes;
-                cfg.lut_bits = lut;
-                cfg.segment_bytes = seg;
-                cfg.verbose = 0;
-                double t = 1e300;
-                for (int r = 0; r <= reps; ++r) {
-                    const uint8_t* d_out = nullptr;
-                    uint64_t out_len = 0;
-                    size_t n = 0;
-                    const double t0 = now_ms();
-                    const int rc = rrg_gpu_decode_device(ctx, d_file, len, h_file, &cfg, &d_out, &out_len, ends.data(),
-                                                         ends.size(), &n, nullptr, err);
-                    const double dt = now_ms() - t0;
-                    if (rc) return rc;  // the sample must decode
-                    if (n > ends.size()) {
-                        set_err(err, RRG_E_ARG, "autotune: more than %zu chunks", ends.size());
-                        return RRG_E_ARG;
-                    }
-                    const std::vector<uint64_t> got(ends.begin(), ends.begin() + (long)n);
-                    if (first) {
-                        ref_ends = got;
-                        first = false;
-                    } else if (got != ref_ends) {
-                        set_err(err, RRG_E_DEVICE, "autotune: chunk boundaries differ between configurations");
-                        return RRG_E_DEVICE;
-                    }
-                    if (r) t = std::min(t, dt);  // the first run of each configuration warms up
+    for (const auto& geo : kGeom)
+        for (size_t seg : kSeg) {
+            const int lanes = geo[0], lut = geo[1];
+            rrg_cfg cfg = b;
+            cfg.lanes_per_block = lanes;
+            cfg.lut_bits = lut;
+            cfg.segment_bytes = seg;
+            cfg.verbose = 0;
+            double t = 1e300;
+            for (int r = 0; r <= reps; ++r) {
+                const uint8_t* d_out = nullptr;
+                uint64_t out_len = 0;
+                size_t n = 0;
+                const double t0 = now_ms();
+                const int rc = rrg_gpu_decode_device(ctx, d_file, len, h_file, &cfg, &d_out, &out_len, ends.data(),
+                                                     ends.size(), &n, nullptr, err);
+                const double dt = now_ms() - t0;
+                if (rc) return rc;  // the sample must decode
+                if (n > ends.size()) {
+                    set_err(err, RRG_E_ARG, "autotune: more than %zu chunks", ends.size());
+                    return RRG_E_ARG;
                 }
-                if (t < best_t) {
-                    best_t = t;
-                    *best = cfg;
+                const std::vector<uint64_t> got(ends.begin(), ends.begin() + (long)n);
+                if (first) {
+                    ref_ends = got;
+                    first = false;
+                } else if (got != ref_ends) {
+                    set_err(err, RRG_E_DEVICE, "autotune: chunk boundaries differ between configurations");
+                    return RRG_E_DEVICE;
                 }
+                if (r) t = std::min(t, dt);  // the first run of each configuration warms up
             }
+            if (t < best_t) {
+                best_t = t;
+                *best = cfg;
+            }
+        }
     best->verbose = b.verbose;
     if (best_ms) *best_ms = best_t;
     return RRG_OK;

@@ -1150,7 +1150,7 @@ __device__ int parse_gz_header(const uint64_t* __restrict__ in64, uint64_t len, 
     return ST_OK;
 }
 
-enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_BLOCK_END, S_EXIT };
+enum : int { S_FETCH = 0, S_BLOCK_START, S_SYM, S_BLOCK_END, S_EXIT, S_FINISH };
 
 // Largest cell count a segment may reach (its cell positions are 32-bit).
 constexpr uint32_t kMaxSegCells = 0xF0000000u;
@@ -1618,6 +1618,920 @@ hipError_t launch_tokens(const uint64_t* d_in64, const SegDesc* d_segs, SegResul
         default:
             hipLaunchKernelGGL(k_tokens<9>, dim3(g.blocks), dim3(g.lanes_per_block), shm, s, d_in64, d_segs, d_res,
                                d_mrec, d_work, n_work, d_counter, d_toks, d_sorted);
+            break;
+    }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Inflate pass 1, wave-cooperative form: one wavefront per segment, one set
+// of Huffman tables per wavefront in LDS, and the 64 lanes decoding one block
+// body together.
+//
+// Lane 0 runs everything that is serial by nature -- block headers, table
+// builds, stored blocks, member trailers and headers, the decode near the end
+// of the input -- with the same state machine as k_tokens.  A Huffman block
+// body is decoded in rounds.  A round from a known symbol start P gives lane k
+// the window [P + kW, P + (k+1)W) of W bits (a multiple of B = 128):
+//
+//  1. Every lane decodes from the start of its window as if a symbol began
+//     there, to the first symbol start at or past the window's end, counting
+//     tokens and cells.  Lane 0 starts at a true symbol start; the others
+//     usually start inside a code, but Huffman codes resynchronise: after a
+//     few symbols a walk from a wrong start lands on a true symbol start and
+//     from there follows the true walk (on FASTQ at gzip -6, median 7 units,
+//     99% within 50).  At each crossing of a multiple of B (from P) a lane
+//     records the first symbol start at or past it (LDS) and its token and
+//     cell counts there (global scratch).
+//  2. A lane that reached its window end walks on into the next windows, and
+//     at each crossing compares its symbol start with the one recorded by the
+//     lane owning that window.  Two walks that share one symbol start agree
+//     from there on, so the first equal crossing is where the later lane's walk
+//     has become the true one: the earlier lane ends there and the later one
+//     starts there (a later lane that never synchronises is skipped).
+//  3. Following these links from lane 0 gives the lanes whose walks are true,
+//     each from its start crossing to the next one's; a wave prefix sum over
+//     their token and cell counts places their output.
+//  4. Those lanes decode their ranges again and store their tokens at their
+//     final positions.
+// Literal tokens (up to three literals) are closed at every crossing, so a
+// range's token count is a difference of the counts recorded in step 1.  The
+// walks of step 1 skip the end-of-input checks: a round only runs where every
+// read stays 64 bits inside the input.  A lane on the true walk that meets an
+// invalid code, or a back-reference before its member, hands the rest of the
+// block back to lane 0's serial decode from the start of that lane's range,
+// which stops at the failing symbol exactly as the lane-per-segment decode
+// does.  The token stream is the one k_tokens writes except for where
+// literal tokens close, so the expansion pass and the host are unchanged.
+// ===========================================================================
+constexpr uint32_t kPB = 128;                // crossing spacing (bits)
+constexpr uint32_t kPWMax = 4096;            // window per lane (bits), at most
+constexpr uint32_t kPWMin = 2 * kPB;         // smaller windows: lane 0 decodes alone
+constexpr uint32_t kPNM = kPWMax / kPB;      // crossings per window
+constexpr int WDLB = 8;                      // distance LUT bits
+constexpr size_t kWaveScratchBytes = 1024 + 64 * (kPNM + 1) * sizeof(uint2);
+
+size_t inflate_wave_scratch_bytes() { return kWaveScratchBytes; }
+
+// lane 0's decode state, kept in LDS while the wave runs (so it holds no
+// registers there)
+struct SerialState {
+    uint64_t pos, hint, seg_byte, nbits, tk_base, body_start;
+    uint32_t state, seg, tok_cap, n_tok, out_n, member_base, spec, mflags, bfinal, n_mb, mb_base, mb_cap;
+    uint32_t fixed_seq, counting, force_serial, parW, status, detail, final_block, pad;
+};
+
+struct WaveHand {
+    uint64_t pos;      // lane 0 -> wave: body start; wave -> lane 0: where lane 0 goes on
+    uint64_t nbits;
+    uint64_t tk_base;  // the segment's token region
+    uint32_t cmd;      // 0: no segment left, 1: decode a block body
+    uint32_t next;     // wave -> lane 0: S_BLOCK_END (body done), S_SYM (decode on serially), S_EXIT (too big)
+    uint32_t n_tok, out_n, tok_cap, member_base, spec, counting, fixed_seq, W;
+};
+
+template <int LB>
+struct WaveLds {
+    uint16_t lit[1 << LB];   // (symbol << 4) | length
+    uint16_t dist[1 << WDLB];  // the same; the 7-bit precode LUT while a header is read
+    uint16_t cntL[16], cntD[16], tnext[16], toffs[16];
+    uint16_t longL[kLongLit], longD[kLongDist];
+    uint32_t lhead[2];
+    uint16_t X[64][kPNM + 1];  // per lane and crossing: first symbol start at or past it (from the window start)
+    WaveHand h;
+    SerialState ss;
+};
+
+template <int LB>
+int wave_lds_bytes() {
+    return (int)sizeof(WaveLds<LB>);
+}
+
+enum : uint32_t { WK_WALK = 0, WK_EOB = 1, WK_ERR = 2 };
+enum : uint32_t { WT_END = 64, WT_EOB = 65, WT_ERR = 66 };
+
+template <int LB>
+// 4 waves per SIMD (<= 128 VGPRs): the rounds need about 64; lane 0's header
+// code, which needs more, spills instead
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_tokens_wave(const uint64_t* __restrict__ in64, const SegDesc* __restrict__ segs,
+                                                    SegResult* __restrict__ res, MemberRec* __restrict__ mrec,
+                                                    const uint32_t* __restrict__ work, uint32_t n_work,
+                                                    uint32_t* __restrict__ counter, uint32_t* __restrict__ toks,
+                                                    uint8_t* __restrict__ wscratch) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    WaveLds<LB>& L = *reinterpret_cast<WaveLds<LB>*>(lds_raw);
+    constexpr uint32_t LMASK = (1u << LB) - 1;
+    constexpr uint32_t DMASK = (1u << WDLB) - 1;
+    const uint32_t lane = threadIdx.x;
+    uint8_t* my_scr = wscratch + (size_t)blockIdx.x * kWaveScratchBytes;
+    uint16_t* sortedL = reinterpret_cast<uint16_t*>(my_scr);
+    uint16_t* sortedD = sortedL + 288;
+    uint8_t* lens = reinterpret_cast<uint8_t*>(sortedL + 320);
+    uint8_t* cltmp = reinterpret_cast<uint8_t*>(sortedL + 480);
+    uint2* recs = reinterpret_cast<uint2*>(my_scr + 1024);
+
+    if (lane == 0) {
+        L.ss = SerialState{};
+        L.ss.state = S_FETCH;
+        L.ss.parW = kPWMax;
+    }
+
+#define WS_REFILL()                       \
+    do {                                  \
+        bitbuf = bits64_at(in64, pos);    \
+        asm volatile("" : "+v"(bitbuf));  \
+        bitcnt = 64;                      \
+    } while (0)
+#define WS_REFILL_W()                                                               \
+    do {                                                                            \
+        const uint64_t _q = pos >> 6;                                               \
+        if (_q != ci) {                                                             \
+            uint32_t _st = 3;                                                       \
+            if (_q > ci && _q - ci < 3) _st = (uint32_t)(_q - ci);                   \
+            ci = _q;                                                                \
+            for (uint32_t _k = _st; _k > 0; --_k) {                                 \
+                C0 = C1;                                                            \
+                C1 = C2;                                                            \
+                C2 = in64[ci + 3 - _k];                                             \
+            }                                                                       \
+        }                                                                           \
+        bitbuf = funnel_lo(C0, C1, (uint32_t)(pos & 63));                           \
+        bitcnt = 64;                                                                \
+    } while (0)
+#define WS_CONSUME(n)            \
+    do {                         \
+        const uint32_t _n = (n); \
+        bitbuf >>= _n;           \
+        bitcnt -= _n;            \
+        pos += _n;               \
+    } while (0)
+#define WS_FAIL(st, det)    \
+    do {                    \
+        status = (st);      \
+        detail = (det);     \
+        state = S_FETCH;    \
+        goto finish;        \
+    } while (0)
+#define WS_GETBITS(dst, n)                                          \
+    do {                                                            \
+        const uint32_t _k = (n);                                    \
+        if (pos + _k > nbits) WS_FAIL(ST_EOF, DET_NONE);            \
+        if (bitcnt < _k) WS_REFILL();                               \
+        (dst) = _k ? (uint32_t)(bitbuf & ((1ull << _k) - 1)) : 0u; \
+        WS_CONSUME(_k);                                             \
+    } while (0)
+#define WS_EMIT(t)                                \
+    do {                                          \
+        if (n_tok >= tok_cap) counting = 1;       \
+        if (!counting) tk[n_tok] = (t);           \
+        ++n_tok;                                  \
+    } while (0)
+#define WS_FLUSH_LITS()                                          \
+    do {                                                         \
+        if (lit_n) {                                             \
+            WS_EMIT((TOK_LIT << 30) | (lit_n << 24) | lit_acc);  \
+            lit_n = 0;                                           \
+            lit_acc = 0;                                         \
+        }                                                        \
+    } while (0)
+// one symbol of lane 0's serial decode (the lane-per-segment kernel's symbol step)
+#define WS_SYMBOL(CHK)                                                                           \
+    {                                                                                            \
+        uint32_t e = L.lit[bitbuf & LMASK];                                                      \
+        uint32_t l = e & 15u, sym = e >> 4;                                                      \
+        if (l == 0) {                                                                            \
+            if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
+            if (!decode_long<LB>(bitbuf, L.cntL, sortedL, L.longL, kLongLit, L.lhead[0], fixed_seq, &sym, &l)) \
+                WS_FAIL(ST_INVALID, DET_NO_CODE);                                                \
+        }                                                                                        \
+        if ((CHK) && pos + l > nbits) WS_FAIL(ST_EOF, DET_NONE);                                 \
+        WS_CONSUME(l);                                                                           \
+        if (sym < 256) {                                                                         \
+            lit_acc |= sym << (8 * lit_n);                                                       \
+            if (++lit_n == 3) {                                                                  \
+                WS_EMIT((TOK_LIT << 30) | (3u << 24) | lit_acc);                                 \
+                lit_n = 0;                                                                       \
+                lit_acc = 0;                                                                     \
+            }                                                                                    \
+            ++out_n;                                                                             \
+            continue;                                                                            \
+        }                                                                                        \
+        if (sym == 256) {                                                                        \
+            state = S_BLOCK_END;                                                                 \
+            break;                                                                               \
+        }                                                                                        \
+        if (sym > 285) WS_FAIL(ST_INVALID, DET_LITLEN_RANGE);                                    \
+        const uint32_t li = sym - 257;                                                           \
+        const uint32_t lx = len_extra(li);                                                       \
+        if ((CHK) && pos + lx > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
+        const uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));          \
+        WS_CONSUME(lx);                                                                          \
+        uint32_t de = L.dist[bitbuf & DMASK];                                                    \
+        uint32_t dl = de & 15u, dsym = de >> 4;                                                  \
+        if (dl == 0) {                                                                           \
+            if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
+            if (!decode_long<WDLB>(bitbuf, L.cntD, sortedD, L.longD, kLongDist, L.lhead[1], 0u, &dsym, &dl)) \
+                WS_FAIL(ST_INVALID, DET_NO_CODE);                                                \
+        }                                                                                        \
+        if ((CHK) && pos + dl > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
+        WS_CONSUME(dl);                                                                          \
+        if (dsym >= 30) WS_FAIL(ST_INVALID, DET_DIST_RANGE);                                     \
+        const uint32_t dx = dist_extra(dsym);                                                    \
+        if ((CHK) && pos + dx > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
+        const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));     \
+        WS_CONSUME(dx);                                                                          \
+        if (distance > out_n - member_base && !spec) WS_FAIL(ST_INVALID, DET_DIST_FAR);          \
+        WS_FLUSH_LITS();                                                                         \
+        WS_EMIT((TOK_MATCH << 30) | ((length - 3) << 16) | (distance - 1));                      \
+        out_n += length;                                                                         \
+    }
+
+    while (true) {
+        // ================= lane 0: the serial decode =================
+        if (lane == 0) {
+            SerialState& S = L.ss;
+            int state = (int)S.state;
+            uint32_t seg = S.seg;
+            uint64_t pos = S.pos, hint = S.hint, seg_byte = S.seg_byte, nbits = S.nbits, tk_base = S.tk_base;
+            uint64_t body_start = S.body_start;
+            uint32_t tok_cap = S.tok_cap, n_tok = S.n_tok, out_n = S.out_n, member_base = S.member_base;
+            uint32_t spec = S.spec, mflags = S.mflags, bfinal = S.bfinal, n_mb = S.n_mb, mb_base = S.mb_base;
+            uint32_t mb_cap = S.mb_cap, fixed_seq = S.fixed_seq, counting = S.counting;
+            uint32_t force_serial = S.force_serial, parW = S.parW;
+            uint32_t status = S.status, detail = S.detail, final_block = S.final_block;
+            uint32_t* tk = toks + tk_base;
+            uint32_t lit_acc = 0, lit_n = 0;
+            uint64_t bitbuf = 0, ci = ~0ull, C0 = 0, C1 = 0, C2 = 0;
+            uint32_t bitcnt = 0;
+            WS_REFILL();
+            uint32_t cmd = 0;
+            while (true) {
+                if (state == S_FINISH) {
+                    // the wave stopped the segment (status set)
+                    state = S_FETCH;
+                    goto finish;
+                }
+                if (state == S_FETCH) {
+                    const uint32_t w = atomicAdd(counter, 1u);
+                    if (w >= n_work) {
+                        state = S_EXIT;
+                    } else {
+                        seg = work[w];
+                        const SegDesc d = segs[seg];
+                        pos = d.start_bit;
+                        seg_byte = d.start_bit >> 3;
+                        hint = d.hint_bit;
+                        counting = 0;
+                        nbits = d.end_byte * 8;
+                        tk_base = d.tok_base;
+                        tk = toks + d.tok_base;
+                        tok_cap = (uint32_t)d.tok_cap;
+                        ci = ~0ull;
+                        n_tok = 0;
+                        lit_n = 0;
+                        lit_acc = 0;
+                        out_n = 0;
+                        member_base = 0;
+                        spec = d.flags & SEG_SPEC;
+                        mflags = d.flags & (SEG_SINGLE_MEMBER | SEG_BLOCK_ENDS);
+                        mb_base = d.mb_base;
+                        mb_cap = d.mb_cap;
+                        n_mb = 0;
+                        status = ST_OK;
+                        detail = DET_NONE;
+                        final_block = 0;
+                        bfinal = 0;
+                        parW = kPWMax;
+                        if (pos > nbits) {
+                            status = ST_EOF;
+                            goto finish;
+                        }
+                        WS_REFILL();
+                        state = S_BLOCK_START;
+                    }
+                }
+                if (state == S_EXIT) break;
+
+                if (state == S_SYM) {
+                    if (!force_serial && pos + 64ull * kPWMin + 160 <= nbits) {
+                        cmd = 1;  // the wave decodes this body
+                        break;
+                    }
+                    // serial decode: near the end of the input, or on from a
+                    // range where the wave met an error
+                    for (int it = 0; it < 64; ++it) {
+                        if (bitcnt < 48) WS_REFILL_W();
+                        if (pos + 64 <= nbits) {
+                            WS_SYMBOL(0)
+                        } else {
+                            WS_SYMBOL(1)
+                        }
+                    }
+                } else if (state == S_BLOCK_START) {
+                    force_serial = 0;
+                    if (pos >= hint) {
+                        state = S_FETCH;
+                        goto finish;
+                    }
+                    uint32_t v, bt;
+                    WS_GETBITS(v, 1);
+                    bfinal = v;
+                    WS_GETBITS(bt, 2);
+                    if (bt == 0) {
+                        pos = (pos + 7) & ~7ull;
+                        WS_REFILL();
+                        uint32_t ln, nln;
+                        WS_GETBITS(ln, 16);
+                        WS_GETBITS(nln, 16);
+                        if ((ln ^ nln) != 0xffffu) WS_FAIL(ST_INVALID, DET_STORED_NLEN);
+                        if (pos + 8ull * ln > nbits) WS_FAIL(ST_EOF, DET_NONE);
+                        if (ln) {
+                            const uint64_t off = (pos >> 3) - seg_byte;
+                            if (off + ln >= (1ull << 30) || out_n + ln >= kMaxSegCells) WS_FAIL(ST_TOO_BIG, DET_NONE);
+                            WS_FLUSH_LITS();
+                            for (uint32_t k = 0; k < ln; k += 256) {
+                                WS_EMIT((TOK_STORED << 30) | min(ln - k, 256u));
+                                WS_EMIT((TOK_OFFSET << 30) | (uint32_t)(off + k));
+                            }
+                            out_n += ln;
+                            pos += 8ull * ln;
+                        }
+                        WS_REFILL();
+                        state = S_BLOCK_END;
+                    } else if (bt == 1) {
+                        fixed_seq = LB < 9 ? 144u : 0u;
+                        for (int s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
+                        build_table(lens, 288, L.lit, LB, L.cntL, L.tnext, L.toffs, sortedL, L.longL, kLongLit, L.lhead);
+                        for (int s = 0; s < 30; ++s) lens[s] = 5;
+                        build_table(lens, 30, L.dist, WDLB, L.cntD, L.tnext, L.toffs, sortedD, L.longD, kLongDist,
+                                    L.lhead + 1);
+                        body_start = pos;
+                        state = S_SYM;
+                    } else if (bt == 2) {
+                        uint32_t a, b2, c2;
+                        WS_GETBITS(a, 5);
+                        WS_GETBITS(b2, 5);
+                        WS_GETBITS(c2, 4);
+                        const int hlit = (int)a + 257, hdist = (int)b2 + 1, hclen = (int)c2 + 4;
+                        if (hlit > 286 || hdist > 30) WS_FAIL(ST_INVALID, DET_HLIT_HDIST);
+                        uint8_t* cl = cltmp;
+                        for (int i = 0; i < 19; ++i) cl[i] = 0;
+                        for (int i = 0; i < hclen; ++i) {
+                            uint32_t x;
+                            WS_GETBITS(x, 3);
+                            cl[c_cl_order[i]] = (uint8_t)x;
+                        }
+                        if (!build_table(cl, 19, L.dist, 7, L.cntD, L.tnext, L.toffs, nullptr, (uint16_t*)nullptr, 0,
+                                         (uint32_t*)nullptr))
+                            WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                        const int total = hlit + hdist;
+                        int i = 0;
+                        uint8_t last = 0;
+                        while (i < total) {
+                            if (pos + 10 > nbits) WS_FAIL(ST_EOF, DET_NONE);
+                            if (bitcnt < 32) WS_REFILL();
+                            const uint32_t e = L.dist[bitbuf & 127u];
+                            const uint32_t l = e & 15u;
+                            if (l == 0) {
+                                if (pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);
+                                WS_FAIL(ST_INVALID, DET_NO_CODE);
+                            }
+                            WS_CONSUME(l);
+                            const uint32_t sym = e >> 4;
+                            if (sym <= 15) {
+                                last = (uint8_t)sym;
+                                lens[i++] = last;
+                            } else if (sym == 16) {
+                                if (i == 0) WS_FAIL(ST_INVALID, DET_CODE16_FIRST);
+                                uint32_t x;
+                                WS_GETBITS(x, 2);
+                                const int n = (int)x + 3;
+                                if (i + n > total) WS_FAIL(ST_INVALID, DET_CODE16_OVERRUN);
+                                for (int j = 0; j < n; ++j) lens[i + j] = last;
+                                i += n;
+                            } else if (sym == 17) {
+                                uint32_t x;
+                                WS_GETBITS(x, 3);
+                                const int n = (int)x + 3;
+                                if (i + n > total) WS_FAIL(ST_INVALID, DET_CODE17_OVERRUN);
+                                for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                                last = 0;
+                                i += n;
+                            } else {
+                                uint32_t x;
+                                WS_GETBITS(x, 7);
+                                const int n = (int)x + 11;
+                                if (i + n > total) WS_FAIL(ST_INVALID, DET_CODE18_OVERRUN);
+                                for (int j = 0; j < n; ++j) lens[i + j] = 0;
+                                last = 0;
+                                i += n;
+                            }
+                        }
+                        if (lens[256] == 0) WS_FAIL(ST_INVALID, DET_EOB_ZERO);
+                        fixed_seq = 0;
+                        if (!build_table(lens, hlit, L.lit, LB, L.cntL, L.tnext, L.toffs, sortedL, L.longL, kLongLit,
+                                         L.lhead))
+                            WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                        if (!build_table(lens + hlit, hdist, L.dist, WDLB, L.cntD, L.tnext, L.toffs, sortedD, L.longD,
+                                         kLongDist, L.lhead + 1))
+                            WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                        body_start = pos;
+                        state = S_SYM;
+                    } else {
+                        WS_FAIL(ST_INVALID, DET_RESERVED_BTYPE);
+                    }
+                } else if (state == S_BLOCK_END) {
+                    if (!bfinal) {
+                        if (mflags & SEG_BLOCK_ENDS) {
+                            if (n_mb >= mb_cap) WS_FAIL(ST_MB_OVERFLOW, DET_NONE);
+                            MemberRec m;
+                            m.out_off = out_n;
+                            m.trailer_byte = MB_BLOCK_END;
+                            m.crc = 0;
+                            m.isize = 0;
+                            mrec[mb_base + n_mb] = m;
+                            ++n_mb;
+                        }
+                        state = S_BLOCK_START;
+                    } else {
+                        pos = (pos + 7) & ~7ull;
+                        const uint64_t tb = pos >> 3;
+                        if (tb + 8 > (nbits >> 3)) WS_FAIL(ST_TRUNCATED, DET_NONE);
+                        const uint64_t w = bits64_at(in64, pos);
+                        if (n_mb >= mb_cap) WS_FAIL(ST_MB_OVERFLOW, DET_NONE);
+                        MemberRec m;
+                        m.out_off = out_n;
+                        m.trailer_byte = tb;
+                        m.crc = (uint32_t)w;
+                        m.isize = (uint32_t)(w >> 32);
+                        mrec[mb_base + n_mb] = m;
+                        ++n_mb;
+                        const uint64_t after = tb + 8;
+                        pos = after * 8;
+                        if (after >= (nbits >> 3)) {
+                            final_block = 1;
+                            state = S_FETCH;
+                            goto finish;
+                        }
+                        if (mflags & SEG_SINGLE_MEMBER) {
+                            state = S_FETCH;
+                            goto finish;
+                        }
+                        uint64_t hl = 0;
+                        const int hs = parse_gz_header(in64, nbits >> 3, after, &hl);
+                        if (hs != ST_OK) WS_FAIL((uint32_t)hs, DET_NONE);
+                        pos = (after + hl) * 8;
+                        WS_REFILL();
+                        member_base = out_n;
+                        spec = 0;
+                        state = S_BLOCK_START;
+                    }
+                }
+                continue;
+            finish : {
+                WS_FLUSH_LITS();
+                if (counting && (status == ST_OK || (mflags & SEG_BLOCK_ENDS))) status = ST_OVERFLOW;
+                SegResult r;
+                r.n_cells = out_n;
+                r.end_bit = pos;
+                r.status = status;
+                r.detail = detail;
+                r.final_block = final_block;
+                r.n_mb = n_mb;
+                r.n_tok = n_tok;
+                r.pad = 0;
+                res[seg] = r;
+            }
+            }
+            // hand the body to the wave (pending literals close first, so the
+            // token count is exact)
+            if (cmd) WS_FLUSH_LITS();
+            S.state = (uint32_t)state;
+            S.seg = seg;
+            S.pos = pos;
+            S.hint = hint;
+            S.seg_byte = seg_byte;
+            S.nbits = nbits;
+            S.tk_base = tk_base;
+            S.body_start = body_start;
+            S.tok_cap = tok_cap;
+            S.n_tok = n_tok;
+            S.out_n = out_n;
+            S.member_base = member_base;
+            S.spec = spec;
+            S.mflags = mflags;
+            S.bfinal = bfinal;
+            S.n_mb = n_mb;
+            S.mb_base = mb_base;
+            S.mb_cap = mb_cap;
+            S.fixed_seq = fixed_seq;
+            S.counting = counting;
+            S.force_serial = force_serial;
+            S.parW = parW;
+            S.status = status;
+            S.detail = detail;
+            S.final_block = final_block;
+            if (cmd) {
+                L.h.pos = pos;
+                L.h.nbits = nbits;
+                L.h.tk_base = tk_base;
+                L.h.n_tok = n_tok;
+                L.h.out_n = out_n;
+                L.h.tok_cap = tok_cap;
+                L.h.member_base = member_base;
+                L.h.spec = spec;
+                L.h.counting = counting;
+                L.h.fixed_seq = fixed_seq;
+                L.h.W = parW;
+            }
+            L.h.cmd = cmd;
+        }
+        __syncthreads();
+        if (uni32(L.h.cmd) == 0) break;
+
+        // ================= the wave: rounds over the block body =================
+        {
+            uint64_t P = uni64(L.h.pos);
+            const uint64_t nb_end = uni64(L.h.nbits);
+            uint32_t* const tkw = toks + uni64(L.h.tk_base);
+            uint32_t w_ntok = uni32(L.h.n_tok), w_outn = uni32(L.h.out_n);
+            const uint32_t w_cap = uni32(L.h.tok_cap), w_mbase = uni32(L.h.member_base);
+            const uint32_t w_spec = uni32(L.h.spec), w_fseq = uni32(L.h.fixed_seq);
+            uint32_t w_counting = uni32(L.h.counting);
+            const uint32_t Wwant = uni32(L.h.W);
+            uint32_t w_next = S_SYM;
+            const uint32_t k = lane;
+            uint2* const myrec = recs + k * (kPNM + 1);
+            uint16_t* const myX = L.X[k];
+
+            // lane walk state
+            uint64_t bb = 0, wci = 0, D0 = 0, D1 = 0, D2 = 0;
+            uint32_t bc = 0, r = 0;
+            uint32_t ukind = 0, usym = 0, ulen = 0, udist = 0;
+#define WP_REFILL()                                                                  \
+    do {                                                                             \
+        const uint64_t _a = P + r;                                                   \
+        const uint64_t _q = _a >> 6;                                                 \
+        if (_q != wci) {                                                             \
+            uint32_t _st = 3;                                                        \
+            if (_q > wci && _q - wci < 3) _st = (uint32_t)(_q - wci);                 \
+            wci = _q;                                                                \
+            for (uint32_t _k = _st; _k > 0; --_k) {                                  \
+                D0 = D1;                                                             \
+                D1 = D2;                                                             \
+                D2 = in64[wci + 3 - _k];                                             \
+            }                                                                        \
+        }                                                                            \
+        bb = funnel_lo(D0, D1, (uint32_t)(_a & 63));                                 \
+        bc = 64;                                                                     \
+    } while (0)
+// one unit at r: a literal (ukind 0, usym), a match (1, ulen, udist), the end of
+// block (2) or an invalid code (3); consumes its bits
+#define WP_UNIT()                                                                                       \
+    do {                                                                                                \
+        if (bc < 48) WP_REFILL();                                                                       \
+        uint32_t _e = L.lit[bb & LMASK];                                                                \
+        uint32_t _l = _e & 15u, _s = _e >> 4;                                                           \
+        ukind = 3;                                                                                      \
+        bool _ok = true;                                                                                \
+        if (_l == 0) _ok = decode_long<LB>(bb, L.cntL, sortedL, L.longL, kLongLit, L.lhead[0], w_fseq, &_s, &_l); \
+        if (_ok) {                                                                                      \
+            bb >>= _l;                                                                                  \
+            bc -= _l;                                                                                   \
+            r += _l;                                                                                    \
+            if (_s < 256) {                                                                             \
+                ukind = 0;                                                                              \
+                usym = _s;                                                                              \
+            } else if (_s == 256) {                                                                     \
+                ukind = 2;                                                                              \
+            } else if (_s <= 285) {                                                                     \
+                const uint32_t _li = _s - 257, _lx = len_extra(_li);                                    \
+                ulen = len_base(_li) + (uint32_t)(bb & ((1ull << _lx) - 1));                            \
+                bb >>= _lx;                                                                             \
+                bc -= _lx;                                                                              \
+                r += _lx;                                                                               \
+                uint32_t _de = L.dist[bb & DMASK];                                                      \
+                uint32_t _dl = _de & 15u, _ds = _de >> 4;                                               \
+                bool _ok2 = true;                                                                       \
+                if (_dl == 0)                                                                           \
+                    _ok2 = decode_long<WDLB>(bb, L.cntD, sortedD, L.longD, kLongDist, L.lhead[1], 0u, &_ds, &_dl); \
+                if (_ok2 && _ds < 30) {                                                                 \
+                    bb >>= _dl;                                                                         \
+                    const uint32_t _dx = dist_extra(_ds);                                               \
+                    udist = dist_base(_ds) + (uint32_t)(bb & ((1ull << _dx) - 1));                      \
+                    bb >>= _dx;                                                                         \
+                    bc -= _dl + _dx;                                                                    \
+                    r += _dl + _dx;                                                                     \
+                    ukind = 1;                                                                          \
+                }                                                                                       \
+            }                                                                                           \
+        }                                                                                               \
+    } while (0)
+
+            while (true) {
+                // the window: as asked, shrunk so that every read stays inside the input
+                const uint64_t room = nb_end > P + 160 ? (nb_end - P - 160) / 64 : 0;
+                const uint32_t Wr = (uint32_t)((uint64_t)Wwant < room ? (uint64_t)Wwant : room) & ~(kPB - 1);
+                if (Wr < kPWMin) {
+                    w_next = S_SYM;
+                    break;
+                }
+                const uint32_t NMr = Wr / kPB;
+                const uint32_t RW = 64u * Wr;
+                // ---- 1. own window ----
+                const uint32_t g = k * Wr;
+                r = g;
+                wci = ~0ull;
+                bc = 0;
+                uint32_t nbd = g, m = 0, tok = 0, cells = 0, litn = 0, kind = WK_WALK;
+                while (true) {
+                    if (r >= nbd) {
+                        tok += litn != 0;
+                        litn = 0;
+                        myX[m] = (uint16_t)(r - g);
+                        myrec[m] = make_uint2(tok, cells);
+                        if (m == NMr) break;
+                        ++m;
+                        nbd += kPB;
+                    }
+                    WP_UNIT();
+                    if (ukind == 0) {
+                        cells += 1;
+                        if (++litn == 3) {
+                            ++tok;
+                            litn = 0;
+                        }
+                    } else if (ukind == 1) {
+                        tok += (litn != 0) + 1;
+                        litn = 0;
+                        cells += ulen;
+                    } else {
+                        if (ukind == 2) tok += litn != 0;
+                        litn = 0;
+                        kind = ukind == 2 ? WK_EOB : WK_ERR;
+                        break;
+                    }
+                }
+                for (uint32_t mm = kind == WK_WALK ? NMr + 1 : m; mm <= NMr; ++mm) myX[mm] = 0xFFFFu;
+                __syncthreads();
+                // ---- 2. on into the next windows until a crossing agrees ----
+                uint32_t tgt = kind == WK_EOB ? WT_EOB : kind == WK_ERR ? WT_ERR : WT_END, tm = 0;
+                if (kind == WK_WALK && k < 63) {
+                    uint32_t j = k + 1, mm = 0, bnd = (k + 1) * Wr;
+                    while (true) {
+                        bool stop = false;
+                        while (r >= bnd) {
+                            if (j == 64) {
+                                tgt = WT_END;
+                                stop = true;
+                                break;
+                            }
+                            tok += litn != 0;
+                            litn = 0;
+                            if ((uint32_t)L.X[j][mm] == r - j * Wr) {
+                                tgt = j;
+                                tm = mm;
+                                stop = true;
+                                break;
+                            }
+                            bnd += kPB;
+                            if (++mm == NMr) {
+                                mm = 0;
+                                ++j;
+                            }
+                        }
+                        if (stop) break;
+                        WP_UNIT();
+                        if (ukind == 0) {
+                            cells += 1;
+                            if (++litn == 3) {
+                                ++tok;
+                                litn = 0;
+                            }
+                        } else if (ukind == 1) {
+                            tok += (litn != 0) + 1;
+                            litn = 0;
+                            cells += ulen;
+                        } else {
+                            if (ukind == 2) tok += litn != 0;
+                            litn = 0;
+                            tgt = ukind == 2 ? WT_EOB : WT_ERR;
+                            break;
+                        }
+                    }
+                }
+                const uint32_t r_end = r;  // where this lane's walk ended
+                // ---- 3. the true lanes, from lane 0 on ----
+                uint64_t valid = 0;
+                uint32_t c = 0, sm = 0, mysm = 0, last = 0, last_t = 0;
+                while (true) {
+                    valid |= 1ull << c;
+                    if (k == c) mysm = sm;
+                    const uint32_t t = uni32((uint32_t)__builtin_amdgcn_readlane((int)tgt, (int)c));
+                    if (t >= 64) {
+                        last = c;
+                        last_t = t;
+                        break;
+                    }
+                    sm = uni32((uint32_t)__builtin_amdgcn_readlane((int)tm, (int)c));
+                    c = t;
+                }
+                const uint64_t committed = last_t == WT_ERR ? (valid & ((1ull << last) - 1)) : valid;
+                const bool com = (committed >> k) & 1ull;
+                const uint32_t start = g + (uint32_t)myX[mysm];  // the lane's range starts here (if true)
+                uint32_t ntok = 0, ncell = 0;
+                if (com) {
+                    const uint2 b0 = mysm ? myrec[mysm] : make_uint2(0u, 0u);
+                    ntok = tok - b0.x;
+                    ncell = cells - b0.y;
+                }
+                uint32_t it = ntok, ic = ncell;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t o1 = (uint32_t)__shfl_up((int)it, d, 64);
+                    const uint32_t o2 = (uint32_t)__shfl_up((int)ic, d, 64);
+                    if ((int)k >= d) {
+                        it += o1;
+                        ic += o2;
+                    }
+                }
+                const uint32_t Tr = uni32((uint32_t)__builtin_amdgcn_readlane((int)it, 63));
+                const uint32_t Cr = uni32((uint32_t)__builtin_amdgcn_readlane((int)ic, 63));
+                if ((uint64_t)w_outn + Cr >= kMaxSegCells) {
+                    w_next = S_EXIT;
+                    break;
+                }
+                if (!w_counting && (uint64_t)w_ntok + Tr > w_cap) w_counting = 1;
+                // ---- 4. the true lanes decode their ranges again and store ----
+                uint32_t bad = 0, emitted = 0;
+                if (com) {
+                    r = start;
+                    wci = ~0ull;
+                    bc = 0;
+                    const uint32_t stop_at = tgt < 64 ? tgt * Wr + (uint32_t)L.X[tgt][tm] : 0xFFFFFFFFu;
+                    uint32_t nb2 = g + (mysm + 1) * kPB;
+                    const uint32_t tbase = w_ntok + (it - ntok);
+                    const uint32_t cbase = w_outn + (ic - ncell);
+                    uint32_t lc = 0, ln = 0, lacc = 0;
+                    while (true) {
+                        if (r >= nb2) {
+                            if (ln) {
+                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
+                                ++emitted;
+                            }
+                            ln = 0;
+                            lacc = 0;
+                            nb2 += kPB;
+                        }
+                        if (r == stop_at || (tgt == WT_END && r >= RW)) break;
+                        WP_UNIT();
+                        if (ukind == 0) {
+                            lacc |= usym << (8 * ln);
+                            if (++ln == 3) {
+                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (3u << 24) | lacc;
+                                ++emitted;
+                                ln = 0;
+                                lacc = 0;
+                            }
+                            ++lc;
+                        } else if (ukind == 1) {
+                            if (ln) {
+                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
+                                ++emitted;
+                            }
+                            ln = 0;
+                            lacc = 0;
+                            if (!w_spec && udist > cbase + lc - w_mbase) {
+                                bad = 1;
+                                break;
+                            }
+                            if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_MATCH << 30) | ((ulen - 3) << 16) | (udist - 1);
+                            ++emitted;
+                            lc += ulen;
+                        } else if (ukind == 2) {
+                            if (ln) {
+                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
+                                ++emitted;
+                            }
+                            break;
+                        } else {
+                            bad = 1;
+                            break;
+                        }
+                        if (r > RW + 64) {  // past every range end: cannot happen on a true walk
+                            bad = 1;
+                            break;
+                        }
+                    }
+                    if (!bad && (emitted != ntok || lc != ncell)) bad = 1;
+                }
+                const uint64_t badm = __ballot(com && bad);
+                if (badm) {
+                    // lane 0 decodes on from the first failing range
+                    const uint32_t f = (uint32_t)__builtin_ctzll(badm);
+                    w_ntok += uni32((uint32_t)__builtin_amdgcn_readlane((int)(it - ntok), (int)f));
+                    w_outn += uni32((uint32_t)__builtin_amdgcn_readlane((int)(ic - ncell), (int)f));
+                    P += uni32((uint32_t)__builtin_amdgcn_readlane((int)start, (int)f));
+                    w_next = S_SYM;
+                    break;
+                }
+                w_ntok += Tr;
+                w_outn += Cr;
+                if (last_t == WT_ERR) {
+                    P += uni32((uint32_t)__builtin_amdgcn_readlane((int)start, (int)last));
+                    w_next = S_SYM;
+                    break;
+                }
+                P += uni32((uint32_t)__builtin_amdgcn_readlane((int)r_end, (int)last));
+                if (last_t == WT_EOB) {
+                    w_next = S_BLOCK_END;
+                    break;
+                }
+                __syncthreads();  // the next round rewrites the crossing table
+            }
+#undef WP_UNIT
+#undef WP_REFILL
+            if (lane == 0) {
+                L.h.pos = P;
+                L.h.n_tok = w_ntok;
+                L.h.out_n = w_outn;
+                L.h.counting = w_counting;
+                L.h.next = w_next;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            // lane 0 goes on from where the wave stopped
+            SerialState& S = L.ss;
+            const uint32_t nx = L.h.next;
+            S.pos = L.h.pos;
+            S.n_tok = L.h.n_tok;
+            S.out_n = L.h.out_n;
+            S.counting = L.h.counting;
+            if (nx == S_BLOCK_END) {
+                // windows for the next body: about this one's size over 64 lanes
+                const uint64_t body = S.pos - S.body_start;
+                const uint64_t w = ((body / 60 + kPB - 1) / kPB) * kPB;
+                S.parW = (uint32_t)(w < kPWMin ? kPWMin : w > kPWMax ? kPWMax : w);
+                S.state = S_BLOCK_END;
+            } else if (nx == S_SYM) {
+                S.force_serial = 1;  // the rest of this body: serially, with every check
+                S.state = S_SYM;
+            } else {
+                S.status = ST_TOO_BIG;
+                S.detail = DET_NONE;
+                S.state = S_FINISH;
+            }
+        }
+        __syncthreads();
+    }
+#undef WS_SYMBOL
+#undef WS_REFILL_W
+#undef WS_REFILL
+#undef WS_CONSUME
+#undef WS_FAIL
+#undef WS_GETBITS
+#undef WS_EMIT
+#undef WS_FLUSH_LITS
+}
+
+int inflate_wave_lds_bytes(int lut_bits) {
+    switch (lut_bits) {
+        case 9: return wave_lds_bytes<9>();
+        case 11: return wave_lds_bytes<11>();
+        default: return wave_lds_bytes<10>();
+    }
+}
+
+int inflate_wave_blocks_per_cu(int lut_bits) {
+    const size_t shm = (size_t)inflate_wave_lds_bytes(lut_bits);
+    int nb = 0;
+    hipError_t e;
+    switch (lut_bits) {
+        case 9: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens_wave<9>, 64, shm); break;
+        case 11: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens_wave<11>, 64, shm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tokens_wave<10>, 64, shm); break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_tokens_wave(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                              const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint32_t* d_toks,
+                              uint8_t* d_scratch, const InflateGeom& g, hipStream_t s) {
+    if (n_work == 0) return hipSuccess;
+    const size_t shm = (size_t)inflate_wave_lds_bytes(g.lut_bits);
+    switch (g.lut_bits) {
+        case 9:
+            hipLaunchKernelGGL(k_tokens_wave<9>, dim3(g.blocks), dim3(64), shm, s, d_in64, d_segs, d_res, d_mrec,
+                               d_work, n_work, d_counter, d_toks, d_scratch);
+            break;
+        case 11:
+            hipLaunchKernelGGL(k_tokens_wave<11>, dim3(g.blocks), dim3(64), shm, s, d_in64, d_segs, d_res, d_mrec,
+                               d_work, n_work, d_counter, d_toks, d_scratch);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tokens_wave<10>, dim3(g.blocks), dim3(64), shm, s, d_in64, d_segs, d_res, d_mrec,
+                               d_work, n_work, d_counter, d_toks, d_scratch);
             break;
     }
     return hipGetLastError();

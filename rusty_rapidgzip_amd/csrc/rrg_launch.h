@@ -38,9 +38,10 @@ hipError_t launch_leg_fixup(const uint64_t* d_in64, uint64_t total_bits, const L
 // Pass 1 of the inflate: persistent lane-per-segment Huffman decode of the
 // segment indices in d_work[0..n_work) into tokens (rrg_device.h).
 struct InflateGeom {
-    int lanes_per_block;  // threads per workgroup (one lane decodes one segment)
+    int lanes_per_block;  // threads per workgroup (one lane decodes one segment); 64 with `wave`
     int blocks;           // persistent workgroups
-    int lut_bits;         // literal/length LUT bits (8..10)
+    int lut_bits;         // literal/length LUT bits (8..10; 9..11 with `wave`)
+    bool wave = false;    // one wavefront per segment (k_tokens_wave)
 };
 int inflate_lds_per_lane(int lut_bits);
 int inflate_scratch_u16_per_lane();  // per-lane global scratch (d_sorted), in u16
@@ -48,6 +49,16 @@ int inflate_blocks_per_cu(int lut_bits, int lanes);
 hipError_t launch_tokens(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
                          const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint32_t* d_toks,
                          uint16_t* d_sorted, const InflateGeom& g, hipStream_t s);
+// The wave-cooperative form of pass 1 (one wavefront and one set of tables per
+// segment; the 64 lanes decode each Huffman block body together): the same
+// results and token stream.  d_scratch holds inflate_wave_scratch_bytes() per
+// workgroup.
+size_t inflate_wave_scratch_bytes();
+int inflate_wave_lds_bytes(int lut_bits);
+int inflate_wave_blocks_per_cu(int lut_bits);
+hipError_t launch_tokens_wave(const uint64_t* d_in64, const SegDesc* d_segs, SegResult* d_res, MemberRec* d_mrec,
+                              const uint32_t* d_work, uint32_t n_work, uint32_t* d_counter, uint32_t* d_toks,
+                              uint8_t* d_scratch, const InflateGeom& g, hipStream_t s);
 
 // Pass 2: one wavefront per item expands its tokens into u16 cells at
 // d_cells[dst] (or bytes at d_out[dst] with EXP_U8); persistent waves take

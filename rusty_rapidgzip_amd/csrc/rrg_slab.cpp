@@ -728,7 +728,7 @@ struct Setup {
     void worker(Worker& W) const {
         W.budget = budget;
         // per-lane scratch, both input buffers' margins and the small buffers
-        W.fixed = W.G.lanes_in_flight * 1024 + 2 * (4ull << 20) + (2ull << 20);
+        W.fixed = W.G.scratch_bytes + 2 * (4ull << 20) + (2ull << 20);
     }
 };
 
