@@ -1668,8 +1668,9 @@ constexpr uint32_t kPB = 128;                // crossing spacing (bits)
 constexpr uint32_t kPWMax = 4096;            // window per lane (bits), at most
 constexpr uint32_t kPWMin = 2 * kPB;         // smaller windows: lane 0 decodes alone
 constexpr uint32_t kPNM = kPWMax / kPB;      // crossings per window
-constexpr int WDLB = 8;                      // distance LUT bits
-constexpr size_t kWaveScratchBytes = 1024 + 64 * (kPNM + 1) * sizeof(uint2);
+constexpr uint32_t kPMS = 16;                // crossings per window where a later lane can take over
+constexpr int WDLB = 9;                      // distance LUT bits
+constexpr size_t kWaveScratchBytes = 256;  // unused (the rounds keep their records in LDS)
 
 size_t inflate_wave_scratch_bytes() { return kWaveScratchBytes; }
 
@@ -1678,26 +1679,158 @@ size_t inflate_wave_scratch_bytes() { return kWaveScratchBytes; }
 struct SerialState {
     uint64_t pos, hint, seg_byte, nbits, tk_base, body_start;
     uint32_t state, seg, tok_cap, n_tok, out_n, member_base, spec, mflags, bfinal, n_mb, mb_base, mb_cap;
-    uint32_t fixed_seq, counting, force_serial, parW, status, detail, final_block, pad;
+    uint32_t fixed_seq, counting, force_serial, parW, status, detail, final_block, hlit, hdist, pad;
 };
 
 struct WaveHand {
     uint64_t pos;      // lane 0 -> wave: body start; wave -> lane 0: where lane 0 goes on
     uint64_t nbits;
     uint64_t tk_base;  // the segment's token region
-    uint32_t cmd;      // 0: no segment left, 1: decode a block body
+    uint32_t cmd;      // 0: no segment left, 1: decode a block body, 2: build its tables first
     uint32_t next;     // wave -> lane 0: S_BLOCK_END (body done), S_SYM (decode on serially), S_EXIT (too big)
     uint32_t n_tok, out_n, tok_cap, member_base, spec, counting, fixed_seq, W;
+    uint32_t hlit, hdist;  // with cmd 2: the code lengths in lens to build the tables from
 };
+
+// Codes longer than a LUT, decoded from the cumulative Kraft sums: with the
+// next 15 input bits read first-bit-highest as `rev`, a code has length l
+// when lj[l-1] <= rev < lj[l] (lj[l] = sum over k <= l of count[k] << (15-k),
+// the left-justified end of the length-l codes), and its index among the
+// symbols sorted by (length, symbol) is (rev >> (15-l)) + bias[l] (bias[l] =
+// symbols shorter than l - first code of length l).
+struct LongTab {
+    uint16_t lj[16];
+    int16_t bias[16];
+};
+
+// The symbol of a code longer than LB bits (false: no such code), from the
+// symbols sorted by (length, symbol).
+template <int LB>
+__device__ __forceinline__ bool wave_long(uint64_t bits, const LongTab& t, const uint16_t* __restrict__ sorted,
+                                          uint32_t* sym, uint32_t* len) {
+    const uint32_t rev = __brev((uint32_t)bits) >> 17;
+    const uint4 q = *reinterpret_cast<const uint4*>(&t.lj[8]);
+    const uint32_t v[8] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16,
+                           q.z & 0xffffu, q.z >> 16, q.w & 0xffffu, q.w >> 16};
+    uint32_t n = 0;
+#pragma unroll
+    for (int l = LB + 1; l <= 15; ++l) n += rev >= v[l - 8] ? 1u : 0u;
+    const uint32_t l = LB + 1 + n;
+    if (l > 15) return false;
+    *sym = sorted[(uint32_t)((int)(rev >> (15 - l)) + (int)t.bias[l])];
+    *len = l;
+    return true;
+}
+
+// LUT entries of the wave kernel.  Literal/length (u16): bits 3..0 code
+// length (0: longer than the LUT), bit 4 clear: a literal, its byte in bits
+// 15..8; bit 4 set: bits 7..5 = extra bits of a length (0..5; 6 = symbol 286
+// or 287, invalid; 7 = end of block) and bits 15..8 = its base - 3.
+// Distance (u32): bits 3..0 code length, 7..4 extra bits, 23..8 base.
+// (Length and distance bases and extra bits: deflate/tables.rs:10-30.)
+__device__ __forceinline__ uint32_t lit_entry(uint32_t s, uint32_t l) {
+    if (s < 256) return (s << 8) | l;
+    if (s == 256) return (7u << 5) | 16u | l;
+    if (s > 285) return (6u << 5) | 16u | l;
+    const uint32_t li = s - 257;
+    return ((len_base(li) - 3u) << 8) | (len_extra(li) << 5) | 16u | l;
+}
+__device__ __forceinline__ uint32_t dist_entry(uint32_t s, uint32_t l) {
+    return (dist_base(s) << 8) | (dist_extra(s) << 4) | l;  // s < 30: HDIST <= 30, the fixed code has 30
+}
+
+// Canonical Huffman table from n code lengths (huffman.rs:215-325 semantics:
+// incomplete codes accepted), built by the 64 lanes of the wave: counts by
+// LDS atomics, each symbol's place in (length, symbol) order by ballots, and
+// every LUT entry decoded from the cumulative Kraft sums.  LUT entries are
+// lit_entry / dist_entry for codes of at most TB bits, 0 otherwise.  Returns
+// false (the same in every lane) when the code is oversubscribed with at least
+// two symbols.  Called by the whole wave (it synchronises).
+template <int TB, bool DIST>
+__device__ bool wave_build(const uint8_t* __restrict__ lens, uint32_t n, void* __restrict__ lutv,
+                           uint16_t* __restrict__ sorted, LongTab& lt, uint32_t* __restrict__ c32, uint32_t lane) {
+    if (lane < 16) c32[lane] = 0;
+    __syncthreads();
+    for (uint32_t s = lane; s < n; s += 64) {
+        const uint32_t l = lens[s];
+        if (l) atomicAdd(&c32[l], 1u);
+    }
+    __syncthreads();
+    uint32_t cnt[16], cur[16], ljv[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) cnt[l] = l ? uni32(c32[l]) : 0u;
+    uint32_t total = 0, nsym = 0, code = 0, off = 0;
+#pragma unroll
+    for (int l = 1; l <= 15; ++l) {
+        total += cnt[l] << (15 - l);
+        nsym += cnt[l];
+    }
+    if (nsym >= 2 && total > 32768u) return false;
+    ljv[0] = 0;
+#pragma unroll
+    for (int l = 1; l <= 15; ++l) {
+        code = (code + cnt[l - 1]) << 1;
+        cur[l] = off;
+        ljv[l] = ljv[l - 1] + (cnt[l] << (15 - l));
+        if (lane == (uint32_t)l) {
+            lt.lj[l] = (uint16_t)ljv[l];
+            lt.bias[l] = (int16_t)((int)off - (int)code);
+        }
+        off += cnt[l];
+    }
+    if (lane == 0) {
+        lt.lj[0] = 0;
+        lt.bias[0] = 0;
+    }
+    // symbols in (length, symbol) order
+    for (uint32_t b = 0; b < n; b += 64) {
+        const uint32_t s = b + lane;
+        const uint32_t l = s < n ? lens[s] : 0u;
+        uint32_t at = 0;
+#pragma unroll
+        for (int ll = 1; ll <= 15; ++ll) {
+            const uint64_t m = __ballot(l == (uint32_t)ll);
+            if (l == (uint32_t)ll) at = cur[ll] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+            cur[ll] += (uint32_t)__popcll(m);
+        }
+        if (l) sorted[at] = (uint16_t)s;
+    }
+    __syncthreads();
+    // LUT: the TB bits x (first input bit lowest) decode to the code of length l
+    // when lj[l-1] <= rev < lj[l], rev = x reversed and left-justified to 15 bits
+    for (uint32_t x = lane; x < (1u << TB); x += 64) {
+        const uint32_t rev = (__brev(x) >> (32 - TB)) << (15 - TB);
+        uint32_t l = 1;
+#pragma unroll
+        for (int ll = 1; ll < TB; ++ll) l += rev >= ljv[ll] ? 1u : 0u;
+        uint32_t e = 0;
+        if (rev < lt.lj[l]) {
+            const uint32_t sym = sorted[(uint32_t)((int)(rev >> (15 - l)) + (int)lt.bias[l])];
+            e = DIST ? dist_entry(sym, l) : lit_entry(sym, l);
+        }
+        if (DIST) static_cast<uint32_t*>(lutv)[x] = e;
+        else static_cast<uint16_t*>(lutv)[x] = (uint16_t)e;
+    }
+    __syncthreads();
+    return true;
+}
 
 template <int LB>
 struct WaveLds {
-    uint16_t lit[1 << LB];   // (symbol << 4) | length
-    uint16_t dist[1 << WDLB];  // the same; the 7-bit precode LUT while a header is read
-    uint16_t cntL[16], cntD[16], tnext[16], toffs[16];
-    uint16_t longL[kLongLit], longD[kLongDist];
-    uint32_t lhead[2];
-    uint16_t X[64][kPNM + 1];  // per lane and crossing: first symbol start at or past it (from the window start)
+    uint16_t lit[1 << LB];     // lit_entry
+    uint32_t dist[1 << WDLB];  // dist_entry
+    uint16_t pre[128];         // precode LUT (7 bits): (symbol << 4) | length
+    LongTab ltL, ltD;          // codes longer than the LUTs
+    uint16_t sortL[288], sortD[32];  // symbols in (length, symbol) order
+    uint8_t lens[320];       // code lengths: literal/length, then distance
+    uint8_t cl[32];          // precode lengths
+    uint32_t c32[16];        // code-length counts (table build)
+    uint16_t cnt[16], tnext[16], toffs[16];  // precode table build
+    // per lane, for its window's first kPMS crossings: the first symbol start at
+    // or past the crossing (from the window start; 0xFFFF: not reached), and
+    // the tokens its walk produced before it
+    uint16_t X[64][kPMS];
+    uint16_t T[64][kPMS];
     WaveHand h;
     SerialState ss;
 };
@@ -1724,11 +1857,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr uint32_t DMASK = (1u << WDLB) - 1;
     const uint32_t lane = threadIdx.x;
     uint8_t* my_scr = wscratch + (size_t)blockIdx.x * kWaveScratchBytes;
-    uint16_t* sortedL = reinterpret_cast<uint16_t*>(my_scr);
-    uint16_t* sortedD = sortedL + 288;
-    uint8_t* lens = reinterpret_cast<uint8_t*>(sortedL + 320);
-    uint8_t* cltmp = reinterpret_cast<uint8_t*>(sortedL + 480);
-    uint2* recs = reinterpret_cast<uint2*>(my_scr + 1024);
+    uint8_t* const lens = L.lens;
+    (void)my_scr;
 
     if (lane == 0) {
         L.ss = SerialState{};
@@ -1798,16 +1928,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #define WS_SYMBOL(CHK)                                                                           \
     {                                                                                            \
         uint32_t e = L.lit[bitbuf & LMASK];                                                      \
-        uint32_t l = e & 15u, sym = e >> 4;                                                      \
+        uint32_t l = e & 15u;                                                                    \
         if (l == 0) {                                                                            \
             if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
-            if (!decode_long<LB>(bitbuf, L.cntL, sortedL, L.longL, kLongLit, L.lhead[0], fixed_seq, &sym, &l)) \
-                WS_FAIL(ST_INVALID, DET_NO_CODE);                                                \
+            uint32_t s2, l2;                                                                     \
+            if (!wave_long<LB>(bitbuf, L.ltL, L.sortL, &s2, &l2)) WS_FAIL(ST_INVALID, DET_NO_CODE); \
+            e = lit_entry(s2, l2);                                                               \
+            l = l2;                                                                              \
         }                                                                                        \
         if ((CHK) && pos + l > nbits) WS_FAIL(ST_EOF, DET_NONE);                                 \
         WS_CONSUME(l);                                                                           \
-        if (sym < 256) {                                                                         \
-            lit_acc |= sym << (8 * lit_n);                                                       \
+        if (!(e & 16u)) {                                                                        \
+            lit_acc |= (e >> 8) << (8 * lit_n);                                                  \
             if (++lit_n == 3) {                                                                  \
                 WS_EMIT((TOK_LIT << 30) | (3u << 24) | lit_acc);                                 \
                 lit_n = 0;                                                                       \
@@ -1816,29 +1948,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             ++out_n;                                                                             \
             continue;                                                                            \
         }                                                                                        \
-        if (sym == 256) {                                                                        \
+        const uint32_t lx = (e >> 5) & 7u;                                                       \
+        if (lx == 7) {                                                                           \
             state = S_BLOCK_END;                                                                 \
             break;                                                                               \
         }                                                                                        \
-        if (sym > 285) WS_FAIL(ST_INVALID, DET_LITLEN_RANGE);                                    \
-        const uint32_t li = sym - 257;                                                           \
-        const uint32_t lx = len_extra(li);                                                       \
+        if (lx == 6) WS_FAIL(ST_INVALID, DET_LITLEN_RANGE);                                      \
         if ((CHK) && pos + lx > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
-        const uint32_t length = len_base(li) + (uint32_t)(bitbuf & ((1ull << lx) - 1));          \
+        const uint32_t length = (e >> 8) + 3u + (uint32_t)(bitbuf & ((1ull << lx) - 1));         \
         WS_CONSUME(lx);                                                                          \
         uint32_t de = L.dist[bitbuf & DMASK];                                                    \
-        uint32_t dl = de & 15u, dsym = de >> 4;                                                  \
+        uint32_t dl = de & 15u;                                                                  \
         if (dl == 0) {                                                                           \
             if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
-            if (!decode_long<WDLB>(bitbuf, L.cntD, sortedD, L.longD, kLongDist, L.lhead[1], 0u, &dsym, &dl)) \
-                WS_FAIL(ST_INVALID, DET_NO_CODE);                                                \
+            uint32_t s2, l2;                                                                     \
+            if (!wave_long<WDLB>(bitbuf, L.ltD, L.sortD, &s2, &l2)) WS_FAIL(ST_INVALID, DET_NO_CODE); \
+            de = dist_entry(s2, l2);                                                             \
+            dl = l2;                                                                             \
         }                                                                                        \
         if ((CHK) && pos + dl > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
         WS_CONSUME(dl);                                                                          \
-        if (dsym >= 30) WS_FAIL(ST_INVALID, DET_DIST_RANGE);                                     \
-        const uint32_t dx = dist_extra(dsym);                                                    \
+        const uint32_t dx = (de >> 4) & 15u;                                                     \
         if ((CHK) && pos + dx > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
-        const uint32_t distance = dist_base(dsym) + (uint32_t)(bitbuf & ((1ull << dx) - 1));     \
+        const uint32_t distance = (de >> 8) + (uint32_t)(bitbuf & ((1ull << dx) - 1));           \
         WS_CONSUME(dx);                                                                          \
         if (distance > out_n - member_base && !spec) WS_FAIL(ST_INVALID, DET_DIST_FAR);          \
         WS_FLUSH_LITS();                                                                         \
@@ -1859,6 +1991,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             uint32_t mb_cap = S.mb_cap, fixed_seq = S.fixed_seq, counting = S.counting;
             uint32_t force_serial = S.force_serial, parW = S.parW;
             uint32_t status = S.status, detail = S.detail, final_block = S.final_block;
+            uint32_t s_hlit = S.hlit, s_hdist = S.hdist;
             uint32_t* tk = toks + tk_base;
             uint32_t lit_acc = 0, lit_n = 0;
             uint64_t bitbuf = 0, ci = ~0ull, C0 = 0, C1 = 0, C2 = 0;
@@ -1959,14 +2092,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         WS_REFILL();
                         state = S_BLOCK_END;
                     } else if (bt == 1) {
-                        fixed_seq = LB < 9 ? 144u : 0u;
                         for (int s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
-                        build_table(lens, 288, L.lit, LB, L.cntL, L.tnext, L.toffs, sortedL, L.longL, kLongLit, L.lhead);
-                        for (int s = 0; s < 30; ++s) lens[s] = 5;
-                        build_table(lens, 30, L.dist, WDLB, L.cntD, L.tnext, L.toffs, sortedD, L.longD, kLongDist,
-                                    L.lhead + 1);
+                        for (int s = 0; s < 30; ++s) lens[288 + s] = 5;
+                        s_hlit = 288;
+                        s_hdist = 30;
                         body_start = pos;
                         state = S_SYM;
+                        cmd = 2;  // the wave builds the tables
+                        break;
                     } else if (bt == 2) {
                         uint32_t a, b2, c2;
                         WS_GETBITS(a, 5);
@@ -1974,14 +2107,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         WS_GETBITS(c2, 4);
                         const int hlit = (int)a + 257, hdist = (int)b2 + 1, hclen = (int)c2 + 4;
                         if (hlit > 286 || hdist > 30) WS_FAIL(ST_INVALID, DET_HLIT_HDIST);
-                        uint8_t* cl = cltmp;
+                        uint8_t* cl = L.cl;
                         for (int i = 0; i < 19; ++i) cl[i] = 0;
                         for (int i = 0; i < hclen; ++i) {
                             uint32_t x;
                             WS_GETBITS(x, 3);
                             cl[c_cl_order[i]] = (uint8_t)x;
                         }
-                        if (!build_table(cl, 19, L.dist, 7, L.cntD, L.tnext, L.toffs, nullptr, (uint16_t*)nullptr, 0,
+                        if (!build_table(cl, 19, L.pre, 7, L.cnt, L.tnext, L.toffs, nullptr, (uint16_t*)nullptr, 0,
                                          (uint32_t*)nullptr))
                             WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
                         const int total = hlit + hdist;
@@ -1990,7 +2123,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         while (i < total) {
                             if (pos + 10 > nbits) WS_FAIL(ST_EOF, DET_NONE);
                             if (bitcnt < 32) WS_REFILL();
-                            const uint32_t e = L.dist[bitbuf & 127u];
+                            const uint32_t e = L.pre[bitbuf & 127u];
                             const uint32_t l = e & 15u;
                             if (l == 0) {
                                 if (pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);
@@ -2028,15 +2161,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                             }
                         }
                         if (lens[256] == 0) WS_FAIL(ST_INVALID, DET_EOB_ZERO);
-                        fixed_seq = 0;
-                        if (!build_table(lens, hlit, L.lit, LB, L.cntL, L.tnext, L.toffs, sortedL, L.longL, kLongLit,
-                                         L.lhead))
-                            WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
-                        if (!build_table(lens + hlit, hdist, L.dist, WDLB, L.cntD, L.tnext, L.toffs, sortedD, L.longD,
-                                         kLongDist, L.lhead + 1))
-                            WS_FAIL(ST_INVALID, DET_OVERSUBSCRIBED);
+                        s_hlit = (uint32_t)hlit;
+                        s_hdist = (uint32_t)hdist;
                         body_start = pos;
                         state = S_SYM;
+                        cmd = 2;  // the wave builds the tables (an oversubscribed code fails there)
+                        break;
                     } else {
                         WS_FAIL(ST_INVALID, DET_RESERVED_BTYPE);
                     }
@@ -2131,6 +2261,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             S.status = status;
             S.detail = detail;
             S.final_block = final_block;
+            S.hlit = s_hlit;
+            S.hdist = s_hdist;
             if (cmd) {
                 L.h.pos = pos;
                 L.h.nbits = nbits;
@@ -2143,11 +2275,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 L.h.counting = counting;
                 L.h.fixed_seq = fixed_seq;
                 L.h.W = parW;
+                L.h.hlit = s_hlit;
+                L.h.hdist = s_hdist;
             }
             L.h.cmd = cmd;
         }
         __syncthreads();
-        if (uni32(L.h.cmd) == 0) break;
+        const uint32_t wcmd = uni32(L.h.cmd);
+        if (wcmd == 0) break;
+        bool tables_ok = true;
+        if (wcmd == 2) {
+            // the block's Huffman tables, by the whole wave
+            const uint32_t hl = uni32(L.h.hlit), hd = uni32(L.h.hdist);
+            tables_ok = wave_build<LB, false>(L.lens, hl, L.lit, L.sortL, L.ltL, L.c32, lane) &&
+                        wave_build<WDLB, true>(L.lens + hl, hd, L.dist, L.sortD, L.ltD, L.c32, lane);
+        }
 
         // ================= the wave: rounds over the block body =================
         {
@@ -2156,79 +2298,91 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             uint32_t* const tkw = toks + uni64(L.h.tk_base);
             uint32_t w_ntok = uni32(L.h.n_tok), w_outn = uni32(L.h.out_n);
             const uint32_t w_cap = uni32(L.h.tok_cap), w_mbase = uni32(L.h.member_base);
-            const uint32_t w_spec = uni32(L.h.spec), w_fseq = uni32(L.h.fixed_seq);
+            const uint32_t w_spec = uni32(L.h.spec);
             uint32_t w_counting = uni32(L.h.counting);
             const uint32_t Wwant = uni32(L.h.W);
-            uint32_t w_next = S_SYM;
+            uint32_t w_next = tables_ok ? (uint32_t)S_SYM : (uint32_t)S_FINISH;
             const uint32_t k = lane;
-            uint2* const myrec = recs + k * (kPNM + 1);
             uint16_t* const myX = L.X[k];
+            uint16_t* const myT = L.T[k];
 
-            // lane walk state
-            uint64_t bb = 0, wci = 0, D0 = 0, D1 = 0, D2 = 0;
-            uint32_t bc = 0, r = 0;
-            uint32_t ukind = 0, usym = 0, ulen = 0, udist = 0;
-#define WP_REFILL()                                                                  \
-    do {                                                                             \
-        const uint64_t _a = P + r;                                                   \
-        const uint64_t _q = _a >> 6;                                                 \
-        if (_q != wci) {                                                             \
-            uint32_t _st = 3;                                                        \
-            if (_q > wci && _q - wci < 3) _st = (uint32_t)(_q - wci);                 \
-            wci = _q;                                                                \
-            for (uint32_t _k = _st; _k > 0; --_k) {                                  \
-                D0 = D1;                                                             \
-                D1 = D2;                                                             \
-                D2 = in64[wci + 3 - _k];                                             \
-            }                                                                        \
-        }                                                                            \
-        bb = funnel_lo(D0, D1, (uint32_t)(_a & 63));                                 \
-        bc = 64;                                                                     \
+            // Lane walk state.  Positions r are bits after P.  The lane's input
+            // window D0..D2 holds the words wq..wq+2 after P's word; a unit
+            // moves r by at most 48 bits, so before each unit the window moves
+            // by at most one word, and the word two ahead is read then and
+            // first used one word later.  The loop bodies are branch-free
+            // (every lane runs every instruction, state changes are selects):
+            // with a branch per lane the compiler copies the loop-carried
+            // registers, the pending input word among them, and has to wait for
+            // it.  Only the rare long-code decode and the stores are
+            // predicated.
+            uint64_t bb = 0, D0 = 0, D1 = 0, D2 = 0;
+            uint32_t r = 0, wq = 0;
+            uint32_t ukind = 0, usym = 0, ulen = 0, udist = 0, un = 0;
+#define WP_START()                                  \
+    do {                                            \
+        const uint32_t _a = Pb + r;                 \
+        wq = _a >> 6;                               \
+        D0 = inP[wq];                               \
+        D1 = inP[wq + 1];                           \
+        D2 = inP[wq + 2];                           \
     } while (0)
-// one unit at r: a literal (ukind 0, usym), a match (1, ulen, udist), the end of
-// block (2) or an invalid code (3); consumes its bits
-#define WP_UNIT()                                                                                       \
+#define WP_WIN()                                    \
+    do {                                            \
+        const uint32_t _a = Pb + r;                 \
+        const uint32_t _q = _a >> 6;                \
+        const bool _adv = _q != wq;                 \
+        D0 = _adv ? D1 : D0;                        \
+        D1 = _adv ? D2 : D1;                        \
+        /* the old D2 is dead here: the load below  \
+           may reuse its register */                \
+        asm volatile("" : "+v"(D1) : : "memory");   \
+        if (_adv) D2 = inP[_q + 2];                 \
+        wq = _q;                                    \
+        bb = funnel_lo(D0, D1, _a & 63u);           \
+    } while (0)
+// One unit at r (in bb): a literal (ukind 0, usym), a match (1, ulen, udist),
+// the end of block (2) or an invalid code (3), un bits long.  Both the literal
+// and the match fields are computed; the distance LUT is read for every lane.
+#define WP_DEC(ACT)                                                                                     \
     do {                                                                                                \
-        if (bc < 48) WP_REFILL();                                                                       \
-        uint32_t _e = L.lit[bb & LMASK];                                                                \
-        uint32_t _l = _e & 15u, _s = _e >> 4;                                                           \
-        ukind = 3;                                                                                      \
-        bool _ok = true;                                                                                \
-        if (_l == 0) _ok = decode_long<LB>(bb, L.cntL, sortedL, L.longL, kLongLit, L.lhead[0], w_fseq, &_s, &_l); \
-        if (_ok) {                                                                                      \
-            bb >>= _l;                                                                                  \
-            bc -= _l;                                                                                   \
-            r += _l;                                                                                    \
-            if (_s < 256) {                                                                             \
-                ukind = 0;                                                                              \
-                usym = _s;                                                                              \
-            } else if (_s == 256) {                                                                     \
-                ukind = 2;                                                                              \
-            } else if (_s <= 285) {                                                                     \
-                const uint32_t _li = _s - 257, _lx = len_extra(_li);                                    \
-                ulen = len_base(_li) + (uint32_t)(bb & ((1ull << _lx) - 1));                            \
-                bb >>= _lx;                                                                             \
-                bc -= _lx;                                                                              \
-                r += _lx;                                                                               \
-                uint32_t _de = L.dist[bb & DMASK];                                                      \
-                uint32_t _dl = _de & 15u, _ds = _de >> 4;                                               \
-                bool _ok2 = true;                                                                       \
-                if (_dl == 0)                                                                           \
-                    _ok2 = decode_long<WDLB>(bb, L.cntD, sortedD, L.longD, kLongDist, L.lhead[1], 0u, &_ds, &_dl); \
-                if (_ok2 && _ds < 30) {                                                                 \
-                    bb >>= _dl;                                                                         \
-                    const uint32_t _dx = dist_extra(_ds);                                               \
-                    udist = dist_base(_ds) + (uint32_t)(bb & ((1ull << _dx) - 1));                      \
-                    bb >>= _dx;                                                                         \
-                    bc -= _dl + _dx;                                                                    \
-                    r += _dl + _dx;                                                                     \
-                    ukind = 1;                                                                          \
-                }                                                                                       \
-            }                                                                                           \
+        uint32_t _e = L.lit[(uint32_t)bb & LMASK];                                                      \
+        if ((ACT) && (_e & 15u) == 0) {                                                                 \
+            uint32_t _s, _l2;                                                                           \
+            _e = wave_long<LB>(bb, L.ltL, L.sortL, &_s, &_l2) ? lit_entry(_s, _l2) : 0u;                \
         }                                                                                               \
+        const uint32_t _l = _e & 15u;                                                                   \
+        const uint32_t _x = (_e >> 5) & 7u;                                                             \
+        const bool _oth = (_e & 16u) != 0;                                                              \
+        const bool _len = _oth && _x < 6;                                                               \
+        const uint32_t _xl = _len ? _x : 0u;                                                            \
+        ulen = (_e >> 8) + 3u + ((uint32_t)(bb >> _l) & ((1u << _xl) - 1u));                            \
+        const uint32_t _o = _l + _xl;                                                                   \
+        const uint64_t _b2 = bb >> _o;                                                                  \
+        uint32_t _d = L.dist[(uint32_t)_b2 & DMASK];                                                    \
+        if ((ACT) && _len && (_d & 15u) == 0) {                                                         \
+            uint32_t _s, _dl2;                                                                          \
+            _d = wave_long<WDLB>(_b2, L.ltD, L.sortD, &_s, &_dl2) ? dist_entry(_s, _dl2) : 0u;          \
+        }                                                                                               \
+        const uint32_t _dl = _d & 15u;                                                                  \
+        const uint32_t _dx = (_d >> 4) & 15u;                                                           \
+        udist = (_d >> 8) + ((uint32_t)(_b2 >> _dl) & ((1u << _dx) - 1u));                              \
+        usym = _e >> 8;                                                                                 \
+        un = _len ? _o + _dl + _dx : _l;                                                                \
+        ukind = _l == 0 ? 3u : !_oth ? 0u : _len ? (_dl ? 1u : 3u) : (_x == 7 ? 2u : 3u);               \
+    } while (0)
+// token counting of a walk (literal tokens close at 3 and before a match or
+// the end of block)
+#define WP_COUNT(A)                                                                                     \
+    do {                                                                                                \
+        const uint32_t _ln1 = litn + 1;                                                                 \
+        const bool _lit = ukind == 0;                                                                   \
+        tok += (A) ? (_lit ? (_ln1 == 3 ? 1u : 0u) : (litn != 0 ? 1u : 0u) + (ukind == 1 ? 1u : 0u)) : 0u; \
+        litn = (A) ? (_lit ? (_ln1 == 3 ? 0u : _ln1) : 0u) : litn;                                      \
+        r += (A) ? un : 0u;                                                                             \
     } while (0)
 
-            while (true) {
+            while (tables_ok) {
                 // the window: as asked, shrunk so that every read stays inside the input
                 const uint64_t room = nb_end > P + 160 ? (nb_end - P - 160) / 64 : 0;
                 const uint32_t Wr = (uint32_t)((uint64_t)Wwant < room ? (uint64_t)Wwant : room) & ~(kPB - 1);
@@ -2238,86 +2392,61 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 const uint32_t NMr = Wr / kPB;
                 const uint32_t RW = 64u * Wr;
+                const uint64_t* const inP = in64 + (P >> 6);
+                const uint32_t Pb = (uint32_t)(P & 63);
                 // ---- 1. own window ----
                 const uint32_t g = k * Wr;
                 r = g;
-                wci = ~0ull;
-                bc = 0;
-                uint32_t nbd = g, m = 0, tok = 0, cells = 0, litn = 0, kind = WK_WALK;
-                while (true) {
-                    if (r >= nbd) {
-                        tok += litn != 0;
-                        litn = 0;
+                WP_START();
+                uint32_t nbd = g, m = 0, tok = 0, litn = 0, kind = WK_WALK;
+                bool act = true;
+                while (__ballot(act)) {
+                    // a crossing at this unit's start: literal tokens close here
+                    const bool cr = act && r >= nbd;
+                    tok += (cr && litn != 0) ? 1u : 0u;
+                    litn = cr ? 0u : litn;
+                    if (cr && m < kPMS) {
                         myX[m] = (uint16_t)(r - g);
-                        myrec[m] = make_uint2(tok, cells);
-                        if (m == NMr) break;
-                        ++m;
-                        nbd += kPB;
+                        myT[m] = (uint16_t)tok;
                     }
-                    WP_UNIT();
-                    if (ukind == 0) {
-                        cells += 1;
-                        if (++litn == 3) {
-                            ++tok;
-                            litn = 0;
-                        }
-                    } else if (ukind == 1) {
-                        tok += (litn != 0) + 1;
-                        litn = 0;
-                        cells += ulen;
-                    } else {
-                        if (ukind == 2) tok += litn != 0;
-                        litn = 0;
-                        kind = ukind == 2 ? WK_EOB : WK_ERR;
-                        break;
-                    }
+                    act = act && !(cr && m == NMr);
+                    m += cr ? 1u : 0u;
+                    nbd += cr ? kPB : 0u;
+                    WP_WIN();
+                    WP_DEC(act);
+                    WP_COUNT(act);
+                    kind = (act && ukind >= 2) ? (ukind == 2 ? WK_EOB : WK_ERR) : kind;
+                    act = act && ukind < 2;
                 }
-                for (uint32_t mm = kind == WK_WALK ? NMr + 1 : m; mm <= NMr; ++mm) myX[mm] = 0xFFFFu;
+                // crossings this walk did not reach hold no position
+                for (uint32_t mm = kind == WK_WALK ? NMr + 1 : m; mm < kPMS; ++mm) myX[mm] = 0xFFFFu;
                 __syncthreads();
                 // ---- 2. on into the next windows until a crossing agrees ----
                 uint32_t tgt = kind == WK_EOB ? WT_EOB : kind == WK_ERR ? WT_ERR : WT_END, tm = 0;
-                if (kind == WK_WALK && k < 63) {
+                {
                     uint32_t j = k + 1, mm = 0, bnd = (k + 1) * Wr;
-                    while (true) {
-                        bool stop = false;
-                        while (r >= bnd) {
-                            if (j == 64) {
-                                tgt = WT_END;
-                                stop = true;
-                                break;
-                            }
-                            tok += litn != 0;
-                            litn = 0;
-                            if ((uint32_t)L.X[j][mm] == r - j * Wr) {
-                                tgt = j;
-                                tm = mm;
-                                stop = true;
-                                break;
-                            }
-                            bnd += kPB;
-                            if (++mm == NMr) {
-                                mm = 0;
-                                ++j;
-                            }
-                        }
-                        if (stop) break;
-                        WP_UNIT();
-                        if (ukind == 0) {
-                            cells += 1;
-                            if (++litn == 3) {
-                                ++tok;
-                                litn = 0;
-                            }
-                        } else if (ukind == 1) {
-                            tok += (litn != 0) + 1;
-                            litn = 0;
-                            cells += ulen;
-                        } else {
-                            if (ukind == 2) tok += litn != 0;
-                            litn = 0;
-                            tgt = ukind == 2 ? WT_EOB : WT_ERR;
-                            break;
-                        }
+                    act = kind == WK_WALK && k < 63;
+                    while (__ballot(act)) {
+                        const bool cr = act && r >= bnd;
+                        const bool end = cr && j == 64;
+                        const bool cr2 = cr && !end;
+                        tok += (cr2 && litn != 0) ? 1u : 0u;
+                        litn = cr2 ? 0u : litn;
+                        const uint32_t xj = (uint32_t)L.X[j < 64 ? j : 63u][mm < kPMS ? mm : 0u];
+                        const bool hit = cr2 && mm < kPMS && xj == r - j * Wr;
+                        tgt = end ? WT_END : hit ? j : tgt;
+                        tm = hit ? mm : tm;
+                        const bool mv = cr2 && !hit;
+                        bnd += mv ? kPB : 0u;
+                        const bool wrap = mv && mm + 1 == NMr;
+                        mm = mv ? (wrap ? 0u : mm + 1) : mm;
+                        j += wrap ? 1u : 0u;
+                        act = act && !end && !hit;
+                        WP_WIN();
+                        WP_DEC(act);
+                        WP_COUNT(act);
+                        tgt = (act && ukind >= 2) ? (ukind == 2 ? WT_EOB : WT_ERR) : tgt;
+                        act = act && ukind < 2;
                     }
                 }
                 const uint32_t r_end = r;  // where this lane's walk ended
@@ -2339,100 +2468,115 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const uint64_t committed = last_t == WT_ERR ? (valid & ((1ull << last) - 1)) : valid;
                 const bool com = (committed >> k) & 1ull;
                 const uint32_t start = g + (uint32_t)myX[mysm];  // the lane's range starts here (if true)
-                uint32_t ntok = 0, ncell = 0;
-                if (com) {
-                    const uint2 b0 = mysm ? myrec[mysm] : make_uint2(0u, 0u);
-                    ntok = tok - b0.x;
-                    ncell = cells - b0.y;
-                }
-                uint32_t it = ntok, ic = ncell;
+                const uint32_t ntok = com ? tok - (uint32_t)myT[mysm] : 0u;
+                uint32_t it = ntok;
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
                     const uint32_t o1 = (uint32_t)__shfl_up((int)it, d, 64);
-                    const uint32_t o2 = (uint32_t)__shfl_up((int)ic, d, 64);
-                    if ((int)k >= d) {
-                        it += o1;
-                        ic += o2;
-                    }
+                    if ((int)k >= d) it += o1;
                 }
                 const uint32_t Tr = uni32((uint32_t)__builtin_amdgcn_readlane((int)it, 63));
-                const uint32_t Cr = uni32((uint32_t)__builtin_amdgcn_readlane((int)ic, 63));
-                if ((uint64_t)w_outn + Cr >= kMaxSegCells) {
-                    w_next = S_EXIT;
-                    break;
-                }
                 if (!w_counting && (uint64_t)w_ntok + Tr > w_cap) w_counting = 1;
                 // ---- 4. the true lanes decode their ranges again and store ----
-                uint32_t bad = 0, emitted = 0;
-                if (com) {
-                    r = start;
-                    wci = ~0ull;
-                    bc = 0;
+                uint32_t bad = 0, emitted = 0, lc = 0;
+                int32_t reach = 0;  // largest distance - cells before it: how far before the range a match reads
+                {
                     const uint32_t stop_at = tgt < 64 ? tgt * Wr + (uint32_t)L.X[tgt][tm] : 0xFFFFFFFFu;
                     uint32_t nb2 = g + (mysm + 1) * kPB;
                     const uint32_t tbase = w_ntok + (it - ntok);
-                    const uint32_t cbase = w_outn + (ic - ncell);
-                    uint32_t lc = 0, ln = 0, lacc = 0;
-                    while (true) {
-                        if (r >= nb2) {
-                            if (ln) {
-                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
-                                ++emitted;
-                            }
-                            ln = 0;
-                            lacc = 0;
-                            nb2 += kPB;
-                        }
-                        if (r == stop_at || (tgt == WT_END && r >= RW)) break;
-                        WP_UNIT();
-                        if (ukind == 0) {
-                            lacc |= usym << (8 * ln);
-                            if (++ln == 3) {
-                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (3u << 24) | lacc;
-                                ++emitted;
-                                ln = 0;
-                                lacc = 0;
-                            }
-                            ++lc;
-                        } else if (ukind == 1) {
-                            if (ln) {
-                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
-                                ++emitted;
-                            }
-                            ln = 0;
-                            lacc = 0;
-                            if (!w_spec && udist > cbase + lc - w_mbase) {
-                                bad = 1;
-                                break;
-                            }
-                            if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_MATCH << 30) | ((ulen - 3) << 16) | (udist - 1);
-                            ++emitted;
-                            lc += ulen;
-                        } else if (ukind == 2) {
-                            if (ln) {
-                                if (!w_counting && tbase + emitted < w_cap) tkw[tbase + emitted] = (TOK_LIT << 30) | (ln << 24) | lacc;
-                                ++emitted;
-                            }
-                            break;
-                        } else {
-                            bad = 1;
-                            break;
-                        }
-                        if (r > RW + 64) {  // past every range end: cannot happen on a true walk
-                            bad = 1;
-                            break;
-                        }
+                    uint32_t ln = 0, lacc = 0;
+                    // tokens leave in aligned groups of four (16-byte stores)
+                    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, nq = 0;
+#define WP_PUT(t, V)                                                                                    \
+    do {                                                                                                \
+        const uint32_t _t = (t);                                                                        \
+        const uint32_t _i = tbase + emitted;                                                            \
+        const bool _single = (V) && nq == 0 && (_i & 3u) != 0;                                          \
+        const bool _enq = (V) && !_single;                                                              \
+        if (_single && !w_counting && _i < w_cap) tkw[_i] = _t;                                         \
+        q0 = _enq ? q1 : q0;                                                                            \
+        q1 = _enq ? q2 : q1;                                                                            \
+        q2 = _enq ? q3 : q2;                                                                            \
+        q3 = _enq ? _t : q3;                                                                            \
+        nq += _enq ? 1u : 0u;                                                                           \
+        const bool _full = nq == 4;                                                                     \
+        if (_full && !w_counting && _i < w_cap)                                                         \
+            *reinterpret_cast<uint4*>(tkw + _i - 3) = make_uint4(q0, q1, q2, q3);                       \
+        nq = _full ? 0u : nq;                                                                           \
+        emitted += (V) ? 1u : 0u;                                                                       \
+    } while (0)
+                    if (com) {
+                        r = start;
+                        WP_START();
                     }
-                    if (!bad && (emitted != ntok || lc != ncell)) bad = 1;
+                    act = com;
+                    while (__ballot(act)) {
+                        // a crossing: the pending literals close (token pa)
+                        const bool cr = act && r >= nb2;
+                        const bool hpa0 = cr && ln != 0;
+                        uint32_t pa = (TOK_LIT << 30) | (ln << 24) | lacc;
+                        ln = cr ? 0u : ln;
+                        lacc = cr ? 0u : lacc;
+                        nb2 += cr ? kPB : 0u;
+                        act = act && !(r == stop_at || (tgt == WT_END && r >= RW));
+                        WP_WIN();
+                        WP_DEC(act);
+                        const bool lit = act && ukind == 0, mat = act && ukind == 1, eob = act && ukind == 2;
+                        // literals close before a match or the end of block (pa; never
+                        // together with a crossing's, which left none pending)
+                        const bool fl = (mat || eob) && ln != 0;
+                        const uint32_t lacc1 = lacc | (usym << (8 * ln));
+                        const bool lit3 = lit && ln == 2;
+                        const uint32_t pb = lit3 ? (TOK_LIT << 30) | (3u << 24) | lacc1
+                                                 : (TOK_MATCH << 30) | ((ulen - 3) << 16) | (udist - 1);
+                        const bool hpb = lit3 || mat;
+                        reach = mat ? max(reach, (int32_t)udist - (int32_t)lc) : reach;
+                        lc += lit ? 1u : mat ? ulen : 0u;
+                        ln = lit ? (lit3 ? 0u : ln + 1) : (act ? 0u : ln);
+                        lacc = lit ? (lit3 ? 0u : lacc1) : (act ? 0u : lacc);
+                        bad |= (act && ukind == 3) ? 1u : 0u;
+                        r += act ? un : 0u;
+                        act = lit || mat;
+                        if (act && r > RW + 64) {  // past every range end: cannot happen on a true walk
+                            bad = 1;
+                            act = false;
+                        }
+                        WP_PUT(pa, hpa0 || fl);
+                        WP_PUT(pb, hpb);
+                    }
+                    if (nq && !w_counting) {  // the last 1-3 tokens
+                        const uint32_t b = tbase + emitted - nq;
+                        if (b < w_cap) tkw[b] = nq == 1 ? q3 : nq == 2 ? q2 : q1;
+                        if (nq > 1 && b + 1 < w_cap) tkw[b + 1] = nq == 2 ? q3 : q2;
+                        if (nq > 2 && b + 2 < w_cap) tkw[b + 2] = q3;
+                    }
+#undef WP_PUT
+                    if (com && !bad && emitted != ntok) bad = 1;
                 }
+                // cells: a prefix sum over the ranges; a back-reference before the
+                // member's start fails its range (the serial decode from that range
+                // then stops at the symbol, fast_inflate.rs:1704-1715)
+                uint32_t ic = lc;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t o2 = (uint32_t)__shfl_up((int)ic, d, 64);
+                    if ((int)k >= d) ic += o2;
+                }
+                const uint32_t Cr = uni32((uint32_t)__builtin_amdgcn_readlane((int)ic, 63));
+                const uint32_t cbase = w_outn + (ic - lc);
+                if (com && !w_spec && reach > 0 && (uint32_t)reach > cbase - w_mbase) bad = 1;
                 const uint64_t badm = __ballot(com && bad);
                 if (badm) {
                     // lane 0 decodes on from the first failing range
                     const uint32_t f = (uint32_t)__builtin_ctzll(badm);
                     w_ntok += uni32((uint32_t)__builtin_amdgcn_readlane((int)(it - ntok), (int)f));
-                    w_outn += uni32((uint32_t)__builtin_amdgcn_readlane((int)(ic - ncell), (int)f));
+                    w_outn += uni32((uint32_t)__builtin_amdgcn_readlane((int)(ic - lc), (int)f));
                     P += uni32((uint32_t)__builtin_amdgcn_readlane((int)start, (int)f));
                     w_next = S_SYM;
+                    break;
+                }
+                if ((uint64_t)w_outn + Cr >= kMaxSegCells) {
+                    w_next = S_EXIT;
                     break;
                 }
                 w_ntok += Tr;
@@ -2449,8 +2593,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 __syncthreads();  // the next round rewrites the crossing table
             }
-#undef WP_UNIT
-#undef WP_REFILL
+#undef WP_COUNT
+#undef WP_DEC
+#undef WP_WIN
+#undef WP_START
             if (lane == 0) {
                 L.h.pos = P;
                 L.h.n_tok = w_ntok;
@@ -2477,6 +2623,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             } else if (nx == S_SYM) {
                 S.force_serial = 1;  // the rest of this body: serially, with every check
                 S.state = S_SYM;
+            } else if (nx == S_FINISH) {
+                S.status = ST_INVALID;  // a table of the block is oversubscribed
+                S.detail = DET_OVERSUBSCRIBED;
+                S.state = S_FINISH;
             } else {
                 S.status = ST_TOO_BIG;
                 S.detail = DET_NONE;
