@@ -40,6 +40,15 @@ __device__ __forceinline__ uint64_t bits64_at(const uint64_t* __restrict__ in64,
     return funnel_lo(lo, hi, s);
 }
 
+// Eight consecutive cells from any cell index (gfx950 global memory takes
+// 16-byte accesses at 2-byte alignment).
+__device__ __forceinline__ uint4 load_cells8(const uint16_t* p) {
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+__device__ __forceinline__ void store_cells8(uint16_t* p, uint4 v) { __builtin_memcpy(p, &v, 16); }
+
 __device__ __forceinline__ uint32_t byte_at(const uint64_t* __restrict__ in64, uint64_t i) {
     return (uint32_t)((in64[i >> 3] >> ((i & 7) * 8)) & 0xffu);
 }
@@ -2430,8 +2439,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         const bool cr = act && r >= bnd;
                         const bool end = cr && j == 64;
                         const bool cr2 = cr && !end;
-                        tok += (cr2 && litn != 0) ? 1u : 0u;
-                        litn = cr2 ? 0u : litn;
+                        // literals close at every crossing, the round's end included,
+                        // as they do in step 4
+                        tok += (cr && litn != 0) ? 1u : 0u;
+                        litn = cr ? 0u : litn;
                         const uint32_t xj = (uint32_t)L.X[j < 64 ? j : 63u][mm < kPMS ? mm : 0u];
                         const bool hit = cr2 && mm < kPMS && xj == r - j * Wr;
                         tgt = end ? WT_END : hit ? j : tgt;
@@ -2786,7 +2797,66 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                 if (dm == ~0ull) break;
                 const uint32_t k0 = (uint32_t)__builtin_ctzll(~dm);
                 const uint32_t frontier = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k0));
+                if (!done && !longtok && need <= frontier && d >= 8) {
+                    // Cell j of the match is cell j - d: with d >= 8 a chunk of 8
+                    // reads only cells written before it.  A chunk whose source lies
+                    // wholly before the segment (markers), in the ring, or below the
+                    // ring (already stored) is copied without per-cell cases.
+                    for (uint32_t j0 = 0; j0 < L; j0 += 8) {
+                        uint32_t v[8];
+                        const uint32_t sj0 = S + j0;
+                        if (sj0 + 8 <= d) {
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) v[u] = 0x8000u | (d - sj0 - (uint32_t)u - 1u);
+                        } else if (sj0 >= d && sj0 - d >= ring_lo) {
+                            const uint32_t q = (sj0 - d) & RM;
+                            if (q + 8 <= kRing) {
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = ring[q + (uint32_t)u];
+                            } else {
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = ring[(q + (uint32_t)u) & RM];
+                            }
+                        } else if (sj0 >= d && sj0 - d + 8 <= ring_lo) {
+                            const uint32_t q = sj0 - d;
+                            if (u8) {
+                                uint2 b;
+                                __builtin_memcpy(&b, o8 + q, 8);
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = ((u < 4 ? b.x : b.y) >> (8 * (u & 3))) & 0xffu;
+                            } else {
+                                const uint4 c = load_cells8(c16 + q);
+                                const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = (cw[u >> 1] >> (16 * (u & 1))) & 0xffffu;
+                            }
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                const uint32_t sj = sj0 + (uint32_t)u;
+                                const uint32_t q = sj - d;
+                                const bool mk = sj < d;
+                                const bool rg = !mk && q >= ring_lo;
+                                uint32_t x = mk ? (0x8000u | (d - sj - 1u)) : ring[q & RM];
+                                if (!mk && !rg && j0 + (uint32_t)u < L) x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
+                                v[u] = x;
+                            }
+                        }
+                        const uint32_t t0 = sj0 & RM;
+                        if (t0 + 8 <= kRing) {
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                if (j0 + (uint32_t)u < L) ring[t0 + (uint32_t)u] = (uint16_t)v[u];
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                if (j0 + (uint32_t)u < L) ring[(sj0 + (uint32_t)u) & RM] = (uint16_t)v[u];
+                        }
+                    }
+                    done = true;
+                }
                 if (!done && !longtok && need <= frontier) {
+                    // d < 8: the source repeats with period d
                     uint32_t jm = 0;  // j mod d
                     for (uint32_t j0 = 0; j0 < L; j0 += 8) {
                         uint32_t v[8];
@@ -2924,14 +2994,6 @@ __device__ __forceinline__ bool marker_target(const ChainSeg* __restrict__ cs, u
     return true;
 }
 
-// Eight consecutive cells from any cell index (gfx950 global memory takes
-// 16-byte accesses at 2-byte alignment).
-__device__ __forceinline__ uint4 load_cells8(const uint16_t* p) {
-    uint4 v;
-    __builtin_memcpy(&v, p, 16);
-    return v;
-}
-__device__ __forceinline__ void store_cells8(uint16_t* p, uint4 v) { __builtin_memcpy(p, &v, 16); }
 
 // Two packed cells with each marker replaced by the window cell it names
 // (tab[WIN - 1 - offset]); literal cells stay.
