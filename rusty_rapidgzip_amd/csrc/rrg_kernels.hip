@@ -2726,7 +2726,9 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                                                uint32_t n_items, uint32_t* __restrict__ counter,
                                                const uint8_t* __restrict__ in8, uint16_t* __restrict__ cells,
                                                uint8_t* __restrict__ out) {
-    __shared__ __align__(16) uint16_t ring[kRing];
+    // the ring, then 8 cells that absorb the unconditional stores of cells past a
+    // token's end (a select of the address instead of a branch per cell)
+    __shared__ __align__(16) uint16_t ring[kRing + 8];
     constexpr uint32_t RM = kRing - 1;
     const int lane = (int)threadIdx.x;
     while (true) {
@@ -2777,7 +2779,8 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
             const bool longtok = L > 32 && (ty == TOK_MATCH || ty == TOK_STORED);
             bool done = !mine || ty == TOK_OFFSET || L == 0;
             if (!done && ty == TOK_LIT) {
-                for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = (uint16_t)((w >> (8 * j)) & 0xffu);
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j) ring[j < L ? ((S + j) & RM) : kRing + j] = (uint16_t)((w >> (8 * j)) & 0xffu);
                 done = true;
             } else if (!done && ty == TOK_STORED && !longtok) {
                 const uint8_t* sp = src_in + (aux & 0x3fffffffu);
@@ -2842,15 +2845,15 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                                 v[u] = x;
                             }
                         }
-                        const uint32_t t0 = sj0 & RM;
+                        const uint32_t t0 = sj0 & RM, nrem = L - j0;
                         if (t0 + 8 <= kRing) {
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
-                                if (j0 + (uint32_t)u < L) ring[t0 + (uint32_t)u] = (uint16_t)v[u];
+                                ring[(uint32_t)u < nrem ? t0 + (uint32_t)u : kRing + (uint32_t)u] = (uint16_t)v[u];
                         } else {
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
-                                if (j0 + (uint32_t)u < L) ring[(sj0 + (uint32_t)u) & RM] = (uint16_t)v[u];
+                                ring[(uint32_t)u < nrem ? ((sj0 + (uint32_t)u) & RM) : kRing + (uint32_t)u] = (uint16_t)v[u];
                         }
                     }
                     done = true;
@@ -2874,7 +2877,7 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                         }
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
-                            if (j0 + (uint32_t)u < L) ring[(S + j0 + (uint32_t)u) & RM] = (uint16_t)v[u];
+                            ring[j0 + (uint32_t)u < L ? ((S + j0 + (uint32_t)u) & RM) : kRing + (uint32_t)u] = (uint16_t)v[u];
                     }
                     done = true;
                 }
