@@ -1418,6 +1418,26 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<SegResult> res;
     std::vector<uint32_t> all(S);
     for (size_t j = 0; j < S; ++j) all[j] = (uint32_t)j;
+    if (G.g.wave) {
+        // a wave takes segments one after another: the longest first, so the
+        // launch does not end waiting on a long one started last (spans in
+        // 2 KiB buckets, largest bucket first, stable within a bucket)
+        auto span_of = [&](uint32_t j) {
+            const uint64_t end = desc[j].hint_bit != UINT64_MAX ? desc[j].hint_bit : desc[j].end_byte * 8;
+            return end > desc[j].start_bit ? (end - desc[j].start_bit) >> 14 : 0;
+        };
+        uint64_t top = 0;
+        for (uint32_t j : all) top = std::max<uint64_t>(top, span_of(j));
+        top = std::min<uint64_t>(top, 4095);
+        std::vector<uint32_t> cnt(top + 2, 0), sorted(S);
+        for (uint32_t j : all) ++cnt[top - std::min<uint64_t>(span_of(j), top)];
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        for (size_t k = S; k-- > 0;) {
+            const uint32_t j = all[k];
+            sorted[--cnt[top - std::min<uint64_t>(span_of(j), top)]] = j;
+        }
+        all.swap(sorted);
+    }
     rc = run_inflate(c, G, desc, res, tok_used, mb_used, all, err);
     if (rc) return rc;
     for (Job& J : jobs) {
