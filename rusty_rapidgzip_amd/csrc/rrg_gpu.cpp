@@ -1284,8 +1284,10 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // 3. segment starts: chunk starts (the reference's, strictly increasing)
     // and every distinct segment find.  A segment cannot be shorter than a
     // deflate block, and one that spans two blocks (its target and the next
-    // found the same block start) doubles the longest lane's work: gaps well
-    // above the median get one more target, a quarter of the way in.
+    // found the same block start) doubles the longest lane's work: with one
+    // lane per segment, gaps well above the median get one more target, a
+    // quarter of the way in.  The wavefront-per-segment geometry takes its
+    // work longest first and does without that second finder pass.
     {
         std::vector<uint64_t> gt, gl, ge;
         std::vector<std::pair<size_t, size_t>> gat;  // (job, gap index)
@@ -1308,7 +1310,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             M.erase(std::unique(M.begin(), M.end()), M.end());
             J.stream_B = B;
             J.orig_start = M;
-            if (M.size() < 16) continue;
+            if (M.size() < 16 || G.g.wave) continue;
             std::vector<uint64_t> span(M.size() - 1);
             for (size_t j = 0; j + 1 < M.size(); ++j) span[j] = M[j + 1] - M[j];
             std::nth_element(span.begin(), span.begin() + (long)(span.size() / 2), span.end());

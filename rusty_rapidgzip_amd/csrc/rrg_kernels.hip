@@ -319,6 +319,20 @@ __device__ bool deep_verify(const uint64_t* __restrict__ in64, FindWave& fw, uin
     return trial_syms(in64, fw, rbase, nbits, pos, 64) != 0;
 }
 
+// deep_verify with the REGION_WORDS words from p's word staged in LDS first:
+// the header and the trial decode are a chain of dependent reads, each an LDS
+// access instead of a trip to L2/HBM.  Reads past the region go to memory.
+__device__ __forceinline__ bool verify_in_region(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t nvalid,
+                                                 uint64_t nbits, uint64_t p, int lane, uint32_t* hdist_out) {
+    const uint64_t rb = p >> 6;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int k = lane; k < REGION_WORDS; k += 64) fw.reg[k] = rb + k < nvalid ? in64[rb + k] : 0ull;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return deep_verify(in64, fw, rb, nbits, p, lane, hdist_out);
+}
+
 // Segment targets take any true block start, so they may pass over true ones.
 // A header with at most this many distance codes constrains random bits far
 // less than the rest (a code of one to three symbols is complete almost by
@@ -576,7 +590,6 @@ __global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     constexpr uint32_t kPendBatch = 32, kPendCap = 128;
-    constexpr uint64_t kNoRegion = ~0ull;  // deep_verify reads global memory
     uint64_t result = ~0ull, weak = ~0ull;
     uint32_t npend = 0;
     bool end = false;
@@ -606,7 +619,7 @@ __global__ __launch_bounds__(64) void k_find_any(const uint64_t* __restrict__ in
                     break;
                 }
                 uint32_t hd = 0;
-                if (uni32(deep_verify(in64, fw, kNoRegion, nbits, pos, lane, &hd) ? 1u : 0u)) {
+                if (uni32(verify_in_region(in64, fw, nvalid, nbits, pos, lane, &hd) ? 1u : 0u)) {
                     if (uni32(hd) > kWeakHdist) {
                         result = pos;
                         end = true;
@@ -747,7 +760,6 @@ __global__ __launch_bounds__(64) void k_find_exact(const uint64_t* __restrict__ 
     const uint64_t stop = stops ? uni64(stops[t]) : ~0ull;
     const uint64_t nvalid = (total_bits >> 6) + 2;  // words that exist (the input is padded)
     constexpr uint32_t kPendBatch = 32, kPendCap = 128;
-    constexpr uint64_t kNoRegion = ~0ull;  // deep_verify reads global memory
     auto fetch = [&](uint64_t q) __attribute__((always_inline)) -> uint64_t { return q < nvalid ? in64[q] : 0ull; };
     uint64_t result = ~0ull;
     uint32_t npend = 0;
@@ -771,7 +783,7 @@ __global__ __launch_bounds__(64) void k_find_exact(const uint64_t* __restrict__ 
                 const uint32_t hi = uni32((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e >> 32), (int)rel));
                 const uint64_t pos = ((uint64_t)hi << 32) | lo;
                 uint32_t hd = 0;
-                if (uni32(deep_verify(in64, fw, kNoRegion, nbits, pos, lane, &hd) ? 1u : 0u)) {
+                if (uni32(verify_in_region(in64, fw, nvalid, nbits, pos, lane, &hd) ? 1u : 0u)) {
                     result = pos;
                     end = true;
                     break;
@@ -2943,7 +2955,8 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
 hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint32_t n_items, uint32_t* d_counter,
                          const uint8_t* d_in8, uint16_t* d_cells, uint8_t* d_out, int n_cu, hipStream_t s) {
     if (n_items == 0) return hipSuccess;
-    // persistent single-wave workgroups, as many as the LDS rings allow
+    // persistent single-wave workgroups (16 per CU; two waves per workgroup,
+    // 32 per CU, measured slower: the kernel is instruction-bound)
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand, 64, 0) != hipSuccess || per_cu < 1) per_cu = 1;
     const uint32_t blocks = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)n_cu, n_items);
