@@ -123,6 +123,14 @@ struct Ctx {
     PostGraphs post_graphs;   // those graphs, built once, their nodes set per batch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evc[2] = {nullptr, nullptr};  // staging copies of the chunk emission
+    // the expansion's start and end: read after a later wait, so that the
+    // host prepares the next stage while the expansion runs
+    hipEvent_t evx[2] = {nullptr, nullptr};
+    bool x_timed = false;
+    // pinned staging of uploads queued behind running kernels (handed out in
+    // order; reset only where the stream has drained)
+    uint8_t* stg = nullptr;
+    size_t stg_cap = 0, stg_used = 0;
     // every device buffer of the context
     std::vector<DevBuf*> bufs() {
         return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,
@@ -133,6 +141,9 @@ struct Ctx {
     ~Ctx() {
         for (DevBuf* b : bufs()) b->release();
         if (pin) (void)hipHostFree(pin);
+        if (stg) (void)hipHostFree(stg);
+        for (hipEvent_t e : evx)
+            if (e) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : evc)

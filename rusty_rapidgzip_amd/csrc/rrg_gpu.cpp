@@ -328,6 +328,7 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     res.resize(S);
     HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
+    c.stg_used = 0;  // the stream has drained: no staged upload is pending
     if (c.verbose) {
         // the launch's largest outputs and its status counts
         std::vector<std::pair<uint64_t, uint32_t>> big;
@@ -360,8 +361,44 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     return RRG_OK;
 }
 
+// Copy n bytes of host memory to device address d on c.s through the pinned
+// staging buffer: the call returns at once, also behind running kernels.  The
+// staged bytes stay put until the copy has run (regions are handed out in
+// order and reused only after the stream has drained).
+int stage_upload(Ctx& c, void* d, const void* h, size_t n, rrg_error* err) {
+    if (n == 0) return RRG_OK;
+    if (c.stg_used + n > c.stg_cap) {
+        // full: wait for the copies in flight, then start over (larger if need be)
+        HIPCHK(hipStreamSynchronize(c.s));
+        c.stg_used = 0;
+        if (n > c.stg_cap) {
+            if (c.stg) (void)hipHostFree(c.stg);
+            c.stg = nullptr;
+            c.stg_cap = 0;
+            const size_t cap = std::max<size_t>(2 * n, 8u << 20);
+            HIPCHK(hipHostMalloc((void**)&c.stg, cap, hipHostMallocDefault));
+            c.stg_cap = cap;
+        }
+    }
+    uint8_t* const p = c.stg + c.stg_used;
+    std::memcpy(p, h, n);
+    c.stg_used = (c.stg_used + n + 255) & ~(size_t)255;
+    HIPCHK(hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, c.s));
+    return RRG_OK;
+}
+
+// The expansion's time, once the stream has passed its end event.
+void collect_expand_time(Ctx& c) {
+    if (!c.x_timed) return;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.evx[0], c.evx[1]) == hipSuccess) c.prof.ms_expand += ms;
+    c.x_timed = false;
+}
+
 // Inflate pass 2 over `items` (tokens to cells, or to bytes of `out`), the
-// largest first so that no long expansion starts last.
+// largest first so that no long expansion starts last.  Returns without
+// waiting: the caller prepares the next stage meanwhile, and the time is
+// collected after a later wait (collect_expand_time).
 int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* err) {
     if (items.empty()) return RRG_OK;
     std::sort(items.begin(), items.end(),
@@ -370,11 +407,12 @@ int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* e
     HIPCHK(c.counter.ensure(16));
     HIPCHK(hipMemcpyAsync(c.xitems.p, items.data(), items.size() * sizeof(ExpandItem), hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
-    StageTimer t(c, &c.prof.ms_expand);
+    HIPCHK(hipEventRecord(c.evx[0], c.s));
     HIPCHK(launch_expand(c.toks.as<uint32_t>(), c.xitems.as<ExpandItem>(), (uint32_t)items.size(),
                          c.counter.as<uint32_t>(), c.in.as<uint8_t>(), c.cells.as<uint16_t>(), out.as<uint8_t>(),
                          c.n_cu, c.s));
-    t.stop();
+    HIPCHK(hipEventRecord(c.evx[1], c.s));
+    c.x_timed = true;
     for (const ExpandItem& it : items) c.prof.n_tokens += it.n_tok;
     return RRG_OK;
 }
@@ -439,22 +477,21 @@ int resolve_and_crc(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_
     // descriptors up, flags cleared
     if (n) {
         HIPCHK(c.chain.ensure(n * sizeof(ChainSeg)));
-        HIPCHK(hipMemcpyAsync(c.chain.p, cs.data(), n * sizeof(ChainSeg), hipMemcpyHostToDevice, c.s));
+        if (int rc = stage_upload(c, c.chain.p, cs.data(), n * sizeof(ChainSeg), err)) return rc;
         HIPCHK(c.err.ensure((size_t)n * 4));
         HIPCHK(hipMemsetAsync(c.err.p, 0, (size_t)n * 4, c.s));
     }
     if (tail) HIPCHK(c.tz.ensure(tail_scratch_bytes(n, G)));
     if (!tiles.empty()) {
         HIPCHK(c.tiles.ensure(tiles.size() * sizeof(ResolveTile)));
-        HIPCHK(hipMemcpyAsync(c.tiles.p, tiles.data(), tiles.size() * sizeof(ResolveTile), hipMemcpyHostToDevice,
-                              c.s));
+        if (int rc = stage_upload(c, c.tiles.p, tiles.data(), tiles.size() * sizeof(ResolveTile), err)) return rc;
     }
     if (!po.empty()) {
         HIPCHK(c.poff.ensure(po.size() * 8));
         HIPCHK(c.plen.ensure(pl.size() * 4));
         HIPCHK(c.pcrc.ensure(pc.size() * 4));
-        HIPCHK(hipMemcpyAsync(c.poff.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, c.s));
-        HIPCHK(hipMemcpyAsync(c.plen.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, c.s));
+        if (int rc = stage_upload(c, c.poff.p, po.data(), po.size() * 8, err)) return rc;
+        if (int rc = stage_upload(c, c.plen.p, pl.data(), pl.size() * 4, err)) return rc;
     }
     PostLaunch pl_;
     pl_.tail = tail;
@@ -508,6 +545,7 @@ int resolve_and_crc(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_
     if (n) HIPCHK(hipMemcpyAsync(flags.data(), c.err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.s));
     if (!po.empty()) HIPCHK(hipMemcpyAsync(pc.data(), c.pcrc.p, pc.size() * 4, hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
+    collect_expand_time(c);
     raw.assign(ranges.size(), 0);
     const uint32_t full_shift = x8n(PIECE);
     for (size_t i = 0; i < ranges.size(); ++i) {
@@ -1548,15 +1586,18 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     }
     HIPCHK(out.ensure(out_total + 64));
     HIPCHK(c.cells.ensure((cells_total + 64) * sizeof(uint16_t)));
-    rc = run_expand(c, items, out, err);
-    if (rc) return rc;
-    for (const Widen& w : windows)
-        HIPCHK(launch_widen(w.src, c.cells.as<uint16_t>() + w.cell, w.n, c.s));
+    // the member records (from the inflate pass) come back first; the host
+    // then lays out the tail chain, resolve tiles and CRC pieces while the
+    // expansion runs
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
         HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
         HIPCHK(hipStreamSynchronize(c.s));
     }
+    rc = run_expand(c, items, out, err);
+    if (rc) return rc;
+    for (const Widen& w : windows)
+        HIPCHK(launch_widen(w.src, c.cells.as<uint16_t>() + w.cell, w.n, c.s));
     // back-references before their member's start (read before the tail pass
     // rewrites markers)
     std::vector<std::vector<uint64_t>> bad(jobs.size());
@@ -1683,6 +1724,7 @@ int load_host(Ctx& c, const uint8_t* file, size_t len, rrg_error* err) {
 
 void begin_profile(Ctx& c, uint64_t len) {
     std::memset(&c.prof, 0, sizeof c.prof);
+    c.x_timed = false;
     c.verbose = false;
     c.prof.compressed_bytes = len;
 }
@@ -1759,6 +1801,7 @@ bool init_ctx(Ctx& c, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c.s) != hipSuccess ||
         hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
+        hipEventCreate(&c.evx[0]) != hipSuccess || hipEventCreate(&c.evx[1]) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc[1], hipEventDisableTiming) != hipSuccess || hipStreamCreate(&c.s2) != hipSuccess ||
         hipEventCreateWithFlags(&c.evf[0], hipEventDisableTiming) != hipSuccess ||
