@@ -390,6 +390,14 @@ class Decoder:
         if not self._ctx:
             raise DeviceError(f"cannot create a decoder context on HIP device {device}")
         self.device = device
+        self._ends_buf = None
+
+    def _ends(self, n: int):
+        """The chunk-end array the library fills, kept from call to call (a
+        fresh ctypes array of 2^20 entries costs half a millisecond to clear)."""
+        if self._ends_buf is None or len(self._ends_buf) < n:
+            self._ends_buf = (ctypes.c_uint64 * n)()
+        return self._ends_buf
 
     def close(self) -> None:
         if self._ctx:
@@ -478,7 +486,7 @@ class Decoder:
         """Device-resident decode.  Returns (d_out_ptr, out_len, chunk_ends, stats); the output
         stays in device memory owned by this decoder until its next call."""
         config = config or Config()
-        ends = (ctypes.c_uint64 * max_chunks)()
+        ends = self._ends(max_chunks)
         d_out = ctypes.c_void_p()
         out_len = ctypes.c_uint64()
         n = ctypes.c_size_t()
@@ -499,7 +507,7 @@ class Decoder:
         (rrg_gpu_decode_device_noalloc): the output goes to the caller's
         buffer d_out.  Returns (out_len, chunk_ends, stats)."""
         config = config or Config()
-        ends = (ctypes.c_uint64 * max_chunks)()
+        ends = self._ends(max_chunks)
         out_len = ctypes.c_uint64()
         n = ctypes.c_size_t()
         st, er = _Stats(), _Err()

@@ -131,6 +131,7 @@ struct Ctx {
     // order; reset only where the stream has drained)
     uint8_t* stg = nullptr;
     size_t stg_cap = 0, stg_used = 0;
+    bool no_alloc = false;  // a call that must not allocate: no staging buffer is made or grown
     // every device buffer of the context
     std::vector<DevBuf*> bufs() {
         return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,

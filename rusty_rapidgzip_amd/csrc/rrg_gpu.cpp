@@ -291,6 +291,45 @@ void grow_segment(SegDesc& d, uint32_t status, uint64_t& tok_used, uint64_t& mb_
 
 inline bool overflowed(uint32_t st) { return st == ST_OVERFLOW || st == ST_MB_OVERFLOW; }
 
+// Copy n bytes of host memory to device address d on c.s through the pinned
+// staging buffer: the call returns at once, also behind running kernels.  The
+// staged bytes stay put until the copy has run (regions are handed out in
+// order and reused only after the stream has drained).
+int stage_upload(Ctx& c, void* d, const void* h, size_t n, rrg_error* err) {
+    if (n == 0) return RRG_OK;
+    if (c.no_alloc && n > c.stg_cap) {
+        // no staging buffer may be made here: a plain copy
+        HIPCHK(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, c.s));
+        return RRG_OK;
+    }
+    if (c.stg_used + n > c.stg_cap) {
+        // full: wait for the copies in flight, then start over (larger if need be)
+        HIPCHK(hipStreamSynchronize(c.s));
+        c.stg_used = 0;
+        if (n > c.stg_cap) {
+            if (c.stg) (void)hipHostFree(c.stg);
+            c.stg = nullptr;
+            c.stg_cap = 0;
+            const size_t cap = std::max<size_t>(2 * n, 8u << 20);
+            HIPCHK(hipHostMalloc((void**)&c.stg, cap, hipHostMallocDefault));
+            c.stg_cap = cap;
+        }
+    }
+    uint8_t* const p = c.stg + c.stg_used;
+    std::memcpy(p, h, n);
+    c.stg_used = (c.stg_used + n + 255) & ~(size_t)255;
+    HIPCHK(hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, c.s));
+    return RRG_OK;
+}
+
+// The expansion's time, once the stream has passed its end event.
+void collect_expand_time(Ctx& c) {
+    if (!c.x_timed) return;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.evx[0], c.evx[1]) == hipSuccess) c.prof.ms_expand += ms;
+    c.x_timed = false;
+}
+
 // One launch of inflate pass 1 over the segment indices in `work`; results
 // for every segment are copied back into `res`.
 int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vector<SegResult>& res,
@@ -307,8 +346,8 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     HIPCHK(c.res.grow_keep(S * sizeof(SegResult), c.res.cap, c.s));
     HIPCHK(c.work.ensure(work.size() * sizeof(uint32_t)));
     HIPCHK(c.counter.ensure(16));
-    HIPCHK(hipMemcpyAsync(c.segs.p, desc.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice, c.s));
-    HIPCHK(hipMemcpyAsync(c.work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c.s));
+    if (int rc = stage_upload(c, c.segs.p, desc.data(), S * sizeof(SegDesc), err)) return rc;
+    if (int rc = stage_upload(c, c.work.p, work.data(), work.size() * sizeof(uint32_t), err)) return rc;
     HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
     const double before = c.prof.ms_inflate;
     {
@@ -361,51 +400,29 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     return RRG_OK;
 }
 
-// Copy n bytes of host memory to device address d on c.s through the pinned
-// staging buffer: the call returns at once, also behind running kernels.  The
-// staged bytes stay put until the copy has run (regions are handed out in
-// order and reused only after the stream has drained).
-int stage_upload(Ctx& c, void* d, const void* h, size_t n, rrg_error* err) {
-    if (n == 0) return RRG_OK;
-    if (c.stg_used + n > c.stg_cap) {
-        // full: wait for the copies in flight, then start over (larger if need be)
-        HIPCHK(hipStreamSynchronize(c.s));
-        c.stg_used = 0;
-        if (n > c.stg_cap) {
-            if (c.stg) (void)hipHostFree(c.stg);
-            c.stg = nullptr;
-            c.stg_cap = 0;
-            const size_t cap = std::max<size_t>(2 * n, 8u << 20);
-            HIPCHK(hipHostMalloc((void**)&c.stg, cap, hipHostMallocDefault));
-            c.stg_cap = cap;
-        }
-    }
-    uint8_t* const p = c.stg + c.stg_used;
-    std::memcpy(p, h, n);
-    c.stg_used = (c.stg_used + n + 255) & ~(size_t)255;
-    HIPCHK(hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, c.s));
-    return RRG_OK;
-}
-
-// The expansion's time, once the stream has passed its end event.
-void collect_expand_time(Ctx& c) {
-    if (!c.x_timed) return;
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, c.evx[0], c.evx[1]) == hipSuccess) c.prof.ms_expand += ms;
-    c.x_timed = false;
-}
-
 // Inflate pass 2 over `items` (tokens to cells, or to bytes of `out`), the
 // largest first so that no long expansion starts last.  Returns without
 // waiting: the caller prepares the next stage meanwhile, and the time is
 // collected after a later wait (collect_expand_time).
 int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* err) {
     if (items.empty()) return RRG_OK;
-    std::sort(items.begin(), items.end(),
-              [](const ExpandItem& a, const ExpandItem& b) { return a.n_cells > b.n_cells; });
+    {
+        // largest first, by 4 Ki-cell buckets (a counting sort: the items are
+        // many and their sizes close)
+        uint64_t top = 0;
+        for (const ExpandItem& it : items) top = std::max<uint64_t>(top, it.n_cells >> 12);
+        top = std::min<uint64_t>(top, 1u << 16);
+        auto key = [&](const ExpandItem& it) { return (size_t)(top - std::min<uint64_t>(it.n_cells >> 12, top)); };
+        std::vector<uint32_t> cnt(top + 2, 0);
+        for (const ExpandItem& it : items) ++cnt[key(it) + 1];
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        std::vector<ExpandItem> by(items.size());
+        for (const ExpandItem& it : items) by[cnt[key(it)]++] = it;
+        items.swap(by);
+    }
     HIPCHK(c.xitems.ensure(items.size() * sizeof(ExpandItem)));
     HIPCHK(c.counter.ensure(16));
-    HIPCHK(hipMemcpyAsync(c.xitems.p, items.data(), items.size() * sizeof(ExpandItem), hipMemcpyHostToDevice, c.s));
+    if (int rc = stage_upload(c, c.xitems.p, items.data(), items.size() * sizeof(ExpandItem), err)) return rc;
     HIPCHK(hipMemsetAsync(c.counter.p, 0, 16, c.s));
     HIPCHK(hipEventRecord(c.evx[0], c.s));
     HIPCHK(launch_expand(c.toks.as<uint32_t>(), c.xitems.as<ExpandItem>(), (uint32_t)items.size(),
@@ -1008,12 +1025,12 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
     HIPCHK(c.limits.ensure(n * 8));
     HIPCHK(c.scan_ends.ensure(n * 8));
     HIPCHK(c.found.ensure(n * 8));
-    HIPCHK(hipMemcpyAsync(c.targets.p, targets.data(), n * 8, hipMemcpyHostToDevice, c.s));
-    HIPCHK(hipMemcpyAsync(c.limits.p, limits.data(), n * 8, hipMemcpyHostToDevice, c.s));
-    HIPCHK(hipMemcpyAsync(c.scan_ends.p, scan_ends.data(), n * 8, hipMemcpyHostToDevice, c.s));
+    if (int rc = stage_upload(c, c.targets.p, targets.data(), n * 8, err)) return rc;
+    if (int rc = stage_upload(c, c.limits.p, limits.data(), n * 8, err)) return rc;
+    if (int rc = stage_upload(c, c.scan_ends.p, scan_ends.data(), n * 8, err)) return rc;
     if (stops) {
         HIPCHK(c.stops.ensure(n * 8));
-        HIPCHK(hipMemcpyAsync(c.stops.p, stops->data(), n * 8, hipMemcpyHostToDevice, c.s));
+        if (int rc = stage_upload(c, c.stops.p, stops->data(), n * 8, err)) return rc;
     }
     if (exits) HIPCHK(c.exits.ensure(n * 8));
     if (want_recs) HIPCHK(c.recs.ensure(n * 32));
@@ -1022,7 +1039,7 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
         if (!(limits[i] & kAnyStart)) exact.push_back((uint32_t)i);
     if (!exact.empty()) {
         HIPCHK(c.exact_idx.ensure(exact.size() * 4));
-        HIPCHK(hipMemcpyAsync(c.exact_idx.p, exact.data(), exact.size() * 4, hipMemcpyHostToDevice, c.s));
+        if (int rc = stage_upload(c, c.exact_idx.p, exact.data(), exact.size() * 4, err)) return rc;
     }
     StageTimer t(c, &c.prof.ms_find);
     HIPCHK(launch_find_blocks(c.in.as<uint64_t>(), (uint64_t)c.in_used * 8, c.targets.as<uint64_t>(),
@@ -1038,6 +1055,7 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
     }
     t.stop();
     HIPCHK(hipStreamSynchronize(c.s));
+    c.stg_used = 0;  // the stream has drained: no staged upload is pending
     if (c.verbose) {
         uint64_t longest = 0, none = 0;
         for (size_t i = 0; i < n; ++i) {

@@ -95,8 +95,10 @@ extern "C" int rrg_gpu_decode_device_noalloc(void* ctx, const void* d_file, size
     c.out.fixed = true;
     const uint64_t lanes_before = c.max_lanes;
     c.max_lanes = kLanes;
+    c.no_alloc = true;  // uploads use the pinned staging buffer only if one exists already
     int rc = decode_device_impl(c, d_file, len, h_file, cfg, out_len, chunk_ends, chunk_cap, n_chunks, stats, err);
     (void)hipStreamSynchronize(c.s);  // no kernel of this call still uses the array
+    c.no_alloc = false;
     c.max_lanes = lanes_before;
     for (size_t i = 0; i < bufs.size(); ++i) *bufs[i] = saved[i];
     if (rc == RRG_E_DEVICE && arena.fixed_short) {
