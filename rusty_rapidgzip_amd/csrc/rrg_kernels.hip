@@ -133,15 +133,18 @@ __device__ __forceinline__ uint64_t ubits(const uint64_t* __restrict__ in64, con
 
 // Canonical decode over a length-sorted symbol table (RFC 1951 3.2.2), with
 // the reference's lookup rules (10 bits available for the LUT probe, 15 for
-// codes longer than 10).  Returns the symbol or -1.
-__device__ int ucanon(const uint64_t* __restrict__ in64, const FindWave& fw, uint64_t rbase, uint64_t nbits,
-                      uint64_t& pos, const uint16_t* cnt, const uint16_t* sorted) {
+// codes longer than 10).  `cnt` holds the code-length counts in registers
+// (wave-uniform).  Returns the symbol or -1.
+__device__ __forceinline__ int ucanon(const uint64_t* __restrict__ in64, const FindWave& fw, uint64_t rbase,
+                                      uint64_t nbits, uint64_t& pos, const uint32_t (&cnt)[16],
+                                      const uint16_t* sorted) {
     if (pos + 10 > nbits) return -1;
     const uint64_t w = ubits(in64, fw, rbase, pos);
     int code = 0, first = 0, index = 0;
+#pragma unroll
     for (int l = 1; l <= 15; ++l) {
         code |= (int)((w >> (l - 1)) & 1u);
-        const int count = (int)uni32(cnt[l]);
+        const int count = (int)cnt[l];
         if (code - count < first) {
             if (l > 10 && pos + 15 > nbits) return -1;
             pos += (uint64_t)l;
@@ -156,15 +159,43 @@ __device__ int ucanon(const uint64_t* __restrict__ in64, const FindWave& fw, uin
 }
 
 // counts + length-sorted symbols of lens[0..n) into cnt[0..16) / sorted[0..n)
-__device__ void sort_lengths(const uint8_t* lens, int n, uint16_t* cnt, uint16_t* sorted) {
-    for (int l = 0; l < 16; ++l) cnt[l] = 0;
-    for (int s = 0; s < n; ++s) cnt[lens[s]]++;
-    cnt[0] = 0;
-    uint16_t off[16];
-    off[1] = 0;
-    for (int l = 1; l < 15; ++l) off[l + 1] = (uint16_t)(off[l] + cnt[l]);
-    for (int s = 0; s < n; ++s)
-        if (lens[s]) sorted[off[lens[s]]++] = (uint16_t)s;
+// (symbol order within a length), by the whole wave: the counts by ballot per
+// length, each symbol's place by its rank among the symbols of its length.
+// Lane l (1..15) keeps length l's count and running offset.
+__device__ void sort_lengths(const uint8_t* lens, int n, uint16_t* cnt, uint16_t* sorted, int lane) {
+    const bool owner = lane >= 1 && lane < 16;
+    uint32_t my_cnt = 0;
+    for (int b = 0; b < n; b += 64) {
+        const int sy = b + lane;
+        const uint32_t L = sy < n ? lens[sy] : 0u;
+        for (int l = 1; l < 16; ++l) {
+            const uint32_t c = (uint32_t)__popcll(__ballot(L == (uint32_t)l));
+            if (lane == l) my_cnt += c;
+        }
+    }
+    // length l's first place: the counts of lengths 1..l-1
+    uint32_t incl = owner ? my_cnt : 0u;
+#pragma unroll
+    for (int dd = 1; dd < 16; dd <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)incl, dd, 64);
+        if (lane >= dd) incl += o;
+    }
+    uint32_t off = owner ? incl - my_cnt : 0u;
+    if (lane < 16) cnt[lane] = (uint16_t)(owner ? my_cnt : 0u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int b = 0; b < n; b += 64) {
+        const int sy = b + lane;
+        const uint32_t L = sy < n ? lens[sy] : 0u;
+        uint32_t place = 0;
+        for (int l = 1; l < 16; ++l) {
+            const uint64_t m = __ballot(L == (uint32_t)l);
+            if (!m) continue;
+            const uint32_t base = uni32((uint32_t)__builtin_amdgcn_readlane((int)off, l));
+            if (L == (uint32_t)l) place = base + (uint32_t)__popcll(m & below);
+            if (lane == l) off += (uint32_t)__popcll(m);
+        }
+        if (L) sorted[place] = (uint16_t)sy;
+    }
 }
 
 // Strict dynamic header (block_finder.rs:196-262) at `p`, the block's first
@@ -276,10 +307,8 @@ __device__ bool strict_header(const uint64_t* __restrict__ in64, FindWave& fw, u
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (uni32(fw.lens[256]) == 0) return false;
     if ((nL >= 2 && sumL != 32768u) || (nD >= 2 && sumD != 32768u)) return false;
-    if (lane == 0) {
-        sort_lengths(fw.lens, (int)hlit, fw.cnt, fw.sorted);
-        sort_lengths(fw.lens + hlit, (int)hdist, fw.cnt + 16, fw.sorted + 288);
-    }
+    sort_lengths(fw.lens, (int)hlit, fw.cnt, fw.sorted, lane);
+    sort_lengths(fw.lens + hlit, (int)hdist, fw.cnt + 16, fw.sorted + 288, lane);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     *after = pos;
@@ -292,8 +321,14 @@ __device__ bool strict_header(const uint64_t* __restrict__ in64, FindWave& fw, u
 // 1 = end of block (pos just after it), 2 = max_syms decoded.
 __device__ int trial_syms(const uint64_t* __restrict__ in64, FindWave& fw, uint64_t rbase, uint64_t nbits,
                           uint64_t& pos, int max_syms) {
+    uint32_t cL[16], cD[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        cL[l] = uni32(fw.cnt[l]);
+        cD[l] = uni32(fw.cnt[16 + l]);
+    }
     for (int k = 0; k < max_syms; ++k) {
-        const int sym = ucanon(in64, fw, rbase, nbits, pos, fw.cnt, fw.sorted);
+        const int sym = ucanon(in64, fw, rbase, nbits, pos, cL, fw.sorted);
         if (sym < 0) return 0;
         if (sym <= 255) continue;
         if (sym == 256) return 1;
@@ -301,7 +336,7 @@ __device__ int trial_syms(const uint64_t* __restrict__ in64, FindWave& fw, uint6
         const uint32_t lx = len_extra((uint32_t)sym - 257);
         if (pos + lx > nbits) return 0;
         pos += lx;
-        const int ds = ucanon(in64, fw, rbase, nbits, pos, fw.cnt + 16, fw.sorted + 288);
+        const int ds = ucanon(in64, fw, rbase, nbits, pos, cD, fw.sorted + 288);
         if (ds < 0 || ds >= 30) return 0;
         const uint32_t dx = dist_extra((uint32_t)ds);
         if (pos + dx > nbits) return 0;
