@@ -1882,11 +1882,17 @@ struct WaveLds {
     uint8_t cl[32];          // precode lengths
     uint32_t c32[16];        // code-length counts (table build)
     uint16_t cnt[16], tnext[16], toffs[16];  // precode table build
-    // per lane, for its window's first kPMS crossings: the first symbol start at
-    // or past the crossing (from the window start; 0xFFFF: not reached), and
-    // the tokens its walk produced before it
-    uint16_t X[64][kPMS];
-    uint16_t T[64][kPMS];
+    union {
+        // per lane, for its window's first kPMS crossings: the first symbol start
+        // at or past the crossing (from the window start; 0xFFFF: not reached),
+        // and the tokens its walk produced before it
+        struct {
+            uint16_t X[64][kPMS];
+            uint16_t T[64][kPMS];
+        };
+        // once a round's ranges are known: each lane's tokens on their way to memory
+        uint32_t TB[64][kPMS];
+    };
     WaveHand h;
     SerialState ss;
 };
@@ -2543,25 +2549,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     uint32_t nb2 = g + (mysm + 1) * kPB;
                     const uint32_t tbase = w_ntok + (it - ntok);
                     uint32_t ln = 0, lacc = 0;
-                    // tokens leave in aligned groups of four (16-byte stores)
-                    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, nq = 0;
-#define WP_PUT(t, V)                                                                                    \
-    do {                                                                                                \
-        const uint32_t _t = (t);                                                                        \
-        const uint32_t _i = tbase + emitted;                                                            \
-        const bool _single = (V) && nq == 0 && (_i & 3u) != 0;                                          \
-        const bool _enq = (V) && !_single;                                                              \
-        if (_single && !w_counting && _i < w_cap) tkw[_i] = _t;                                         \
-        q0 = _enq ? q1 : q0;                                                                            \
-        q1 = _enq ? q2 : q1;                                                                            \
-        q2 = _enq ? q3 : q2;                                                                            \
-        q3 = _enq ? _t : q3;                                                                            \
-        nq += _enq ? 1u : 0u;                                                                           \
-        const bool _full = nq == 4;                                                                     \
-        if (_full && !w_counting && _i < w_cap)                                                         \
-            *reinterpret_cast<uint4*>(tkw + _i - 3) = make_uint4(q0, q1, q2, q3);                       \
-        nq = _full ? 0u : nq;                                                                           \
-        emitted += (V) ? 1u : 0u;                                                                       \
+                    // The crossing table has been read; its space now buffers each
+                    // lane's tokens (at most 15).  They go to memory together, in
+                    // 16-byte groups, when some lane holds 12 or more: stores in
+                    // bursts instead of one every few units, so that the input
+                    // reads of the walk seldom wait behind them.
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    uint32_t* const tbuf = L.TB[k];
+                    uint32_t nbf = 0, flushed = 0;  // tokens in the buffer; tokens stored
+#define WP_PUT(t, V)                 \
+    do {                             \
+        tbuf[nbf & 15u] = (t);       \
+        nbf += (V) ? 1u : 0u;        \
+        emitted += (V) ? 1u : 0u;    \
+    } while (0)
+// stores the buffered tokens: single tokens up to a 16-byte boundary, then
+// groups of four; fewer than four stay behind unless ALL
+#define WP_FLUSH(ALL)                                                                      \
+    do {                                                                                   \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                             \
+        __builtin_amdgcn_wave_barrier();                                                   \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                             \
+        const uint32_t _i0 = tbase + flushed;                                              \
+        const uint32_t _al = min((4u - (_i0 & 3u)) & 3u, nbf);                             \
+        uint32_t _j = 0;                                                                   \
+        for (; _j < _al; ++_j)                                                             \
+            if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[_j];                 \
+        for (; _j + 4 <= nbf; _j += 4) {                                                   \
+            const uint4 _v = make_uint4(tbuf[_j], tbuf[_j + 1], tbuf[_j + 2], tbuf[_j + 3]); \
+            if (!w_counting && _i0 + _j + 3 < w_cap) *reinterpret_cast<uint4*>(tkw + _i0 + _j) = _v; \
+        }                                                                                  \
+        if (ALL)                                                                           \
+            for (; _j < nbf; ++_j)                                                         \
+                if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[_j];             \
+        const uint32_t _rest = nbf - _j;                                                   \
+        for (uint32_t _m = 0; _m < _rest; ++_m) tbuf[_m] = tbuf[_j + _m];                  \
+        flushed += _j;                                                                     \
+        nbf = _rest;                                                                       \
     } while (0)
                     if (com) {
                         r = start;
@@ -2601,13 +2627,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         }
                         WP_PUT(pa, hpa0 || fl);
                         WP_PUT(pb, hpb);
+                        if (__ballot(nbf >= 12u)) WP_FLUSH(false);
                     }
-                    if (nq && !w_counting) {  // the last 1-3 tokens
-                        const uint32_t b = tbase + emitted - nq;
-                        if (b < w_cap) tkw[b] = nq == 1 ? q3 : nq == 2 ? q2 : q1;
-                        if (nq > 1 && b + 1 < w_cap) tkw[b + 1] = nq == 2 ? q3 : q2;
-                        if (nq > 2 && b + 2 < w_cap) tkw[b + 2] = q3;
-                    }
+                    WP_FLUSH(true);
+#undef WP_FLUSH
 #undef WP_PUT
                     if (com && !bad && emitted != ntok) bad = 1;
                 }
