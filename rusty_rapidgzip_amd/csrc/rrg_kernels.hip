@@ -2551,7 +2551,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     uint32_t ln = 0, lacc = 0;
                     // The crossing table has been read; its space now buffers each
                     // lane's tokens (at most 15).  They go to memory together, in
-                    // 16-byte groups, when some lane holds 12 or more: stores in
+                    // 16-byte groups, when some lane holds 14 or more: stores in
                     // bursts instead of one every few units, so that the input
                     // reads of the walk seldom wait behind them.
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2627,7 +2627,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         }
                         WP_PUT(pa, hpa0 || fl);
                         WP_PUT(pb, hpb);
-                        if (__ballot(nbf >= 12u)) WP_FLUSH(false);
+                        if (__ballot(nbf >= 14u)) WP_FLUSH(false);  // a unit adds at most 2: never past 15
                     }
                     WP_FLUSH(true);
 #undef WP_FLUSH
