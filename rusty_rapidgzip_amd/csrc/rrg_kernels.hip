@@ -2362,7 +2362,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const uint32_t w_cap = uni32(L.h.tok_cap), w_mbase = uni32(L.h.member_base);
             const uint32_t w_spec = uni32(L.h.spec);
             uint32_t w_counting = uni32(L.h.counting);
-            const uint32_t Wwant = uni32(L.h.W);
+            uint32_t Wwant = uni32(L.h.W);
             uint32_t w_next = tables_ok ? (uint32_t)S_SYM : (uint32_t)S_FINISH;
             const uint32_t k = lane;
             uint16_t* const myX = L.X[k];
@@ -2672,6 +2672,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     w_next = S_BLOCK_END;
                     break;
                 }
+                // the body goes on past this round: the windows were sized from
+                // the body before, which may have been much shorter
+                Wwant = min(2u * Wwant, kPWMax);
                 __syncthreads();  // the next round rewrites the crossing table
             }
 #undef WP_COUNT
