@@ -635,12 +635,14 @@ void make_targets(const Job& J, const Geometry& G, std::vector<uint64_t>& target
     uint64_t seg_bytes = J.segment_bytes;
     if (!seg_bytes) {
         // about one segment per lane in flight (a lane decodes one segment in
-        // roughly the time the launch takes), between 24 and 64 KiB: shorter
-        // segments cost finder work (a segment cannot be shorter than a
-        // deflate block, so targets closer than the blocks are wasted scans),
-        // longer ones idle lanes
+        // roughly the time the launch takes; four per wavefront of the
+        // wave-per-segment geometry), between 24 and 64 KiB (96 KiB for that
+        // geometry, whose wavefronts take their segments from one queue):
+        // shorter segments cost finder work (a segment cannot be shorter than
+        // a deflate block, so targets closer than the blocks are wasted
+        // scans), longer ones idle lanes
         seg_bytes = (total / 8) / std::max<uint64_t>(1, G.lanes_in_flight);
-        seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 24 * 1024), 64 * 1024);
+        seg_bytes = std::min<uint64_t>(std::max<uint64_t>(seg_bytes, 24 * 1024), (G.g.wave ? 96 : 64) * 1024);
     }
     const uint64_t D = std::max<uint64_t>(1, chunk_bits / (seg_bytes * 8));
     const uint64_t seg_bits = J.serial ? seg_bytes * 8 : chunk_bits / D;
