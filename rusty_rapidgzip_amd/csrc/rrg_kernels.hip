@@ -1885,13 +1885,17 @@ struct WaveLds {
     union {
         // per lane, for its window's first kPMS crossings: the first symbol start
         // at or past the crossing (from the window start; 0xFFFF: not reached),
-        // and the tokens its walk produced before it
+        // and the tokens its walk produced before it; crossing-major (crossing
+        // m of lane k at X[m][k]) so that the lanes' records of one crossing
+        // fall in distinct banks
         struct {
-            uint16_t X[64][kPMS];
-            uint16_t T[64][kPMS];
+            uint16_t X[kPMS][64];
+            uint16_t T[kPMS][64];
         };
-        // once a round's ranges are known: each lane's tokens on their way to memory
-        uint32_t TB[64][kPMS];
+        // once a round's ranges are known: each lane's tokens on their way to
+        // memory, slot-major (slot j of lane k at TB[j][k]: the lanes' accesses
+        // to one slot fall in distinct banks)
+        uint32_t TB[kPMS][64];
     };
     WaveHand h;
     SerialState ss;
@@ -2365,8 +2369,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             uint32_t Wwant = uni32(L.h.W);
             uint32_t w_next = tables_ok ? (uint32_t)S_SYM : (uint32_t)S_FINISH;
             const uint32_t k = lane;
-            uint16_t* const myX = L.X[k];
-            uint16_t* const myT = L.T[k];
+            uint16_t* const myX = &L.X[0][k];  // crossing m at myX[64 * m]
+            uint16_t* const myT = &L.T[0][k];
 
             // Lane walk state.  Positions r are bits after P.  The lane's input
             // window D0..D2 holds the words wq..wq+2 after P's word; a unit
@@ -2468,8 +2472,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     tok += (cr && litn != 0) ? 1u : 0u;
                     litn = cr ? 0u : litn;
                     if (cr && m < kPMS) {
-                        myX[m] = (uint16_t)(r - g);
-                        myT[m] = (uint16_t)tok;
+                        myX[64u * m] = (uint16_t)(r - g);
+                        myT[64u * m] = (uint16_t)tok;
                     }
                     act = act && !(cr && m == NMr);
                     m += cr ? 1u : 0u;
@@ -2481,7 +2485,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     act = act && ukind < 2;
                 }
                 // crossings this walk did not reach hold no position
-                for (uint32_t mm = kind == WK_WALK ? NMr + 1 : m; mm < kPMS; ++mm) myX[mm] = 0xFFFFu;
+                for (uint32_t mm = kind == WK_WALK ? NMr + 1 : m; mm < kPMS; ++mm) myX[64u * mm] = 0xFFFFu;
                 __syncthreads();
                 // ---- 2. on into the next windows until a crossing agrees ----
                 uint32_t tgt = kind == WK_EOB ? WT_EOB : kind == WK_ERR ? WT_ERR : WT_END, tm = 0;
@@ -2496,7 +2500,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         // as they do in step 4
                         tok += (cr && litn != 0) ? 1u : 0u;
                         litn = cr ? 0u : litn;
-                        const uint32_t xj = (uint32_t)L.X[j < 64 ? j : 63u][mm < kPMS ? mm : 0u];
+                        const uint32_t xj = (uint32_t)L.X[mm < kPMS ? mm : 0u][j < 64 ? j : 63u];
                         const bool hit = cr2 && mm < kPMS && xj == r - j * Wr;
                         tgt = end ? WT_END : hit ? j : tgt;
                         tm = hit ? mm : tm;
@@ -2531,8 +2535,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 const uint64_t committed = last_t == WT_ERR ? (valid & ((1ull << last) - 1)) : valid;
                 const bool com = (committed >> k) & 1ull;
-                const uint32_t start = g + (uint32_t)myX[mysm];  // the lane's range starts here (if true)
-                const uint32_t ntok = com ? tok - (uint32_t)myT[mysm] : 0u;
+                const uint32_t start = g + (uint32_t)myX[64u * mysm];  // the lane's range starts here (if true)
+                const uint32_t ntok = com ? tok - (uint32_t)myT[64u * mysm] : 0u;
                 uint32_t it = ntok;
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
@@ -2545,7 +2549,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 uint32_t bad = 0, emitted = 0, lc = 0;
                 int32_t reach = 0;  // largest distance - cells before it: how far before the range a match reads
                 {
-                    const uint32_t stop_at = tgt < 64 ? tgt * Wr + (uint32_t)L.X[tgt][tm] : 0xFFFFFFFFu;
+                    const uint32_t stop_at = tgt < 64 ? tgt * Wr + (uint32_t)L.X[tm][tgt] : 0xFFFFFFFFu;
                     uint32_t nb2 = g + (mysm + 1) * kPB;
                     const uint32_t tbase = w_ntok + (it - ntok);
                     uint32_t ln = 0, lacc = 0;
@@ -2557,11 +2561,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    uint32_t* const tbuf = L.TB[k];
+                    uint32_t* const tbuf = &L.TB[0][k];  // slot j at tbuf[64 * j]
                     uint32_t nbf = 0, flushed = 0;  // tokens in the buffer; tokens stored
 #define WP_PUT(t, V)                 \
     do {                             \
-        tbuf[nbf & 15u] = (t);       \
+        tbuf[64u * (nbf & 15u)] = (t); \
         nbf += (V) ? 1u : 0u;        \
         emitted += (V) ? 1u : 0u;    \
     } while (0)
@@ -2576,16 +2580,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint32_t _al = min((4u - (_i0 & 3u)) & 3u, nbf);                             \
         uint32_t _j = 0;                                                                   \
         for (; _j < _al; ++_j)                                                             \
-            if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[_j];                 \
+            if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[64u * _j];           \
         for (; _j + 4 <= nbf; _j += 4) {                                                   \
-            const uint4 _v = make_uint4(tbuf[_j], tbuf[_j + 1], tbuf[_j + 2], tbuf[_j + 3]); \
+            const uint4 _v = make_uint4(tbuf[64u * _j], tbuf[64u * (_j + 1)], tbuf[64u * (_j + 2)], \
+                                        tbuf[64u * (_j + 3)]);                             \
             if (!w_counting && _i0 + _j + 3 < w_cap) *reinterpret_cast<uint4*>(tkw + _i0 + _j) = _v; \
         }                                                                                  \
         if (ALL)                                                                           \
             for (; _j < nbf; ++_j)                                                         \
-                if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[_j];             \
+                if (!w_counting && _i0 + _j < w_cap) tkw[_i0 + _j] = tbuf[64u * _j];       \
         const uint32_t _rest = nbf - _j;                                                   \
-        for (uint32_t _m = 0; _m < _rest; ++_m) tbuf[_m] = tbuf[_j + _m];                  \
+        for (uint32_t _m = 0; _m < _rest; ++_m) tbuf[64u * _m] = tbuf[64u * (_j + _m)];    \
         flushed += _j;                                                                     \
         nbf = _rest;                                                                       \
     } while (0)
