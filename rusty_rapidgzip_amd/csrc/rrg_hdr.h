@@ -21,11 +21,17 @@ namespace rrg {
 
 // the 7 bits of x, reversed (the first stream bit becomes the most significant)
 RRG_HD uint32_t rev7(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bitreverse32(x) >> 25;
+#else
     x = ((x & 0x55u) << 1) | ((x >> 1) & 0x55u);
     x = ((x & 0x33u) << 2) | ((x >> 2) & 0x33u);
     x = ((x & 0x0fu) << 4) | ((x >> 4) & 0x0fu);
     return (x & 0xffu) >> 1;
+#endif
 }
+
+RRG_HD uint32_t popcount32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
 
 // Cheap stage (block_finder.rs:196-213): header fields + strict precode Kraft.
 // `lo`/`hi` hold the 128 bits at p; branch-free over the 19 possible fields.
@@ -120,14 +126,20 @@ RRG_HD bool strict_header_lane(const Fetch& fetch, uint64_t nbits, uint64_t p) {
     if (!(nsym == 1 || (nsym >= 2 && kraft == 128u))) return false;
     // canonical code (RFC 1951 3.2.2), left-justified to 7 bits: a code of
     // length l is below lim[l] and not below lim[l - 1]; its index in the
-    // length-sorted symbol list is base[l] + (code >> (7 - l)) (mod 256)
-    uint64_t lim = 0, base = 0, nxt = 0;
+    // length-sorted symbol list is base[l] + (code >> (7 - l)) (mod 256).
+    // lim[1..7] sit in bytes 1..7 of limlo/limhi (byte 0 is zero).
+    uint32_t limlo = 0, limhi = 0;
+    uint64_t base = 0, nxt = 0;
     {
         uint32_t code = 0, off = 0;
 #pragma unroll
         for (uint32_t l = 1; l <= 7; ++l) {
             const uint32_t n = (uint32_t)(cnt >> (8 * l)) & 255u;
-            lim |= (uint64_t)((code + n) << (7 - l)) << (8 * l);
+            const uint32_t lim = (code + n) << (7 - l);
+            if (l < 4)
+                limlo |= lim << (8 * l);
+            else
+                limhi |= lim << (8 * (l - 4));
             base |= (uint64_t)((off - code) & 255u) << (8 * l);
             nxt |= (uint64_t)off << (8 * l);
             off += n;
@@ -153,43 +165,35 @@ RRG_HD bool strict_header_lane(const Fetch& fetch, uint64_t nbits, uint64_t p) {
         if (pos + 10 > nbits) return false;
         const uint64_t x = peek();
         const uint32_t r = rev7((uint32_t)x & 127u);
-        uint32_t l = 1;
-#pragma unroll
-        for (uint32_t k = 1; k <= 7; ++k) l += r >= ((uint32_t)(lim >> (8 * k)) & 255u) ? 1u : 0u;
+        // l = 1 + #{k : r >= lim[k]}, the seven compares at once: in each byte
+        // (r + 128) - lim[k] keeps bit 7 exactly when r >= lim[k] (lim <= 128,
+        // so no byte borrows from the next)
+        const uint32_t rb = (r | 128u) * 0x01010101u;
+        const uint32_t ge = ((rb & 0xFFFFFF00u) - limlo) & 0x80808000u;
+        const uint32_t gh = (rb - limhi) & 0x80808080u;
+        const uint32_t l = 1u + (uint32_t)(popcount32(ge) + popcount32(gh));
         if (l > 7) return false;  // an unused pattern of an incomplete (single-code) precode
         const uint32_t k = (((uint32_t)(base >> (8 * l)) + (r >> (7 - l))) & 255u);
         const uint32_t sym = (uint32_t)((k < 12 ? sa >> (5 * k) : sb >> (5 * (k - 12))) & 31u);
-        uint32_t L, n, adv;
-        if (sym <= 15) {
-            L = sym;
-            n = 1;
-            adv = l;
-            last = sym;
-        } else if (sym == 16) {
-            if (i == 0 || pos + l + 2 > nbits) return false;
-            L = last;
-            n = 3u + (uint32_t)((x >> l) & 3u);
-            adv = l + 2;
-        } else if (sym == 17) {
-            if (pos + l + 3 > nbits) return false;
-            L = 0;
-            n = 3u + (uint32_t)((x >> l) & 7u);
-            adv = l + 3;
-            last = 0;
-        } else {
-            if (pos + l + 7 > nbits) return false;
-            L = 0;
-            n = 11u + (uint32_t)((x >> l) & 127u);
-            adv = l + 7;
-            last = 0;
-        }
+        // the four kinds by selects: a length (n = 1), 16 (repeat the last, 3-6
+        // times, 2 extra bits), 17 (3-10 zeros, 3 bits), 18 (11-138 zeros, 7 bits)
+        const bool is_len = sym <= 15, is16 = sym == 16, is17 = sym == 17;
+        const uint32_t eb = is_len ? 0u : is16 ? 2u : is17 ? 3u : 7u;
+        const uint32_t n = (is_len ? 1u : is16 || is17 ? 3u : 11u) + ((uint32_t)(x >> l) & ((1u << eb) - 1u));
+        const uint32_t L = is_len ? sym : is16 ? last : 0u;
+        const uint32_t adv = l + eb;
+        if (is16 && i == 0) return false;
+        if (pos + adv > nbits) return false;
         if (i + n > total) return false;
+        last = L;
         skip(adv);
-        if (L) {
-            const uint32_t nl = i < hlit ? (n < hlit - i ? n : hlit - i) : 0u, nd = n - nl;
-            sumL += nl << (15 - L);
+        {
+            const uint32_t nl = L ? (i < hlit ? (n < hlit - i ? n : hlit - i) : 0u) : 0u;
+            const uint32_t nd = L ? n - nl : 0u;
+            const uint32_t sh = 15u - (L ? L : 15u);
+            sumL += nl << sh;
             nL += nl;
-            sumD += nd << (15 - L);
+            sumD += nd << sh;
             nD += nd;
         }
         if (i <= 256u && 256u < i + n) len256 = L;
