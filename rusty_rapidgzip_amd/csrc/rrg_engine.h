@@ -132,6 +132,15 @@ struct Ctx {
     uint8_t* stg = nullptr;
     size_t stg_cap = 0, stg_used = 0;
     bool no_alloc = false;  // a call that must not allocate: no staging buffer is made or grown
+    // pinned landing area of device-to-host copies, moved to their
+    // destinations after the wait that ends them (finish_downloads)
+    struct Download {
+        void* dst;
+        size_t off, n;
+    };
+    uint8_t* dl = nullptr;
+    size_t dl_cap = 0, dl_used = 0;
+    std::vector<Download> dl_pend;
     // every device buffer of the context
     std::vector<DevBuf*> bufs() {
         return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,
@@ -143,6 +152,7 @@ struct Ctx {
         for (DevBuf* b : bufs()) b->release();
         if (pin) (void)hipHostFree(pin);
         if (stg) (void)hipHostFree(stg);
+        if (dl) (void)hipHostFree(dl);
         for (hipEvent_t e : evx)
             if (e) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);

@@ -322,6 +322,43 @@ int stage_upload(Ctx& c, void* d, const void* h, size_t n, rrg_error* err) {
     return RRG_OK;
 }
 
+// Moves every queued download to its destination; call after a wait that
+// covers the copies.
+void finish_downloads(Ctx& c) {
+    for (const Ctx::Download& d : c.dl_pend) std::memcpy(d.dst, c.dl + d.off, d.n);
+    c.dl_pend.clear();
+    c.dl_used = 0;
+}
+
+// Copy n bytes from device address s to host memory dst on c.s, landing in
+// pinned memory (a pageable destination makes the runtime stage the copy
+// itself, synchronously): the bytes reach dst at the next finish_downloads(),
+// which the caller runs after its wait on the stream.
+int download(Ctx& c, void* dst, const void* s, size_t n, rrg_error* err) {
+    if (n == 0) return RRG_OK;
+    if (c.dl_used + n > c.dl_cap) {
+        if (c.no_alloc && n > c.dl_cap) {
+            HIPCHK(hipMemcpyAsync(dst, s, n, hipMemcpyDeviceToHost, c.s));
+            return RRG_OK;
+        }
+        // full: land what is queued, then start over (larger if need be)
+        HIPCHK(hipStreamSynchronize(c.s));
+        finish_downloads(c);
+        if (n > c.dl_cap) {
+            if (c.dl) (void)hipHostFree(c.dl);
+            c.dl = nullptr;
+            c.dl_cap = 0;
+            const size_t cap = std::max<size_t>(2 * n, 4u << 20);
+            HIPCHK(hipHostMalloc((void**)&c.dl, cap, hipHostMallocDefault));
+            c.dl_cap = cap;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(c.dl + c.dl_used, s, n, hipMemcpyDeviceToHost, c.s));
+    c.dl_pend.push_back(Ctx::Download{dst, c.dl_used, n});
+    c.dl_used = (c.dl_used + n + 255) & ~(size_t)255;
+    return RRG_OK;
+}
+
 // The expansion's time, once the stream has passed its end event.
 void collect_expand_time(Ctx& c) {
     if (!c.x_timed) return;
@@ -365,8 +402,9 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     }
     c.prof.n_inflate_launches++;
     res.resize(S);
-    HIPCHK(hipMemcpyAsync(res.data(), c.res.p, S * sizeof(SegResult), hipMemcpyDeviceToHost, c.s));
+    if (int rc = download(c, res.data(), c.res.p, S * sizeof(SegResult), err)) return rc;
     HIPCHK(hipStreamSynchronize(c.s));
+    finish_downloads(c);
     c.stg_used = 0;  // the stream has drained: no staged upload is pending
     if (c.verbose) {
         // the launch's largest outputs and its status counts
@@ -559,9 +597,12 @@ int resolve_and_crc(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_
         }
     }
     if (tail) c.prof.n_tail_rounds += (n + G - 1) / G;
-    if (n) HIPCHK(hipMemcpyAsync(flags.data(), c.err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.s));
-    if (!po.empty()) HIPCHK(hipMemcpyAsync(pc.data(), c.pcrc.p, pc.size() * 4, hipMemcpyDeviceToHost, c.s));
+    if (n)
+        if (int rc = download(c, flags.data(), c.err.p, (size_t)n * 4, err)) return rc;
+    if (!po.empty())
+        if (int rc = download(c, pc.data(), c.pcrc.p, pc.size() * 4, err)) return rc;
     HIPCHK(hipStreamSynchronize(c.s));
+    finish_downloads(c);
     collect_expand_time(c);
     raw.assign(ranges.size(), 0);
     const uint32_t full_shift = x8n(PIECE);
@@ -1050,13 +1091,14 @@ int run_finder(Ctx& c, const std::vector<uint64_t>& targets, const std::vector<u
                               exits ? c.exits.as<uint64_t>() : nullptr, want_recs ? c.recs.as<uint64_t>() : nullptr,
                               (uint32_t)n, c.exact_idx.as<uint32_t>(), (uint32_t)exact.size(), c.s, c.s2, c.evf[0],
                               c.evf[1]));
-    HIPCHK(hipMemcpyAsync(found.data(), c.found.p, n * 8, hipMemcpyDeviceToHost, c.s));
+    if (int rc = download(c, found.data(), c.found.p, n * 8, err)) return rc;
     if (exits) {
         exits->assign(n, 0);
-        HIPCHK(hipMemcpyAsync(exits->data(), c.exits.p, n * 8, hipMemcpyDeviceToHost, c.s));
+        if (int rc = download(c, exits->data(), c.exits.p, n * 8, err)) return rc;
     }
     t.stop();
     HIPCHK(hipStreamSynchronize(c.s));
+    finish_downloads(c);
     c.stg_used = 0;  // the stream has drained: no staged upload is pending
     if (c.verbose) {
         uint64_t longest = 0, none = 0;
@@ -1298,6 +1340,9 @@ int split_redo(Ctx& c, std::vector<Job>& jobs, std::vector<Redo>& redo, std::vec
 
 // Decode every job of the batch (inputs already in c.in) into `out`.
 int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out, rrg_error* err) {
+    // downloads queued by a call that failed before its wait are dropped
+    c.dl_pend.clear();
+    c.dl_used = 0;
     const uint64_t total_bits = (uint64_t)c.in_used * 8;
     // 1. host plan
     for (Job& J : jobs)
@@ -1611,8 +1656,9 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // expansion runs
     std::vector<MemberRec> recs(mb_used);
     if (mb_used) {
-        HIPCHK(hipMemcpyAsync(recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), hipMemcpyDeviceToHost, c.s));
+        if (int rc2 = download(c, recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), err)) return rc2;
         HIPCHK(hipStreamSynchronize(c.s));
+        finish_downloads(c);
     }
     rc = run_expand(c, items, out, err);
     if (rc) return rc;
