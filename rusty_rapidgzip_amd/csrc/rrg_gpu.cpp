@@ -160,6 +160,18 @@ uint32_t gf2_mul(uint32_t a, uint32_t b) {
     }
     return prod;
 }
+// Multiplication by a fixed a: gf2_mul is linear in its second operand, so
+// four tables of 256 products (one per byte of b) replace the 32 steps.
+struct GfMulBy {
+    uint32_t t[4][256];
+    explicit GfMulBy(uint32_t a) {
+        for (int k = 0; k < 4; ++k)
+            for (uint32_t b = 0; b < 256; ++b) t[k][b] = gf2_mul(a, b << (8 * k));
+    }
+    uint32_t operator()(uint32_t b) const {
+        return t[0][b & 255u] ^ t[1][(b >> 8) & 255u] ^ t[2][(b >> 16) & 255u] ^ t[3][b >> 24];
+    }
+};
 struct X2N {
     uint32_t t[64];
     X2N() {
@@ -605,10 +617,11 @@ int resolve_and_crc(Ctx& c, std::vector<ChainSeg>& cs, const std::vector<uint64_
     finish_downloads(c);
     collect_expand_time(c);
     raw.assign(ranges.size(), 0);
-    const uint32_t full_shift = x8n(PIECE);
+    static const GfMulBy full_shift(x8n(PIECE));  // most pieces are whole
     for (size_t i = 0; i < ranges.size(); ++i) {
         uint32_t r = 0;
-        for (size_t k = first[i]; k < first[i + 1]; ++k) r = gf2_mul(pl[k] == PIECE ? full_shift : x8n(pl[k]), r) ^ pc[k];
+        for (size_t k = first[i]; k < first[i + 1]; ++k)
+            r = (pl[k] == PIECE ? full_shift(r) : gf2_mul(x8n(pl[k]), r)) ^ pc[k];
         raw[i] = r;
     }
     return RRG_OK;
