@@ -1759,11 +1759,11 @@ struct LongTab {
     int16_t bias[16];
 };
 
-// The symbol of a code longer than LB bits (false: no such code), from the
-// symbols sorted by (length, symbol).
-template <int LB>
-__device__ __forceinline__ bool wave_long(uint64_t bits, const LongTab& t, const uint16_t* __restrict__ sorted,
-                                          uint32_t* sym, uint32_t* len) {
+// The LUT entry (lit_entry / dist_entry) of a code longer than LB bits, 0 when
+// no code matches, from the entries sorted by (length, symbol): the length is
+// the first l with rev < lj[l].
+template <int LB, int N, typename E>
+__device__ __forceinline__ uint32_t wave_long(uint64_t bits, const LongTab& t, const E (&sorted)[N]) {
     const uint32_t rev = __brev((uint32_t)bits) >> 17;
     const uint4 q = *reinterpret_cast<const uint4*>(&t.lj[8]);
     const uint32_t v[8] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16,
@@ -1772,10 +1772,12 @@ __device__ __forceinline__ bool wave_long(uint64_t bits, const LongTab& t, const
 #pragma unroll
     for (int l = LB + 1; l <= 15; ++l) n += rev >= v[l - 8] ? 1u : 0u;
     const uint32_t l = LB + 1 + n;
-    if (l > 15) return false;
-    *sym = sorted[(uint32_t)((int)(rev >> (15 - l)) + (int)t.bias[l])];
-    *len = l;
-    return true;
+    // past every code (l == 16): the lookup below stays in the list and is dropped
+    const uint32_t lc = l > 15 ? 15u : l;
+    int idx = (int)(rev >> (15 - lc)) + (int)t.bias[lc];
+    idx = idx < 0 ? 0 : idx > N - 1 ? N - 1 : idx;
+    const uint32_t e = (uint32_t)sorted[idx];
+    return l > 15 ? 0u : e | l;
 }
 
 // LUT entries of the wave kernel.  Literal/length (u16): bits 3..0 code
@@ -1802,9 +1804,9 @@ __device__ __forceinline__ uint32_t dist_entry(uint32_t s, uint32_t l) {
 // lit_entry / dist_entry for codes of at most TB bits, 0 otherwise.  Returns
 // false (the same in every lane) when the code is oversubscribed with at least
 // two symbols.  Called by the whole wave (it synchronises).
-template <int TB, bool DIST>
+template <int TB, bool DIST, typename E>
 __device__ bool wave_build(const uint8_t* __restrict__ lens, uint32_t n, void* __restrict__ lutv,
-                           uint16_t* __restrict__ sorted, LongTab& lt, uint32_t* __restrict__ c32, uint32_t lane) {
+                           E* __restrict__ sorted, LongTab& lt, uint32_t* __restrict__ c32, uint32_t lane) {
     if (lane < 16) c32[lane] = 0;
     __syncthreads();
     for (uint32_t s = lane; s < n; s += 64) {
@@ -1849,7 +1851,7 @@ __device__ bool wave_build(const uint8_t* __restrict__ lens, uint32_t n, void* _
             if (l == (uint32_t)ll) at = cur[ll] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
             cur[ll] += (uint32_t)__popcll(m);
         }
-        if (l) sorted[at] = (uint16_t)s;
+        if (l) sorted[at] = (E)(DIST ? dist_entry(s, 0) : lit_entry(s, 0));
     }
     __syncthreads();
     // LUT: the TB bits x (first input bit lowest) decode to the code of length l
@@ -1860,10 +1862,7 @@ __device__ bool wave_build(const uint8_t* __restrict__ lens, uint32_t n, void* _
 #pragma unroll
         for (int ll = 1; ll < TB; ++ll) l += rev >= ljv[ll] ? 1u : 0u;
         uint32_t e = 0;
-        if (rev < lt.lj[l]) {
-            const uint32_t sym = sorted[(uint32_t)((int)(rev >> (15 - l)) + (int)lt.bias[l])];
-            e = DIST ? dist_entry(sym, l) : lit_entry(sym, l);
-        }
+        if (rev < lt.lj[l]) e = (uint32_t)sorted[(uint32_t)((int)(rev >> (15 - l)) + (int)lt.bias[l])] | l;
         if (DIST) static_cast<uint32_t*>(lutv)[x] = e;
         else static_cast<uint16_t*>(lutv)[x] = (uint16_t)e;
     }
@@ -1877,7 +1876,8 @@ struct WaveLds {
     uint32_t dist[1 << WDLB];  // dist_entry
     uint16_t pre[128];         // precode LUT (7 bits): (symbol << 4) | length
     LongTab ltL, ltD;          // codes longer than the LUTs
-    uint16_t sortL[288], sortD[32];  // symbols in (length, symbol) order
+    uint16_t sortL[288];  // lit_entry(symbol, 0) in (length, symbol) order
+    uint32_t sortD[32];   // dist_entry(symbol, 0) in (length, symbol) order
     uint8_t lens[320];       // code lengths: literal/length, then distance
     uint8_t cl[32];          // precode lengths
     uint32_t c32[16];        // code-length counts (table build)
@@ -1997,10 +1997,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t l = e & 15u;                                                                    \
         if (l == 0) {                                                                            \
             if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
-            uint32_t s2, l2;                                                                     \
-            if (!wave_long<LB>(bitbuf, L.ltL, L.sortL, &s2, &l2)) WS_FAIL(ST_INVALID, DET_NO_CODE); \
-            e = lit_entry(s2, l2);                                                               \
-            l = l2;                                                                              \
+            e = wave_long<LB>(bitbuf, L.ltL, L.sortL);                                           \
+            if (!e) WS_FAIL(ST_INVALID, DET_NO_CODE);                                            \
+            l = e & 15u;                                                                         \
         }                                                                                        \
         if ((CHK) && pos + l > nbits) WS_FAIL(ST_EOF, DET_NONE);                                 \
         WS_CONSUME(l);                                                                           \
@@ -2027,10 +2026,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t dl = de & 15u;                                                                  \
         if (dl == 0) {                                                                           \
             if ((CHK) && pos + 15 > nbits) WS_FAIL(ST_EOF, DET_NONE);                            \
-            uint32_t s2, l2;                                                                     \
-            if (!wave_long<WDLB>(bitbuf, L.ltD, L.sortD, &s2, &l2)) WS_FAIL(ST_INVALID, DET_NO_CODE); \
-            de = dist_entry(s2, l2);                                                             \
-            dl = l2;                                                                             \
+            de = wave_long<WDLB>(bitbuf, L.ltD, L.sortD);                                        \
+            if (!de) WS_FAIL(ST_INVALID, DET_NO_CODE);                                           \
+            dl = de & 15u;                                                                       \
         }                                                                                        \
         if ((CHK) && pos + dl > nbits) WS_FAIL(ST_EOF, DET_NONE);                                \
         WS_CONSUME(dl);                                                                          \
@@ -2413,10 +2411,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #define WP_DEC(ACT)                                                                                     \
     do {                                                                                                \
         uint32_t _e = L.lit[(uint32_t)bb & LMASK];                                                      \
-        if ((ACT) && (_e & 15u) == 0) {                                                                 \
-            uint32_t _s, _l2;                                                                           \
-            _e = wave_long<LB>(bb, L.ltL, L.sortL, &_s, &_l2) ? lit_entry(_s, _l2) : 0u;                \
-        }                                                                                               \
+        if ((ACT) && (_e & 15u) == 0) _e = wave_long<LB>(bb, L.ltL, L.sortL);                          \
         const uint32_t _l = _e & 15u;                                                                   \
         const uint32_t _x = (_e >> 5) & 7u;                                                             \
         const bool _oth = (_e & 16u) != 0;                                                              \
@@ -2426,10 +2421,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint32_t _o = _l + _xl;                                                                   \
         const uint64_t _b2 = bb >> _o;                                                                  \
         uint32_t _d = L.dist[(uint32_t)_b2 & DMASK];                                                    \
-        if ((ACT) && _len && (_d & 15u) == 0) {                                                         \
-            uint32_t _s, _dl2;                                                                          \
-            _d = wave_long<WDLB>(_b2, L.ltD, L.sortD, &_s, &_dl2) ? dist_entry(_s, _dl2) : 0u;          \
-        }                                                                                               \
+        if ((ACT) && _len && (_d & 15u) == 0) _d = wave_long<WDLB>(_b2, L.ltD, L.sortD);               \
         const uint32_t _dl = _d & 15u;                                                                  \
         const uint32_t _dx = (_d >> 4) & 15u;                                                           \
         udist = (_d >> 8) + ((uint32_t)(_b2 >> _dl) & ((1u << _dx) - 1u));                              \
