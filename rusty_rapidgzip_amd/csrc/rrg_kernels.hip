@@ -2776,19 +2776,19 @@ hipError_t launch_tokens_wave(const uint64_t* d_in64, const SegDesc* d_segs, Seg
 // members that cannot hold markers, bytes).  Lane k of a batch owns token k:
 // a wave prefix sum over the tokens' lengths places them (at most 64 tokens
 // and kBatchCells cells per batch), and each lane writes its token's cells
-// into the wave's LDS ring of recent output (kRing cells).  Literals and
-// stored bytes need nothing earlier; a match copies the cells d before it,
-// by periodicity always from cells before its own start, and runs in the
-// first round in which every one of those cells is complete (they lie before
-// the batch's first unfinished token, whose own source always does: each
-// round finishes at least that token).  Sources still in the ring are read
-// from LDS, older ones from the stored output, and sources before the
-// segment's output become the markers 0x8000 | (offset before the segment)
-// of fast_inflate.rs:2209-2222.  Complete 64-cell lines then go from the
-// ring to memory as 16-byte stores.
+// into the wave's LDS ring of recent output (kRing cells) in one pass.
+// Literals and stored bytes are final; a match cell j is cell j - d (by
+// periodicity, for d < 8, a cell before the match's start), which is a marker
+// 0x8000 | (offset before the segment) when it precedes the segment
+// (fast_inflate.rs:2209-2222), a final cell when it precedes the batch (read
+// from the ring, or from the stored output when older than the ring), and
+// otherwise a cell of this batch: then the cell holds a link to it, and the
+// links are resolved in cell order afterwards.  Complete 64-cell lines then
+// go from the ring to memory as 16-byte stores.
 // ===========================================================================
 constexpr uint32_t kRing = 2048;        // cells of recent output per wave, in LDS (8 waves per SIMD)
 constexpr uint32_t kBatchCells = 1024;  // cells per batch (<= kRing / 2)
+constexpr uint32_t kLink = 0x4000u;     // a ring cell that names the ring slot of its value's cell
 
 __device__ __forceinline__ uint32_t tok_len(uint32_t w) {
     const uint32_t ty = w >> 30;
@@ -2847,54 +2847,58 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
             const uint32_t S = P0 + incl - L;  // this token's first cell
             uint32_t aux = (uint32_t)__shfl((int)w, lane < 63 ? lane + 1 : 63, 64);
             if (ty == TOK_STORED && lane == 63 && i + 1 < n_tok) aux = tk[i + 1];
-            const bool mine = (uint32_t)lane < nb;
+            const bool mine = (uint32_t)lane < nb && L != 0;
             // long matches and stored pieces are copied by the whole wave
             const bool longtok = L > 32 && (ty == TOK_MATCH || ty == TOK_STORED);
-            bool done = !mine || ty == TOK_OFFSET || L == 0;
-            if (!done && ty == TOK_LIT) {
-#pragma unroll
-                for (uint32_t j = 0; j < 3; ++j) ring[j < L ? ((S + j) & RM) : kRing + j] = (uint16_t)((w >> (8 * j)) & 0xffu);
-                done = true;
-            } else if (!done && ty == TOK_STORED && !longtok) {
-                const uint8_t* sp = src_in + (aux & 0x3fffffffu);
-                for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = sp[j];
-                done = true;
-            }
-            // matches, in rounds (32-bit cell arithmetic: a segment has < 2^32 cells)
-            const uint32_t d = (w & 0x7fffu) + 1u;
-            const uint32_t reach = S + min(L, d);
-            const uint32_t need = reach > d ? reach - d : 0u;  // the cells it reads lie below this (or before the segment)
             const uint32_t ring_lo = P0 + T > kRing ? P0 + T - kRing : 0u;  // older cells are no longer in the ring
-            while (true) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const uint64_t dm = __ballot(done);
-                if (dm == ~0ull) break;
-                const uint32_t k0 = (uint32_t)__builtin_ctzll(~dm);
-                const uint32_t frontier = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)k0));
-                if (!done && !longtok && need <= frontier && d >= 8) {
-                    // Cell j of the match is cell j - d: with d >= 8 a chunk of 8
-                    // reads only cells written before it.  A chunk whose source lies
-                    // wholly before the segment (markers), in the ring, or below the
-                    // ring (already stored) is copied without per-cell cases.
+            // The value of cell sj, a copy of cell q = sj - dd: a marker when q
+            // precedes the segment, a link to cell q when it lies in this batch
+            // (kLink | ring slot, resolved below), the cell itself when it is
+            // still in the ring, else the stored cell (read only when ld).
+            auto cell = [&](uint32_t sj, uint32_t dd, bool ld) -> uint32_t {
+                const uint32_t q = sj - dd;  // wraps when the source precedes the segment
+                const bool mk = sj < dd;
+                uint32_t x = ring[q & RM];
+                x = q >= P0 ? (kLink | (q & RM)) : x;
+                x = mk ? (0x8000u | (dd - sj - 1u)) : x;
+                if (!mk && q < ring_lo && ld) x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
+                return x;
+            };
+            // Every token is written once.  Literals and stored bytes are final;
+            // a match's cells are final when its source precedes the batch and
+            // links otherwise (32-bit cell arithmetic: a segment has < 2^32 cells).
+            const uint32_t d = (w & 0x7fffu) + 1u;
+            if (mine && !longtok) {
+                if (ty == TOK_LIT) {
+#pragma unroll
+                    for (uint32_t j = 0; j < 3; ++j) ring[j < L ? ((S + j) & RM) : kRing + j] = (uint16_t)((w >> (8 * j)) & 0xffu);
+                } else if (ty == TOK_STORED) {
+                    const uint8_t* sp = src_in + (aux & 0x3fffffffu);
+                    for (uint32_t j = 0; j < L; ++j) ring[(S + j) & RM] = sp[j];
+                } else if (d >= 8) {
+                    // cell j is cell j - d: a chunk of 8 never reads its own cells.
+                    // A chunk whose source lies wholly before the segment
+                    // (markers), in the batch (links), in the ring before the
+                    // batch, or below the ring (stored) needs no per-cell cases.
                     for (uint32_t j0 = 0; j0 < L; j0 += 8) {
                         uint32_t v[8];
-                        const uint32_t sj0 = S + j0;
+                        const uint32_t sj0 = S + j0, q = sj0 - d;
                         if (sj0 + 8 <= d) {
 #pragma unroll
                             for (int u = 0; u < 8; ++u) v[u] = 0x8000u | (d - sj0 - (uint32_t)u - 1u);
-                        } else if (sj0 >= d && sj0 - d >= ring_lo) {
-                            const uint32_t q = (sj0 - d) & RM;
-                            if (q + 8 <= kRing) {
+                        } else if (sj0 >= d && q >= P0) {
 #pragma unroll
-                                for (int u = 0; u < 8; ++u) v[u] = ring[q + (uint32_t)u];
+                            for (int u = 0; u < 8; ++u) v[u] = kLink | ((q + (uint32_t)u) & RM);
+                        } else if (sj0 >= d && q >= ring_lo && q + 8 <= P0) {
+                            const uint32_t r = q & RM;
+                            if (r + 8 <= kRing) {
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = ring[r + (uint32_t)u];
                             } else {
 #pragma unroll
-                                for (int u = 0; u < 8; ++u) v[u] = ring[(q + (uint32_t)u) & RM];
+                                for (int u = 0; u < 8; ++u) v[u] = ring[(r + (uint32_t)u) & RM];
                             }
-                        } else if (sj0 >= d && sj0 - d + 8 <= ring_lo) {
-                            const uint32_t q = sj0 - d;
+                        } else if (sj0 >= d && q + 8 <= ring_lo) {
                             if (u8) {
                                 uint2 b;
                                 __builtin_memcpy(&b, o8 + q, 8);
@@ -2908,77 +2912,82 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                             }
                         } else {
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                const uint32_t sj = sj0 + (uint32_t)u;
-                                const uint32_t q = sj - d;
-                                const bool mk = sj < d;
-                                const bool rg = !mk && q >= ring_lo;
-                                uint32_t x = mk ? (0x8000u | (d - sj - 1u)) : ring[q & RM];
-                                if (!mk && !rg && j0 + (uint32_t)u < L) x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
-                                v[u] = x;
-                            }
+                            for (int u = 0; u < 8; ++u) v[u] = cell(sj0 + (uint32_t)u, d, j0 + (uint32_t)u < L);
                         }
-                        const uint32_t t0 = sj0 & RM, nrem = L - j0;
-                        if (t0 + 8 <= kRing) {
+                        const uint32_t t0r = sj0 & RM, nrem = L - j0;
+                        if (t0r + 8 <= kRing) {
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
-                                ring[(uint32_t)u < nrem ? t0 + (uint32_t)u : kRing + (uint32_t)u] = (uint16_t)v[u];
+                                ring[(uint32_t)u < nrem ? t0r + (uint32_t)u : kRing + (uint32_t)u] = (uint16_t)v[u];
                         } else {
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
                                 ring[(uint32_t)u < nrem ? ((sj0 + (uint32_t)u) & RM) : kRing + (uint32_t)u] = (uint16_t)v[u];
                         }
                     }
-                    done = true;
-                }
-                if (!done && !longtok && need <= frontier) {
+                } else {
                     // d < 8: the source repeats with period d
                     uint32_t jm = 0;  // j mod d
                     for (uint32_t j0 = 0; j0 < L; j0 += 8) {
                         uint32_t v[8];
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
-                            const uint32_t sj = S + jm;
-                            const uint32_t q = sj - d;  // wraps when the source precedes the segment
-                            const bool mk = sj < d;
-                            const bool rg = !mk && q >= ring_lo;
-                            const uint32_t xr = ring[q & RM];
-                            uint32_t x = mk ? (0x8000u | (d - sj - 1u)) : xr;
-                            if (!mk && !rg && j0 + (uint32_t)u < L) x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
-                            v[u] = x;
+                            v[u] = cell(S + jm, d, j0 + (uint32_t)u < L);
                             jm = (jm + 1 == d) ? 0u : jm + 1;
                         }
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
                             ring[j0 + (uint32_t)u < L ? ((S + j0 + (uint32_t)u) & RM) : kRing + (uint32_t)u] = (uint16_t)v[u];
                     }
-                    done = true;
                 }
-                // the first unfinished token is always ready; if it is long, the
-                // 64 lanes copy it together, 64 cells at a time (cell j comes
-                // from the cell j mod d after the start of its source)
-                if (uni32((uint32_t)__builtin_amdgcn_readlane((int)(longtok ? 1u : 0u), (int)k0))) {
-                    const uint32_t kw = uni32((uint32_t)__builtin_amdgcn_readlane((int)w, (int)k0));
-                    const uint32_t kL = tok_len(kw);
-                    const uint32_t kS = frontier;
-                    if ((kw >> 30) == TOK_STORED) {
-                        const uint8_t* sp =
-                            src_in + (uni32((uint32_t)__builtin_amdgcn_readlane((int)aux, (int)k0)) & 0x3fffffffu);
-                        for (uint32_t j = (uint32_t)lane; j < kL; j += 64) ring[(kS + j) & RM] = sp[j];
-                    } else {
-                        const uint32_t kd = (kw & 0x7fffu) + 1u;
-                        for (uint32_t j = (uint32_t)lane; j < kL; j += 64) {
-                            const uint32_t jm = kd == 1 ? 0u : (j < kd ? j : j % kd);
-                            const uint32_t sj = kS + jm;
-                            const uint32_t q = sj - kd;
-                            uint32_t x;
-                            if (sj < kd) x = 0x8000u | (kd - sj - 1u);
-                            else if (q >= ring_lo) x = ring[q & RM];
-                            else x = u8 ? (uint32_t)o8[q] : (uint32_t)c16[q];
-                            ring[(kS + j) & RM] = (uint16_t)x;
-                        }
+            }
+            // long tokens: the 64 lanes copy each together, 64 cells at a time
+            // (cell j comes from the cell j mod d after the start of its source)
+            for (uint64_t lm = __ballot(mine && longtok); lm; lm &= lm - 1) {
+                const int k = (int)__builtin_ctzll(lm);
+                const uint32_t kw = uni32((uint32_t)__builtin_amdgcn_readlane((int)w, k));
+                const uint32_t kL = tok_len(kw);
+                const uint32_t kS = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, k));
+                if ((kw >> 30) == TOK_STORED) {
+                    const uint8_t* sp = src_in + (uni32((uint32_t)__builtin_amdgcn_readlane((int)aux, k)) & 0x3fffffffu);
+                    for (uint32_t j = (uint32_t)lane; j < kL; j += 64) ring[(kS + j) & RM] = sp[j];
+                } else {
+                    const uint32_t kd = (kw & 0x7fffu) + 1u;
+                    for (uint32_t j = (uint32_t)lane; j < kL; j += 64) {
+                        const uint32_t jm = kd == 1 ? 0u : (j < kd ? j : j % kd);
+                        ring[(kS + j) & RM] = (uint16_t)cell(kS + jm, kd, true);
                     }
-                    if ((uint32_t)lane == k0) done = true;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Links, in cell order, 64 cells at a time: a link points to an
+            // earlier cell, so every cell of the rows before is final; within a
+            // row each pass replaces a link by what its target holds (the final
+            // value or that target's own link, further back), so a chain of n
+            // links in a row takes about log2(n) passes.
+            const uint64_t lkm = __ballot(mine && ty == TOK_MATCH && S + L > d + P0);  // its last source cell is in the batch
+            if (lkm) {
+                const uint32_t lo = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)__builtin_ctzll(lkm)));
+                const uint32_t hi = uni32((uint32_t)__builtin_amdgcn_readlane((int)(S + L), 63 - (int)__builtin_clzll(lkm)));
+                for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
+                    const uint32_t c = c0 + (uint32_t)lane;
+                    uint32_t v = ring[c & RM];
+                    bool lk = c < hi && (v & 0xc000u) == kLink;
+                    while (__ballot(lk)) {
+                        const uint32_t x = ring[v & RM];
+                        v = lk ? x : v;
+                        const bool lk2 = lk && (x & 0xc000u) == kLink;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (lk) ring[c & RM] = (uint16_t)v;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        lk = lk2;
+                    }
                 }
             }
             // store the complete lines (all of the rest after the last batch)
