@@ -382,7 +382,8 @@ void collect_expand_time(Ctx& c) {
 // One launch of inflate pass 1 over the segment indices in `work`; results
 // for every segment are copied back into `res`.
 int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vector<SegResult>& res,
-                uint64_t tok_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err) {
+                uint64_t tok_used, uint64_t mb_used, const std::vector<uint32_t>& work, rrg_error* err,
+                std::vector<MemberRec>* recs) {
     const size_t S = desc.size();
     if (work.empty()) return RRG_OK;
     HIPCHK(c.sorted.ensure(G.scratch_bytes));
@@ -415,6 +416,12 @@ int run_inflate(Ctx& c, const Geometry& G, std::vector<SegDesc>& desc, std::vect
     c.prof.n_inflate_launches++;
     res.resize(S);
     if (int rc = download(c, res.data(), c.res.p, S * sizeof(SegResult), err)) return rc;
+    // the member records of every segment so far come back with the same wait
+    if (recs) {
+        recs->resize(mb_used);
+        if (mb_used)
+            if (int rc = download(c, recs->data(), c.mrec.p, mb_used * sizeof(MemberRec), err)) return rc;
+    }
     HIPCHK(hipStreamSynchronize(c.s));
     finish_downloads(c);
     c.stg_used = 0;  // the stream has drained: no staged upload is pending
@@ -1558,7 +1565,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         }
         all.swap(sorted);
     }
-    rc = run_inflate(c, G, desc, res, tok_used, mb_used, all, err);
+    std::vector<MemberRec> recs;  // member records, from the last inflate launch
+    rc = run_inflate(c, G, desc, res, tok_used, mb_used, all, err, &recs);
     if (rc) return rc;
     for (Job& J : jobs) {
         // a capacity overflow is not a decode failure (the region is grown below)
@@ -1597,7 +1605,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
         rc = split_redo(c, jobs, redo, desc, res, tok_used, mb_used, again, err);
         if (rc) return rc;
         c.prof.n_redecoded += again.size();
-        rc = run_inflate(c, G, desc, res, tok_used, mb_used, again, err);
+        rc = run_inflate(c, G, desc, res, tok_used, mb_used, again, err, &recs);
         if (rc) return rc;
     }
     c.prof.n_segments += desc.size();
@@ -1666,15 +1674,8 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     }
     HIPCHK(out.ensure(out_total + 64));
     HIPCHK(c.cells.ensure((cells_total + 64) * sizeof(uint16_t)));
-    // the member records (from the inflate pass) come back first; the host
-    // then lays out the tail chain, resolve tiles and CRC pieces while the
-    // expansion runs
-    std::vector<MemberRec> recs(mb_used);
-    if (mb_used) {
-        if (int rc2 = download(c, recs.data(), c.mrec.p, mb_used * sizeof(MemberRec), err)) return rc2;
-        HIPCHK(hipStreamSynchronize(c.s));
-        finish_downloads(c);
-    }
+    // the host lays out the tail chain, resolve tiles and CRC pieces while
+    // the expansion runs
     rc = run_expand(c, items, out, err);
     if (rc) return rc;
     for (const Widen& w : windows)

@@ -3363,7 +3363,18 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(const ResolveTile* 
             for (uint32_t q = 0; q < np; ++q) {
                 const uint64_t src = part_src[q];
                 const uint32_t dst = part_dst[q], n = part_n[q];
-                for (uint32_t i = threadIdx.x; i < n; i += RESOLVE_THREADS) win[dst + i] = (uint8_t)cells[src + i];
+                // 8 cells per load (the window is read once per tile)
+                for (uint32_t i = 8u * threadIdx.x; i < n; i += 8u * RESOLVE_THREADS) {
+                    if (i + 8 <= n) {
+                        const uint4 g = load_cells8(cells + src + i);
+                        const uint32_t lo = __builtin_amdgcn_perm(g.y, g.x, 0x06040200u);
+                        const uint32_t hi = __builtin_amdgcn_perm(g.w, g.z, 0x06040200u);
+#pragma unroll
+                        for (uint32_t b = 0; b < 8; ++b) win[dst + i + b] = (uint8_t)((b < 4 ? lo : hi) >> (8 * (b & 3)));
+                    } else {
+                        for (uint32_t k = i; k < n; ++k) win[dst + k] = (uint8_t)cells[src + k];
+                    }
+                }
             }
         __syncthreads();
     }
@@ -3375,6 +3386,8 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(const ResolveTile* 
     const uint32_t phase = (uint32_t)((uint64_t)delta & 7u);  // cells, within a 16-byte cell group
     const uint32_t dsh = phase >> 1, odd = phase & 1u;
     const uint4* c128 = reinterpret_cast<const uint4*>(cells);
+    // two steps per iteration: the next step's loads are issued before this one's store
+#pragma unroll 2
     for (uint64_t p0 = (ob & ~15ull) + 16ull * threadIdx.x; p0 < oe; p0 += 16ull * RESOLVE_THREADS) {
         const int64_t q = (delta + (int64_t)p0 - (int64_t)phase) >> 3;  // first 16-byte cell group
         // a group before the buffer start only holds cells of bytes before the tile
