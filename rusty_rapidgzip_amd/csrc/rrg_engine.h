@@ -110,7 +110,8 @@ struct Ctx {
     int n_cu = 0;
     uint64_t max_lanes = 0;  // cap on inflate lanes in flight (0 = as many as fit)
     DevBuf in, targets, limits, scan_ends, found, segs, res, mrec, work, counter, toks, xitems, cells, sorted, chain, tz,
-        tiles, out, poff, plen, pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx;
+        tiles, out, poff, plen, pcrc, err, reach, first, stops, exits, recs, legtg, lfound, lnext, exact_idx,
+        xcut, cuts;
     DevBuf in2, out2;  // the other input and output buffer of the slab decode
     DevBuf win;        // the last 32 KiB of output before the next slab
     size_t in_used = 0;  // bytes of c.in holding the current batch (files + padding)
@@ -146,7 +147,7 @@ struct Ctx {
         return {&in,    &targets, &limits, &scan_ends, &found, &segs,  &res,   &mrec, &work, &counter,
                 &toks,  &xitems,  &cells,  &sorted,    &chain, &tz,    &tiles, &out,  &poff, &plen,
                 &pcrc,  &err,     &reach,  &first,     &stops, &exits, &recs,  &legtg, &lfound, &lnext, &exact_idx,
-                &in2,   &out2,    &win};
+                &xcut,  &cuts,    &in2,    &out2,      &win};
     }
     ~Ctx() {
         for (DevBuf* b : bufs()) b->release();

@@ -491,6 +491,126 @@ int run_expand(Ctx& c, std::vector<ExpandItem>& items, DevBuf& out, rrg_error* e
     return RRG_OK;
 }
 
+// Cells of an expansion piece.  An item of at least 2 * kPiece cells (a
+// segment of runs or short-period repeats expands to millions of cells per
+// compressed KiB) would keep one wavefront busy long after the others end.
+constexpr uint64_t kPiece = 1u << 19;
+
+// Expansion items of chain segments with at least 2 * kPiece cells are cut at
+// token boundaries into pieces of about kPiece cells (k_expand_cuts finds the
+// tokens).  Each piece expands on a wavefront of its own with its cells
+// numbered from its own start, so a source before the piece becomes a marker
+// 0x8000 | offset before the piece, and each piece becomes a chain segment of
+// its own that may hold markers: the tail chain and the resolve pass fill
+// them in from the pieces before, as for any segment.  `item_cs[i]` is the
+// chain segment of items[i] (SIZE_MAX: not a chain segment).  On return
+// `of` maps each new chain segment to the one it was cut from (empty: nothing
+// was cut), `cs_orig` holds the chain before the cut and `first_len[k]` the
+// cells of its segment k's first piece (whose markers alone may name bytes
+// before the segment).
+int split_long_items(Ctx& c, std::vector<ExpandItem>& items, const std::vector<size_t>& item_cs,
+                     std::vector<ChainSeg>& cs, std::vector<uint64_t>& cs_out, std::vector<ChainSeg>& cs_orig,
+                     std::vector<uint32_t>& of, std::vector<uint64_t>& first_len, rrg_error* err) {
+    cs_orig.clear();
+    of.clear();
+    first_len.clear();
+    std::vector<size_t> big;
+    for (size_t i = 0; i < items.size(); ++i)
+        if (item_cs[i] != SIZE_MAX && !(items[i].flags & EXP_U8) && items[i].n_tok && items[i].n_cells >= 2 * kPiece)
+            big.push_back(i);
+    if (big.empty()) return RRG_OK;
+    const uint32_t nb = (uint32_t)big.size();
+    std::vector<ExpandItem> bi(nb);
+    std::vector<uint32_t> firsts(nb);
+    uint64_t R = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        bi[b] = items[big[b]];
+        firsts[b] = (uint32_t)R;
+        R += (bi[b].n_cells + kPiece - 1) / kPiece - 1;
+    }
+    const size_t items_bytes = ((size_t)nb * sizeof(ExpandItem) + 255) & ~(size_t)255;
+    HIPCHK(c.xcut.ensure(items_bytes + (size_t)nb * 4));
+    HIPCHK(c.cuts.ensure((size_t)R * sizeof(uint2)));
+    if (int rc = stage_upload(c, c.xcut.p, bi.data(), (size_t)nb * sizeof(ExpandItem), err)) return rc;
+    uint8_t* const d_first = static_cast<uint8_t*>(c.xcut.p) + items_bytes;
+    if (int rc = stage_upload(c, d_first, firsts.data(), (size_t)nb * 4, err)) return rc;
+    // every record is written once; a record left at ~0 would leave its item whole
+    HIPCHK(hipMemsetAsync(c.cuts.p, 0xff, (size_t)R * sizeof(uint2), c.s));
+    HIPCHK(launch_expand_cuts(c.toks.as<uint32_t>(), c.xcut.as<ExpandItem>(), reinterpret_cast<const uint32_t*>(d_first),
+                              nb, (uint32_t)kPiece, c.cuts.as<uint2>(), c.s));
+    std::vector<uint2> cuts(R);
+    HIPCHK(hipMemcpyAsync(cuts.data(), c.cuts.p, (size_t)R * sizeof(uint2), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    // the pieces of each cut item: (first token, first cell), in order
+    std::vector<std::vector<std::pair<uint32_t, uint64_t>>> pieces(cs.size());
+    std::vector<ExpandItem> out_items;
+    out_items.reserve(items.size() + R);
+    size_t bpos = 0;
+    for (size_t i = 0; i < items.size(); ++i) {
+        if (bpos < nb && big[bpos] == i) {
+            const ExpandItem& it = items[i];
+            const uint32_t b = (uint32_t)bpos++;
+            const uint64_t nr = (it.n_cells + kPiece - 1) / kPiece - 1;
+            std::vector<std::pair<uint32_t, uint64_t>> pc{{0u, 0ull}};
+            bool ok = true;
+            for (uint64_t r = 0; r < nr && ok; ++r) {
+                const uint2 q = cuts[firsts[b] + r];
+                if (q.x == pc.back().first && q.y == pc.back().second) continue;  // a long stored token holds several
+                ok = q.x > pc.back().first && q.y > pc.back().second && q.x < it.n_tok && q.y < it.n_cells;
+                pc.push_back({q.x, (uint64_t)q.y});
+            }
+            if (ok && pc.size() > 1) {
+                for (size_t p = 0; p < pc.size(); ++p) {
+                    const uint64_t t1 = p + 1 < pc.size() ? pc[p + 1].first : it.n_tok;
+                    const uint64_t c1 = p + 1 < pc.size() ? pc[p + 1].second : it.n_cells;
+                    out_items.push_back(ExpandItem{it.tok_base + pc[p].first, it.dst + pc[p].second, it.in_byte,
+                                                   c1 - pc[p].second, (uint32_t)(t1 - pc[p].first), it.flags});
+                }
+                pieces[item_cs[i]] = std::move(pc);
+                continue;
+            }
+        }
+        out_items.push_back(items[i]);
+    }
+    items.swap(out_items);
+    // the chain with each cut segment replaced by its pieces
+    std::vector<ChainSeg> cs2;
+    std::vector<uint64_t> out2;
+    cs2.reserve(cs.size() + R);
+    out2.reserve(cs.size() + R);
+    first_len.resize(cs.size());
+    for (size_t k = 0; k < cs.size(); ++k) {
+        const std::vector<std::pair<uint32_t, uint64_t>>& pc = pieces[k];
+        first_len[k] = pc.size() > 1 ? pc[1].second : cs[k].len;
+        if (pc.size() <= 1) {
+            cs2.push_back(cs[k]);
+            out2.push_back(cs_out[k]);
+            of.push_back((uint32_t)k);
+            continue;
+        }
+        for (size_t p = 0; p < pc.size(); ++p) {
+            const uint64_t c1 = p + 1 < pc.size() ? pc[p + 1].second : cs[k].len;
+            ChainSeg s = cs[k];
+            s.cells_base = cs[k].cells_base + pc[p].second;
+            s.len = c1 - pc[p].second;
+            if (p) {
+                s.spec = 1;   // cells before the piece are markers
+                s.first = 0;  // and the pieces before it are reachable
+            }
+            cs2.push_back(s);
+            out2.push_back(cs_out[k] + pc[p].second);
+            of.push_back((uint32_t)k);
+        }
+    }
+    if (c.verbose)
+        std::fprintf(stderr, "[rrg-gpu] expansion: %u long items cut into %zu pieces\n", nb,
+                     cs2.size() - cs.size() + nb);
+    cs.swap(cs2);
+    cs_out.swap(out2);
+    cs_orig.swap(cs2);
+    return RRG_OK;
+}
+
 // The kernels after the expansion, for the chain segments `cs` into `out`
 // (out_off[k]: chain segment k's first output byte) and the CRC ranges
 // [lo, hi) of `out`: tail chain, resolve + narrow, CRC pieces.  Every
@@ -1031,7 +1151,7 @@ void finish_serial(Job& J, const std::vector<SegDesc>& desc, const std::vector<S
 // that start less than 32 KiB after their member's start can hold one.
 int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc>& desc,
                  const std::vector<SegResult>& res, const std::vector<MemberRec>& recs, const std::vector<ChainSeg>& cs,
-                 std::vector<std::vector<uint64_t>>& bad, rrg_error* err) {
+                 const std::vector<uint64_t>& first_len, std::vector<std::vector<uint64_t>>& bad, rrg_error* err) {
     constexpr uint64_t W = 32768;
     std::vector<ReachScan> items;
     std::vector<std::pair<size_t, size_t>> where;  // (job, chain position)
@@ -1051,7 +1171,11 @@ int member_reach(Ctx& c, const std::vector<Job>& jobs, const std::vector<SegDesc
                 check = !J.seg_is_chunk[k] && mstart > cstart;  // not the chunk's first member
             }
             if (check && (desc[j].flags & SEG_SPEC) && o - mstart < W && res[j].n_cells) {
-                items.push_back(ReachScan{cs[J.c_lo + nb + k].cells_base, (uint32_t)res[j].n_cells, (uint32_t)(o - mstart)});
+                // a segment expanded in pieces: the markers of the later
+                // pieces name cells of the segment itself
+                const uint64_t len = first_len.empty() ? res[j].n_cells
+                                                       : std::min<uint64_t>(res[j].n_cells, first_len[J.c_lo + nb + k]);
+                items.push_back(ReachScan{cs[J.c_lo + nb + k].cells_base, (uint32_t)len, (uint32_t)(o - mstart)});
                 where.push_back({ji, k});
             }
             for (uint32_t q = 0; q < res[j].n_mb; ++q) {
@@ -1160,8 +1284,9 @@ struct OpenTarget {
 // range is cut into legs of kLegBits, all legs of all targets walked at once
 // (one wavefront each, recording their first chain positions), and
 // k_leg_fixup stitches each target's legs along its true chain.  The part
-// grows 4x per round (1 MiB, 4 MiB, ... 64 MiB) until every target has its
-// find or reached its file end.
+// doubles per round (1 MiB, 2 MiB, ... 64 MiB) until every target has its
+// find or reached its file end: legs past a find are walked for nothing, and
+// doubling bounds them by the distance walked before.
 int resolve_open(Ctx& c, std::vector<OpenTarget> open, std::vector<uint64_t>& found, rrg_error* err) {
     constexpr uint64_t kLegBits = 8ull * 16384;
     constexpr uint64_t kFirstSpan = 8ull * (1ull << 20), kMaxSpan = 8ull * (64ull << 20);
@@ -1215,7 +1340,7 @@ int resolve_open(Ctx& c, std::vector<OpenTarget> open, std::vector<uint64_t>& fo
                 }
                 found[o.idx] = of[i];
             } else if (on[i] + 13 <= o.limit && on[i] > o.p) {
-                still.push_back(OpenTarget{o.idx, on[i], o.limit, std::min(kMaxSpan, 4 * o.span)});
+                still.push_back(OpenTarget{o.idx, on[i], o.limit, std::min(kMaxSpan, 2 * o.span)});
             }
         }
         if (c.verbose)
@@ -1619,6 +1744,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<ChainSeg> cs;
     std::vector<uint64_t> cs_out;
     std::vector<ExpandItem> items;
+    std::vector<size_t> item_cs;  // chain segment of each item (SIZE_MAX: a BGZF member)
     uint64_t out_total = 0, cells_total = 0;
     // the tokens of a segment that overflowed its region are not all stored
     auto expandable = [&](size_t j) -> uint32_t {
@@ -1650,9 +1776,11 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             J.off.push_back(o);
             cs.push_back(ChainSeg{0, 0, 0, 0u, 1u});
             cs_out.push_back(out_total + o);
-            if (res[j].n_cells)
+            if (res[j].n_cells) {
                 items.push_back(ExpandItem{desc[j].tok_base, out_total + o, desc[j].start_bit >> 3, res[j].n_cells,
                                            expandable(j), EXP_U8});
+                item_cs.push_back(SIZE_MAX);
+            }
             o += res[j].n_cells;
         }
         for (size_t k = 0; k < J.n_out; ++k) {
@@ -1663,9 +1791,11 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
             cs.push_back(ChainSeg{base, res[j].n_cells, 0, (desc[j].flags & SEG_SPEC) ? 1u : 0u,
                                   k == 0 && !win ? 1u : 0u});
             cs_out.push_back(out_total + o);
-            if (res[j].n_cells)
+            if (res[j].n_cells) {
                 items.push_back(ExpandItem{desc[j].tok_base, base, desc[j].start_bit >> 3, res[j].n_cells,
                                            expandable(j), 0u});
+                item_cs.push_back(cs.size() - 1);
+            }
             o += res[j].n_cells;
         }
         J.off.push_back(o);
@@ -1674,6 +1804,13 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     }
     HIPCHK(out.ensure(out_total + 64));
     HIPCHK(c.cells.ensure((cells_total + 64) * sizeof(uint16_t)));
+    // long items expand in pieces, each a chain segment of its own (the
+    // member checks below still see the original chain)
+    std::vector<ChainSeg> cs_orig;  // the chain before the cut (empty: nothing was cut)
+    std::vector<uint32_t> cs_of;
+    std::vector<uint64_t> first_len;
+    rc = split_long_items(c, items, item_cs, cs, cs_out, cs_orig, cs_of, first_len, err);
+    if (rc) return rc;
     // the host lays out the tail chain, resolve tiles and CRC pieces while
     // the expansion runs
     rc = run_expand(c, items, out, err);
@@ -1683,7 +1820,7 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     // back-references before their member's start (read before the tail pass
     // rewrites markers)
     std::vector<std::vector<uint64_t>> bad(jobs.size());
-    rc = member_reach(c, jobs, desc, res, recs, cs, bad, err);
+    rc = member_reach(c, jobs, desc, res, recs, cs_orig.empty() ? cs : cs_orig, first_len, bad, err);
     if (rc) return rc;
     // 7. tail chain, resolve and the CRC of every member of every file (BGZF
     // members, then member-path members)
@@ -1713,6 +1850,12 @@ int decode_batch(Ctx& c, const Geometry& G, std::vector<Job>& jobs, DevBuf& out,
     std::vector<uint32_t> flags, raw;
     rc = resolve_and_crc(c, cs, cs_out, out, ranges, flags, raw, err);
     if (rc) return rc;
+    if (!cs_of.empty()) {
+        // per original chain segment
+        std::vector<uint32_t> f(cs_orig.size(), 0);
+        for (size_t k = 0; k < cs_of.size(); ++k) f[cs_of[k]] |= flags[k];
+        flags.swap(f);
+    }
     // 8. per-file verdicts and chunk boundaries
     for (Job& J : jobs) {
         Decoded& dec = J.dec;

@@ -2942,8 +2942,20 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                 }
             }
             // long tokens: the 64 lanes copy each together, 64 cells at a time
-            // (cell j comes from the cell j mod d after the start of its source)
-            for (uint64_t lm = __ballot(mine && longtok); lm; lm &= lm - 1) {
+            // (cell j comes from the cell j mod d after the start of its source).
+            // With several in the batch (runs, short-period repeats) they go in
+            // token order and their cells are final: an in-batch source cell is
+            // read after the tokens before it are written, following the links
+            // of short matches back to a final cell, so a batch of long matches
+            // leaves no links to resolve.  A single one writes links (no waits).
+            const uint64_t lmask = __ballot(mine && longtok);
+            const bool lfinal = __popcll(lmask) >= 2;
+            if (lfinal) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            for (uint64_t lm = lmask; lm; lm &= lm - 1) {
                 const int k = (int)__builtin_ctzll(lm);
                 const uint32_t kw = uni32((uint32_t)__builtin_amdgcn_readlane((int)w, k));
                 const uint32_t kL = tok_len(kw);
@@ -2955,8 +2967,18 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                     const uint32_t kd = (kw & 0x7fffu) + 1u;
                     for (uint32_t j = (uint32_t)lane; j < kL; j += 64) {
                         const uint32_t jm = kd == 1 ? 0u : (j < kd ? j : j % kd);
-                        ring[(kS + j) & RM] = (uint16_t)cell(kS + jm, kd, true);
+                        uint32_t x = cell(kS + jm, kd, true);
+                        // a link names an earlier cell of the batch: the chain ends
+                        if (lfinal)
+                            while ((x & 0xc000u) == kLink) x = ring[x & RM];
+                        ring[(kS + j) & RM] = (uint16_t)x;
                     }
+                }
+                if (lfinal) {
+                    // the next long token may read these cells
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2967,7 +2989,7 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
             // row each pass replaces a link by what its target holds (the final
             // value or that target's own link, further back), so a chain of n
             // links in a row takes about log2(n) passes.
-            const uint64_t lkm = __ballot(mine && ty == TOK_MATCH && S + L > d + P0);  // its last source cell is in the batch
+            const uint64_t lkm = __ballot(mine && !(longtok && lfinal) && ty == TOK_MATCH && S + L > d + P0);  // its last source cell is in the batch
             if (lkm) {
                 const uint32_t lo = uni32((uint32_t)__builtin_amdgcn_readlane((int)S, (int)__builtin_ctzll(lkm)));
                 const uint32_t hi = uni32((uint32_t)__builtin_amdgcn_readlane((int)(S + L), 63 - (int)__builtin_clzll(lkm)));
@@ -3003,7 +3025,7 @@ __global__ __launch_bounds__(64) void k_expand(const uint32_t* __restrict__ toks
                         const uint32_t hi = __builtin_amdgcn_perm(g.w, g.z, 0x06040200u);
                         *reinterpret_cast<uint2*>(o8 + c) = make_uint2(lo, hi);
                     } else {
-                        *reinterpret_cast<uint4*>(c16 + c) = g;
+                        store_cells8(c16 + c, g);  // a piece of a long item may start at any cell
                     }
                 } else {
                     for (uint32_t k = c; k < upto; ++k) {
@@ -3032,6 +3054,49 @@ hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint
     const uint32_t blocks = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)n_cu, n_items);
     hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(64), 0, s, d_toks, d_items, n_items, d_counter, d_in8, d_cells,
                        d_out);
+    return hipGetLastError();
+}
+
+// Cut points of long expansion items: one wavefront per item walks its
+// tokens with a prefix sum over their lengths; for every m >= 1 with
+// m * piece < n_cells, the token holding cell m * piece writes
+// cuts[first[b] + m - 1] = (token index, its first cell).  Tokens cover the
+// item's cells without gaps, so each record is written exactly once (offset
+// tokens hold no cells and are never a cut).
+__global__ __launch_bounds__(64) void k_expand_cuts(const uint32_t* __restrict__ toks,
+                                                    const ExpandItem* __restrict__ items, const uint32_t* __restrict__ first,
+                                                    uint32_t n, uint32_t piece, uint2* __restrict__ cuts) {
+    const uint32_t b = blockIdx.x;
+    if (b >= n) return;
+    const int lane = (int)threadIdx.x;
+    const uint32_t* tk = toks + items[b].tok_base;
+    const uint32_t n_tok = items[b].n_tok;
+    const uint64_t n_cells = items[b].n_cells;
+    uint2* out = cuts + first[b];
+    uint64_t base = 0;
+    for (uint32_t t0 = 0; t0 < n_tok; t0 += 64) {
+        const uint32_t i = t0 + (uint32_t)lane;
+        const uint32_t L = i < n_tok ? tok_len(tk[i]) : 0u;
+        uint32_t incl = L;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        const uint64_t s0 = base + incl - L, e0 = s0 + L;
+        if (L != 0 && e0 <= n_cells) {
+            uint64_t m = (s0 + piece - 1) / piece;
+            if (m == 0) m = 1;
+            for (; m * piece < e0; ++m) out[m - 1] = make_uint2(i, (uint32_t)s0);
+        }
+        base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+}
+
+hipError_t launch_expand_cuts(const uint32_t* d_toks, const ExpandItem* d_items, const uint32_t* d_first, uint32_t n,
+                              uint32_t piece, uint2* d_cuts, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_cuts, dim3(n), dim3(64), 0, s, d_toks, d_items, d_first, n, piece, d_cuts);
     return hipGetLastError();
 }
 

@@ -66,6 +66,13 @@ hipError_t launch_tokens_wave(const uint64_t* d_in64, const SegDesc* d_segs, Seg
 hipError_t launch_expand(const uint32_t* d_toks, const ExpandItem* d_items, uint32_t n_items, uint32_t* d_counter,
                          const uint8_t* d_in8, uint16_t* d_cells, uint8_t* d_out, int n_cu, hipStream_t s);
 
+// Cut points of the n long items d_items[b] into pieces of about `piece`
+// cells: d_cuts[d_first[b] + m - 1] = (index of the token holding cell
+// m * piece, that token's first cell) for every m >= 1 with m * piece below
+// the item's cells.
+hipError_t launch_expand_cuts(const uint32_t* d_toks, const ExpandItem* d_items, const uint32_t* d_first, uint32_t n,
+                              uint32_t piece, uint2* d_cuts, hipStream_t s);
+
 // Tail chain: blocked scan over blocks of G chain segments.  d_scratch holds
 // tail_scratch_bytes(n_seg, G); d_err[s] is set to 1 for a chain segment s
 // whose marker reaches before the start of its file's output (d_err holds
