@@ -307,7 +307,27 @@ static void gz_piece(void* c, size_t i) {
             }
             if (lo < g->nseg && g->seg_end[lo] < end) end = g->seg_end[lo];
             if (cur != lvl) {
-                if (deflateParams(&s, cur, Z_DEFAULT_STRATEGY) != Z_OK) ok = 0;
+                /* A new level: the stream so far ends in a sync flush and a new
+                 * one starts at the new level, primed with the 32 KiB before
+                 * (deflateParams of zlib 1.2.11 crashed inside deflate on some
+                 * of these inputs). */
+                s.next_in = 0;
+                s.avail_in = 0;
+                if (deflate(&s, Z_SYNC_FLUSH) == Z_STREAM_ERROR) ok = 0;
+                Bytef* const no = s.next_out;
+                const uInt na = s.avail_out;
+                deflateEnd(&s);
+                memset(&s, 0, sizeof s);
+                if (deflateInit2(&s, cur, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) {
+                    ok = 0;
+                    break;
+                }
+                if (p) {
+                    size_t dl = p < 32768 ? p : 32768;
+                    deflateSetDictionary(&s, g->in + p - dl, (uInt)dl);
+                }
+                s.next_out = no;
+                s.avail_out = na;
                 lvl = cur;
             }
         }

@@ -91,3 +91,16 @@ def test_member_pieces(decoder, capfd):
 def test_small_items_not_cut(decoder, capfd):
     raw = text(3 << 20, 21)
     check(decoder, gz(raw, 6), 1 << 20, raw, capfd=capfd, cut=False)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_mixed_seeds(decoder, seed):
+    # stored random stretches, runs, short-period repeats, distance-32768
+    # repeats and text at levels 1/9 in one member: cut pieces and long
+    # exact finder walks together
+    from rusty_rapidgzip_amd import synth
+
+    raw, lv, ends = synth.mixed(40 << 20, seed=seed, threads=8)
+    g = synth.gzip_segments(raw, lv, ends, threads=8, piece=16 << 20).tobytes()
+    _say("mixed seed", seed, len(g))
+    check(decoder, g, 1 << 20, raw.tobytes(), seg=32 << 10)
